@@ -1,0 +1,150 @@
+/*
+ * whhd.h -- C ABI of the MI355X-native wavelet potential-recovery library
+ * (libwhhd.so, HIP kernels for gfx950).
+ *
+ * The reference (yours321dog/WaveletPotentials) exposes this path as C++
+ * classes behind an empty CUWHHD_API macro (WaveletPotentials/cuWHHD_Export.h:
+ * 11-20).  Each entry point below replaces one of those methods; the comment
+ * on each names the reference interface (file:line).  Conventions:
+ *
+ *   - Every pointer argument is device memory owned by the caller, except
+ *     handle out-parameters and host-side result structs.
+ *   - Arrays are y-fastest: element (x, y, z) of an array with extents
+ *     (nx, ny, nz) is at y + x*ny + z*nx*ny (WaveletPotentials/Utils.h:46-47).
+ *     MAC staggering follows the reference: for nx*ny*nz cells,
+ *     xv (nx+1, ny, nz), yv (nx, ny+1, nz), zv (nx, ny, nz+1),
+ *     qx (nx, ny+1, nz+1), qy (nx+1, ny, nz+1), qz (nx+1, ny+1, nz).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls enqueue work on it and return without synchronizing unless the
+ *     comment says otherwise.
+ *   - Functions return a whhd_status; they never abort the process.  The
+ *     message of the last failure on the calling thread is whhd_last_error().
+ *   - Bulk arithmetic is FP32 (float*); entry points suffixed _f64 take double*.
+ */
+#ifndef WHHD_H
+#define WHHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WHHD_API __attribute__((visibility("default")))
+
+typedef enum whhd_status {
+    WHHD_OK = 0,
+    WHHD_INVALID_ARG = 1,
+    WHHD_NOT_POW2 = 2,             /* wavelet path needs 2^L (+1 on staggered axes) samples */
+    WHHD_UNSUPPORTED_WAVELET = 3,  /* (type, bc) has no kernel in the reference either */
+    WHHD_HIP_ERROR = 4,
+    WHHD_OOM = 5,
+    WHHD_NONCONVERGED = 6
+} whhd_status;
+
+/* WaveletType, same enumerator order as WaveletPotentials/WaveletTypes.h:7-28 */
+typedef enum whhd_wavelet {
+    WHHD_WT_CDF3_9 = 0, WHHD_WT_CDF3_11, WHHD_WT_CDF3_5, WHHD_WT_CDF2_10, WHHD_WT_CDF2_12,
+    WHHD_WT_CDF2_6, WHHD_WT_CDF4_4, WHHD_WT_CDF4_6, WHHD_WT_CDF4_8, WHHD_WT_CDF4_10,
+    WHHD_WT_CDF2_8, WHHD_WT_CDF3_7, WHHD_WT_CDF26_44, WHHD_WT_CDF28_46, WHHD_WT_CDF44_26,
+    WHHD_WT_CDF46_28, WHHD_WT_CDF2_0, WHHD_WT_CDF1_1, WHHD_WT_CDF1_7
+} whhd_wavelet;
+
+/* ---- library ---------------------------------------------------------- */
+WHHD_API const char* whhd_status_string(int status);
+WHHD_API const char* whhd_last_error(void);
+WHHD_API const char* whhd_version(void);
+/* proj_bc (WaveletPotentials/WaveletTypes.h:102-118) */
+WHHD_API char whhd_proj_bc(char bc);
+/* proj_WaveletType (WaveletPotentials/WaveletTypes.h:120-143); -1 if none */
+WHHD_API int whhd_proj_wavelet(int wavelet);
+/* 1 if (wavelet, bc) has a kernel (WaveletPotentials/cuWHHDForward.cu:744-1080) */
+WHHD_API int whhd_wavelet_supported(int wavelet, char bc);
+
+/* ---- 1-D line transforms ---------------------------------------------- */
+/* CuWHHDForward::Solve1D(dst, src, int3 dim, level, direction, bc, type, is_forward)
+ * (WaveletPotentials/cuWHHDForward.cuh:20-21, cuWHHDForward.cu:1122-1253):
+ * multi-level forward (forward=1) or inverse (0) transform of every line of
+ * the (nx, ny, nz) array along axis 'x', 'y' or 'z'.  dst may equal src.
+ * `level` must be floor(log2(line length)) and the length 2^level or
+ * 2^level + 1. */
+WHHD_API int whhd_transform1d(float* dst, const float* src, int nx, int ny, int nz, int level,
+                              char axis, char bc, int wavelet, int forward, void* stream);
+WHHD_API int whhd_transform1d_f64(double* dst, const double* src, int nx, int ny, int nz,
+                                  int level, char axis, char bc, int wavelet, int forward,
+                                  void* stream);
+
+/* CuGetInnerProduct::GenerateIp(type, bc, n) (WaveletPotentials/cuGetInnerProduct.cu:
+ * 92-126 with cuSelectIpPart :11-77): Gram table of the synthesis basis on n
+ * samples, written as 8 rows of n doubles to device memory `out` (row 0 the
+ * diagonal, rows 1-4 the +-1/+-2 off-diagonals, rows 5-7 the parent entries).
+ * Computed in FP64.  Synchronizes `stream`. */
+WHHD_API int whhd_gram_table(double* out, int wavelet, char bc, int n, void* stream);
+
+/* ---- per-cell projection in wavelet space ------------------------------ */
+typedef struct whhd_project3d_s* whhd_project3d_t;
+/* CuDivLocalProject3D(int3 dim, int3 levels, DTYPE3 dx, WaveletType type, char bc)
+ * (WaveletPotentials/cuDivLocalProject3D.cuh:11, .cu:43-72).  nx, ny, nz are
+ * the cell counts (powers of two; the reference passes dim+1), wavelet/bc the
+ * divergence ("base") type and its bc.  Only bc 'd' (velocity bc 'n') is
+ * implemented. */
+WHHD_API int whhd_project3d_create(int nx, int ny, int nz, double dx, double dy, double dz,
+                                   int wavelet_div, char bc_div, whhd_project3d_t* out);
+/* ProjectLocal_q_ccc(q..., v..., levels) and its 9-argument form
+ * (WaveletPotentials/cuDivLocalProject3D.cuh:17-20, .cu:5273-5308, 5966-6004).
+ * xv_out/yv_out/zv_out may be NULL; when given they receive s x q at the
+ * projected coefficients and are left untouched elsewhere. */
+WHHD_API int whhd_project3d_run(whhd_project3d_t h, float* qx, float* qy, float* qz,
+                                float* xv_out, float* yv_out, float* zv_out, const float* xv,
+                                const float* yv, const float* zv, void* stream);
+WHHD_API void whhd_project3d_destroy(whhd_project3d_t h);
+
+typedef struct whhd_project2d_s* whhd_project2d_t;
+/* CuDivLocalProject(int2 dim, int2 levels, DTYPE2 dx, WaveletType, char bc)
+ * (WaveletPotentials/cuDivLocalProject.cuh:10, .cu:26-42); nx, ny cell counts. */
+WHHD_API int whhd_project2d_create(int nx, int ny, double dx, double dy, int wavelet_div,
+                                   char bc_div, whhd_project2d_t* out);
+/* ProjectSingle(q, xv, yv) (WaveletPotentials/cuDivLocalProject.cu:2162-2198). */
+WHHD_API int whhd_project2d_run(whhd_project2d_t h, float* qz, const float* xv,
+                                const float* yv, void* stream);
+WHHD_API void whhd_project2d_destroy(whhd_project2d_t h);
+
+/* ---- potential recovery ------------------------------------------------ */
+typedef struct whhd_recover3d_s* whhd_recover3d_t;
+/* CuWaveletPotentialRecover3D(int3 dim, DTYPE3 dx, char bc, WaveletType curl,
+ * WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover3D.cuh:12,
+ * .cu:6-12).  Allocates the handle's scratch (three velocity-sized arrays). */
+WHHD_API int whhd_recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz,
+                                   char bc, int wavelet_curl, int wavelet_div,
+                                   whhd_recover3d_t* out);
+/* Solve(qx, qy, qz, xv, yv, zv) (cuWaveletPotentialRecover3D.cu:16-61). */
+WHHD_API int whhd_recover3d_solve(whhd_recover3d_t h, float* qx, float* qy, float* qz,
+                                  const float* xv, const float* yv, const float* zv,
+                                  void* stream);
+/* Solve(qx, qy, qz, xv_out, yv_out, zv_out, xv, yv, zv) (cuWaveletPotentialRecover3D.cu:
+ * 63-121): also returns the divergence-free projected velocity. */
+WHHD_API int whhd_recover3d_solve_proj(whhd_recover3d_t h, float* qx, float* qy, float* qz,
+                                       float* xv_out, float* yv_out, float* zv_out,
+                                       const float* xv, const float* yv, const float* zv,
+                                       void* stream);
+/* CutCoefficients(q_out..., q..., level) (cuWaveletPotentialRecover3D.cu:123-239). */
+WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out,
+                                float* qz_out, const float* qx, const float* qy,
+                                const float* qz, int level, void* stream);
+WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h);
+
+typedef struct whhd_recover2d_s* whhd_recover2d_t;
+/* CuWaveletPotentialRecover2D(int2 dim, DTYPE2 dx, char bc, WaveletType curl,
+ * WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover2D.cuh:12, .cu:6-12). */
+WHHD_API int whhd_recover2d_create(int nx, int ny, double dx, double dy, char bc,
+                                   int wavelet_curl, int wavelet_div, whhd_recover2d_t* out);
+/* Solve(qz, xv, yv) (WaveletPotentials/cuWaveletPotentialRecover2D.cu:16-37). */
+WHHD_API int whhd_recover2d_solve(whhd_recover2d_t h, float* qz, const float* xv,
+                                  const float* yv, void* stream);
+WHHD_API void whhd_recover2d_destroy(whhd_recover2d_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WHHD_H */

@@ -1,0 +1,202 @@
+"""numpy wrappers over liboracle.so (test infrastructure only; see __init__.py).
+
+Arrays follow the reference layout: a field of extents (nx, ny, nz) is a numpy
+array of shape (nz, nx, ny) (y fastest), WaveletPotentials/Utils.h:46-47.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+OK, INVALID, UNSUPPORTED, UNDEFINED = 0, -1, -2, -3
+
+# WaveletType order of WaveletPotentials/WaveletTypes.h:7-28
+WAVELETS = ["cdf3.9", "cdf3.11", "cdf3.5", "cdf2.10", "cdf2.12", "cdf2.6", "cdf4.4", "cdf4.6",
+            "cdf4.8", "cdf4.10", "cdf2.8", "cdf3.7", "cdf26.44", "cdf28.46", "cdf44.26",
+            "cdf46.28", "cdf2.0", "cdf1.1", "cdf1.7"]
+WT = {name: i for i, name in enumerate(WAVELETS)}
+
+
+def build():
+    """Compile liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def _setup(L):
+    i, c, d = ctypes.c_int, ctypes.c_char, ctypes.c_double
+    L.orc_transform.argtypes = [_D, _D, i, i, i, i, c, c, i, i]
+    L.orc_emulate_line.argtypes = [_D, _D, i, i, c, i, i]
+    L.orc_gram_table.argtypes = [_D, i, c, i]
+    L.orc_project3d.argtypes = [_D] * 9 + [i, i, i, d, d, d, i, c]
+    L.orc_project2d.argtypes = [_D, _D, _D, i, i, d, d, i, c]
+    L.orc_recover3d.argtypes = [_D] * 9 + [i, i, i, d, d, d, c, i, i]
+    L.orc_recover2d.argtypes = [_D, _D, _D, i, i, d, d, c, i, i]
+    L.orc_cut3d.argtypes = [_D] * 6 + [i, i, i, c, i, i, i]
+    L.orc_curl3d.argtypes = [_D] * 6 + [i, i, i, d, d, d]
+    L.orc_curl2d.argtypes = [_D] * 3 + [i, i, d, d]
+    L.orc_div3d.argtypes = [_D] * 4 + [i, i, i, d, d, d]
+    L.orc_curl3d.restype = None
+    L.orc_curl2d.restype = None
+    L.orc_div3d.restype = None
+    L.orc_wavelet_supported.argtypes = [i, c]
+    L.orc_wavelet_halo.argtypes = [i]
+    L.orc_proj_wavelet.argtypes = [i]
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def _wt(w):
+    return WT[w] if isinstance(w, str) else int(w)
+
+
+def _b(ch):
+    return ch.encode() if isinstance(ch, str) else ch
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise OracleError(f"{what}: oracle status {rc}")
+
+
+def field(nx, ny, nz=1):
+    return np.zeros((nz, nx, ny), dtype=np.float64)
+
+
+def transform(src, levels, axis, bc, wt, forward=True, allow_undefined=False):
+    """Solve1D on a (nz, nx, ny) float64 array; returns a new array."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    nz, nx, ny = src.shape
+    dst = np.empty_like(src)
+    rc = lib().orc_transform(_p(dst), _p(src), nx, ny, nz, levels, _b(axis), _b(bc), _wt(wt),
+                             int(forward))
+    if not (allow_undefined and rc == UNDEFINED):
+        _check(rc, "transform")
+    return dst, rc
+
+
+def emulate_line(src, levels, bc, wt, forward=True):
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.empty_like(src)
+    rc = lib().orc_emulate_line(_p(dst), _p(src), src.size, levels, _b(bc), _wt(wt),
+                                int(forward))
+    return dst, rc
+
+
+def gram_table(wt, bc, n):
+    out = np.zeros(8 * n, dtype=np.float64)
+    _check(lib().orc_gram_table(_p(out), _wt(wt), _b(bc), n), "gram_table")
+    return out.reshape(8, n)
+
+
+def project3d(xv, yv, zv, n, dx, wt_div="cdf4.6", bc_div="d", with_vout=False):
+    nx, ny, nz = n
+    qx, qy, qz = field(nx, ny + 1, nz + 1), field(nx + 1, ny, nz + 1), field(nx + 1, ny + 1, nz)
+    xo = yo = zo = None
+    if with_vout:
+        xo, yo, zo = xv.copy(), yv.copy(), zv.copy()
+    _check(lib().orc_project3d(_p(qx), _p(qy), _p(qz), _p(xo), _p(yo), _p(zo), _p(xv), _p(yv),
+                               _p(zv), nx, ny, nz, dx[0], dx[1], dx[2], _wt(wt_div), _b(bc_div)),
+           "project3d")
+    return (qx, qy, qz, xo, yo, zo) if with_vout else (qx, qy, qz)
+
+
+def project2d(xv, yv, n, dx, wt_div="cdf4.6", bc_div="d"):
+    nx, ny = n
+    qz = field(nx + 1, ny + 1)
+    _check(lib().orc_project2d(_p(qz), _p(xv), _p(yv), nx, ny, dx[0], dx[1], _wt(wt_div),
+                               _b(bc_div)), "project2d")
+    return qz
+
+
+def recover3d(xv, yv, zv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6", with_vout=False):
+    nx, ny, nz = n
+    qx, qy, qz = field(nx, ny + 1, nz + 1), field(nx + 1, ny, nz + 1), field(nx + 1, ny + 1, nz)
+    xo = yo = zo = None
+    if with_vout:
+        xo, yo, zo = np.empty_like(xv), np.empty_like(yv), np.empty_like(zv)
+    _check(lib().orc_recover3d(_p(qx), _p(qy), _p(qz), _p(xo), _p(yo), _p(zo), _p(xv), _p(yv),
+                               _p(zv), nx, ny, nz, dx[0], dx[1], dx[2], _b(bc), _wt(wt_curl),
+                               _wt(wt_div)), "recover3d")
+    return (qx, qy, qz, xo, yo, zo) if with_vout else (qx, qy, qz)
+
+
+def recover2d(xv, yv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6"):
+    nx, ny = n
+    qz = field(nx + 1, ny + 1)
+    _check(lib().orc_recover2d(_p(qz), _p(xv), _p(yv), nx, ny, dx[0], dx[1], _b(bc),
+                               _wt(wt_curl), _wt(wt_div)), "recover2d")
+    return qz
+
+
+def cut3d(qx, qy, qz, n, level, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6"):
+    nx, ny, nz = n
+    ox, oy, oz = np.empty_like(qx), np.empty_like(qy), np.empty_like(qz)
+    _check(lib().orc_cut3d(_p(ox), _p(oy), _p(oz), _p(qx), _p(qy), _p(qz), nx, ny, nz, _b(bc),
+                           _wt(wt_curl), _wt(wt_div), level), "cut3d")
+    return ox, oy, oz
+
+
+def curl3d(qx, qy, qz, n, dx):
+    nx, ny, nz = n
+    xv, yv, zv = field(nx + 1, ny, nz), field(nx, ny + 1, nz), field(nx, ny, nz + 1)
+    lib().orc_curl3d(_p(xv), _p(yv), _p(zv), _p(np.ascontiguousarray(qx)),
+                     _p(np.ascontiguousarray(qy)), _p(np.ascontiguousarray(qz)), nx, ny, nz,
+                     dx[0], dx[1], dx[2])
+    return xv, yv, zv
+
+
+def curl2d(qz, n, dx):
+    nx, ny = n
+    xv, yv = field(nx + 1, ny), field(nx, ny + 1)
+    lib().orc_curl2d(_p(xv), _p(yv), _p(np.ascontiguousarray(qz)), nx, ny, dx[0], dx[1])
+    return xv, yv
+
+
+def div3d(xv, yv, zv, n, dx):
+    nx, ny, nz = n
+    d = field(nx, ny, nz)
+    lib().orc_div3d(_p(d), _p(xv), _p(yv), _p(zv), nx, ny, nz, dx[0], dx[1], dx[2])
+    return d
+
+
+def wavelet_supported(wt, bc):
+    return bool(lib().orc_wavelet_supported(_wt(wt), _b(bc)))
+
+
+def halo(wt):
+    return lib().orc_wavelet_halo(_wt(wt))
+
+
+def rel_l2(a, b):
+    a = np.concatenate([np.ravel(x) for x in (a if isinstance(a, (list, tuple)) else [a])])
+    b = np.concatenate([np.ravel(x) for x in (b if isinstance(b, (list, tuple)) else [b])])
+    return float(np.sqrt(np.sum((a - b) ** 2) / (np.sum(a ** 2) + 1e-300)))

@@ -1,0 +1,1138 @@
+/*
+ * ORACLE -- test infrastructure only (see whhd_oracle.h).
+ *
+ * FP64 CPU restatement of the reference's wavelet potential-recovery path.
+ * Each function names the reference file:line it follows.  The per-pair
+ * lifting statements are written as whole-array passes over an extended pair
+ * sequence (pairs k in [-halo, W+halo)), which is what the reference's
+ * 32-lane shuffle windows compute for the pairs they write; orc_emulate_line()
+ * re-does the reference kernels lane by lane so tests can check the two agree.
+ */
+#include "whhd_oracle.h"
+#include "orc_tables.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EPS 1e-12 /* eps<DTYPE>, WaveletPotentials/Utils.h:52-53 */
+
+/* ------------------------------------------------------------------------- */
+/* Index/sign reflections (WaveletPotentials/cuWaveletCompute.cuh:180-374,    */
+/* 513-540 and 602-760).  Forward forms take a=2k, d=2k+1; inverse forms      */
+/* take a=d=k.  L is the level dimension.                                    */
+/* ------------------------------------------------------------------------- */
+typedef void (*idx_fn)(int* a, int* d, double* sa, double* sd, int L);
+
+static void ix_cdf35_n(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:180-201 */
+    if (*a < 0) *a = -1 - *a; else if (*a >= L) *a = 2 * L - 1 - *a;
+    if (*d < 0) *d = -1 - *d; else if (*d >= L) *d = 2 * L - 1 - *d;
+    *sa = 1.; *sd = 1.;
+}
+static void ix_cdf35_d(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:203-227 */
+    *sa = 1.; *sd = 1.;
+    if (*d < 0) { *d = -1 - *d; *sd = -1.; } else if (*d >= L) { *d = 2 * L - 1 - *d; *sd = -1.; }
+    if (*a < 0) { *a = -1 - *a; *sa = -1.; } else if (*a >= L) { *a = 2 * L - 1 - *a; *sa = -1.; }
+}
+static void ix_cdf26_n(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:256-270 (also cuWT_CDF4_4_n, :336-350) */
+    if (*a < 0) { *a = -*a; *d = -*d; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d; }
+    *sa = 1.; *sd = 1.;
+}
+static void ix_cdf26_d(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:272-294 */
+    if (*a < 0) *a = -*a; else if (*a > L) *a = 2 * L - *a;
+    *sa = 1.; *sd = 1.;
+    if (*d < 0) { *d = -*d; *sd = -1.; } else if (*d >= L) { *d = 2 * L - *d; *sd = -1.; }
+}
+static void ix_cdf26_z(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:296-310 */
+    if (*a < 0) { *a = -*a; *d = -*d; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d; }
+    *sa = (*a == 0 || *a == L) ? 2. : 1.;
+    *sd = 1.;
+}
+static void ix_cdf44_d(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:352-374 */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -*a; *d = -*d; *sd = -1.; *sa = -1.; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d; *sd = -1.; *sa = -1.; }
+    if (*a == L) *sa = 1.;
+}
+/* inverse-direction reflections */
+static void ix_cdf35_n_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:602-616 */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -1 - *a; *sd = -1.; } else if (*a >= L) { *a = 2 * L - 1 - *a; *sd = -1.; }
+    *d = *a;
+}
+static void ix_cdf35_d_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:617-631 */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -1 - *a; *sa = -1.; } else if (*a >= L) { *a = 2 * L - 1 - *a; *sa = -1.; }
+    *d = *a;
+}
+static void ix_cdf26_n_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:655-668 (also cuWT_CDF4_4_n_i, :716-729) */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -*a; *d = -*d - 1; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d - 1; }
+}
+static void ix_cdf26_d_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:669-684 */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -*a; *d = -*d - 1; *sd = -1.; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d - 1; *sd = -1.; }
+}
+static void ix_cdf26_z_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:685-698: the sign is taken before the reflection */
+    *sa = (*a == 0 || *a == L) ? 2. : 1.;
+    *sd = 1.;
+    if (*a < 0) { *a = -*a; *d = -*d - 1; }
+    else if (*a >= L) { *a = 2 * L - *a; *d = 2 * L - *d - 1; }
+}
+static void ix_cdf44_d_i(int* a, int* d, double* sa, double* sd, int L)
+{ /* cuWaveletCompute.cuh:730-751 */
+    *sa = 1.; *sd = 1.;
+    if (*a < 0) { *a = -*a; *d = -*d - 1; *sd = -1.; *sa = -1.; }
+    else if (*a >= L) { *d = 2 * L - *d - 1; *sd = -1.; }
+    if (*a > L) { *a = 2 * L - *a; *sa = -1.; }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Lifting updates on an extended pair sequence.  Each statement of the       */
+/* reference's per-lane update is one pass over all pairs; a neighbour        */
+/* outside [0, n) reads the pair's own value (shuffle semantics).            */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int n;
+    double *a, *d, *sa, *sd;
+} pairs_t;
+typedef void (*upd_fn)(pairs_t* p);
+
+#define UP(X, m) ((j) - (m) >= 0 ? (X)[(j) - (m)] : (X)[j])
+#define DN(X, m) ((j) + (m) < n ? (X)[(j) + (m)] : (X)[j])
+#define FORJ for (int j = 0; j < n; ++j)
+
+static void up_cdf35(pairs_t* p)
+{ /* cuUpdateCdf35_ad, cuWaveletCompute.cuh:229-253 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] = p->sa[j] * a[j] + WT_Q5_fu_1[0] * UP(d, 1);
+    FORJ d[j] += WT_Q5_fp[0] * DN(a, 1) + WT_Q5_fp[1] * a[j];
+    FORJ a[j] += WT_Q5_fu_2[4] * UP(d, 2);
+    FORJ a[j] += WT_Q5_fu_2[3] * UP(d, 1);
+    FORJ a[j] += WT_Q5_fu_2[2] * d[j];
+    FORJ a[j] += WT_Q5_fu_2[1] * DN(d, 1);
+    FORJ a[j] += WT_Q5_fu_2[0] * DN(d, 2);
+    FORJ { a[j] *= WT_Q5_scl[0]; d[j] *= WT_Q5_scl[1]; }
+}
+static void up_cdf35_i(pairs_t* p)
+{ /* cuUpdateCdf35_ad_i, cuWaveletCompute.cuh:632-653 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { a[j] = p->sa[j] * a[j] * WT_Q5_scl[1]; d[j] = p->sd[j] * d[j] * WT_Q5_scl[0]; }
+    FORJ a[j] -= WT_Q5_fu_2[4] * UP(d, 2);
+    FORJ a[j] -= WT_Q5_fu_2[3] * UP(d, 1);
+    FORJ a[j] -= WT_Q5_fu_2[2] * d[j];
+    FORJ a[j] -= WT_Q5_fu_2[1] * DN(d, 1);
+    FORJ a[j] -= WT_Q5_fu_2[0] * DN(d, 2);
+    FORJ d[j] -= WT_Q5_fp[1] * a[j] + WT_Q5_fp[0] * DN(a, 1);
+    FORJ a[j] -= WT_Q5_fu_1[0] * UP(d, 1);
+}
+static void up_cdf37(pairs_t* p)
+{ /* cuUpdateCdf37_ad, cuWaveletCompute.cuh:876-900 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] = p->sa[j] * a[j] + WT_Q5_fu_1[0] * UP(d, 1);
+    FORJ d[j] += WT_Q5_fp[0] * DN(a, 1) + WT_Q5_fp[1] * a[j];
+    FORJ a[j] += WT_Q7_fu_2[6] * UP(d, 3);
+    FORJ a[j] += WT_Q7_fu_2[5] * UP(d, 2);
+    FORJ a[j] += WT_Q7_fu_2[4] * UP(d, 1);
+    FORJ a[j] += WT_Q7_fu_2[3] * d[j];
+    FORJ a[j] += WT_Q7_fu_2[2] * DN(d, 1);
+    FORJ a[j] += WT_Q7_fu_2[1] * DN(d, 2);
+    FORJ a[j] += WT_Q7_fu_2[0] * DN(d, 3);
+    FORJ { a[j] *= WT_Q5_scl[0]; d[j] *= WT_Q5_scl[1]; }
+}
+static void up_cdf37_i(pairs_t* p)
+{ /* cuUpdateCdf37_ad_i, cuWaveletCompute.cuh:944-964 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { a[j] = p->sa[j] * a[j] * WT_Q5_scl[1]; d[j] = p->sd[j] * d[j] * WT_Q5_scl[0]; }
+    FORJ a[j] -= WT_Q7_fu_2[6] * UP(d, 3);
+    FORJ a[j] -= WT_Q7_fu_2[5] * UP(d, 2);
+    FORJ a[j] -= WT_Q7_fu_2[4] * UP(d, 1);
+    FORJ a[j] -= WT_Q7_fu_2[3] * d[j];
+    FORJ a[j] -= WT_Q7_fu_2[2] * DN(d, 1);
+    FORJ a[j] -= WT_Q7_fu_2[1] * DN(d, 2);
+    FORJ a[j] -= WT_Q7_fu_2[0] * DN(d, 3);
+    FORJ d[j] -= WT_Q5_fp[1] * a[j] + WT_Q5_fp[0] * DN(a, 1);
+    FORJ a[j] -= WT_Q5_fu_1[0] * UP(d, 1);
+}
+static void up_cdf26(pairs_t* p)
+{ /* cuUpdateCdf26_ad, cuWaveletCompute.cuh:312-334 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ a[j] *= p->sa[j];
+    FORJ d[j] += WT_L6_fp[0] * a[j] + WT_L6_fp[1] * DN(a, 1);
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] += WT_L6_fu[5] * UP(d, 3);
+    FORJ a[j] += WT_L6_fu[4] * UP(d, 2);
+    FORJ a[j] += WT_L6_fu[3] * UP(d, 1);
+    FORJ a[j] += WT_L6_fu[2] * d[j];
+    FORJ a[j] += WT_L6_fu[1] * DN(d, 1);
+    FORJ a[j] += WT_L6_fu[0] * DN(d, 2);
+    FORJ { a[j] *= WT_L6_scl[0]; d[j] *= WT_L6_scl[1]; }
+    FORJ a[j] /= p->sa[j];
+}
+static void up_cdf26_i(pairs_t* p)
+{ /* cuUpdateCdf26_ad_i, cuWaveletCompute.cuh:699-714 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = p->sd[j] * d[j] * WT_L6_scl[0]; a[j] = p->sa[j] * a[j] * WT_L6_scl[1]; }
+    FORJ a[j] -= WT_L6_fu[5] * UP(d, 3);
+    FORJ a[j] -= WT_L6_fu[4] * UP(d, 2);
+    FORJ a[j] -= WT_L6_fu[3] * UP(d, 1);
+    FORJ a[j] -= WT_L6_fu[2] * d[j];
+    FORJ a[j] -= WT_L6_fu[1] * DN(d, 1);
+    FORJ a[j] -= WT_L6_fu[0] * DN(d, 2);
+    FORJ d[j] -= WT_L6_fp[1] * a[j] + WT_L6_fp[0] * DN(a, 1);
+    FORJ a[j] /= p->sa[j];
+}
+static void up_cdf28(pairs_t* p)
+{ /* cuUpdateCdf28_ad, cuWaveletCompute.cuh:902-923 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ a[j] *= p->sa[j];
+    FORJ d[j] += WT_L6_fp[0] * a[j] + WT_L6_fp[1] * DN(a, 1);
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] += WT_L8_fu[7] * UP(d, 4);
+    FORJ a[j] += WT_L8_fu[6] * UP(d, 3);
+    FORJ a[j] += WT_L8_fu[5] * UP(d, 2);
+    FORJ a[j] += WT_L8_fu[4] * UP(d, 1);
+    FORJ a[j] += WT_L8_fu[3] * d[j];
+    FORJ a[j] += WT_L8_fu[2] * DN(d, 1);
+    FORJ a[j] += WT_L8_fu[1] * DN(d, 2);
+    FORJ a[j] += WT_L8_fu[0] * DN(d, 3);
+    FORJ { a[j] *= WT_L6_scl[0]; d[j] *= WT_L6_scl[1]; }
+    FORJ a[j] /= p->sa[j];
+}
+static void up_cdf28_i(pairs_t* p)
+{ /* cuUpdateCdf28_ad_i, cuWaveletCompute.cuh:966-981 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = p->sd[j] * d[j] * WT_L6_scl[0]; a[j] = p->sa[j] * a[j] * WT_L6_scl[1]; }
+    FORJ a[j] -= WT_L8_fu[7] * UP(d, 4);
+    FORJ a[j] -= WT_L8_fu[6] * UP(d, 3);
+    FORJ a[j] -= WT_L8_fu[5] * UP(d, 2);
+    FORJ a[j] -= WT_L8_fu[4] * UP(d, 1);
+    FORJ a[j] -= WT_L8_fu[3] * d[j];
+    FORJ a[j] -= WT_L8_fu[2] * DN(d, 1);
+    FORJ a[j] -= WT_L8_fu[1] * DN(d, 2);
+    FORJ a[j] -= WT_L8_fu[0] * DN(d, 3);
+    FORJ d[j] -= WT_L6_fp[1] * a[j] + WT_L6_fp[0] * DN(a, 1);
+    FORJ a[j] /= p->sa[j];
+}
+static void up_cdf44(pairs_t* p)
+{ /* cuUpdateCdf44_ad, cuWaveletCompute.cuh:376-391 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { a[j] *= p->sa[j]; d[j] *= p->sd[j]; }
+    FORJ a[j] += WT_C4_fu_1[1] * UP(d, 1) + WT_C4_fu_1[0] * d[j];
+    FORJ d[j] += WT_C4_fp[1] * a[j] + WT_C4_fp[0] * DN(a, 1);
+    FORJ a[j] += WT_C4_fu_2[3] * UP(d, 2);
+    FORJ a[j] += WT_C4_fu_2[2] * UP(d, 1);
+    FORJ a[j] += WT_C4_fu_2[1] * d[j];
+    FORJ a[j] += WT_C4_fu_2[0] * DN(d, 1);
+    FORJ { a[j] *= WT_C4_scl[0]; d[j] *= WT_C4_scl[1]; }
+}
+static void up_cdf44_i(pairs_t* p)
+{ /* cuUpdateCdf44_ad_i, cuWaveletCompute.cuh:752-770 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = p->sd[j] * d[j] * WT_C4_scl_inv[1]; a[j] = p->sa[j] * a[j] * WT_C4_scl_inv[0]; }
+    FORJ a[j] -= WT_C4_fu_2[3] * UP(d, 2);
+    FORJ a[j] -= WT_C4_fu_2[2] * UP(d, 1);
+    FORJ a[j] -= WT_C4_fu_2[1] * d[j];
+    FORJ a[j] -= WT_C4_fu_2[0] * DN(d, 1);
+    FORJ { a[j] *= p->sa[j]; d[j] *= p->sd[j]; }
+    FORJ d[j] -= WT_C4_fp[1] * a[j] + WT_C4_fp[0] * DN(a, 1);
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] -= WT_C4_fu_1[1] * UP(d, 1) + WT_C4_fu_1[0] * d[j];
+}
+static void up_cdf46(pairs_t* p)
+{ /* cuUpdateCdf46_ad, cuWaveletCompute.cuh:925-942 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { a[j] *= p->sa[j]; d[j] *= p->sd[j]; }
+    FORJ a[j] += WT_C4_fu_1[1] * UP(d, 1) + WT_C4_fu_1[0] * d[j];
+    FORJ d[j] += WT_C4_fp[1] * a[j] + WT_C4_fp[0] * DN(a, 1);
+    FORJ a[j] += WT_C6_fu_2[5] * UP(d, 3);
+    FORJ a[j] += WT_C6_fu_2[4] * UP(d, 2);
+    FORJ a[j] += WT_C6_fu_2[3] * UP(d, 1);
+    FORJ a[j] += WT_C6_fu_2[2] * d[j];
+    FORJ a[j] += WT_C6_fu_2[1] * DN(d, 1);
+    FORJ a[j] += WT_C6_fu_2[0] * DN(d, 2);
+    FORJ { a[j] *= WT_C4_scl[0]; d[j] *= WT_C4_scl[1]; }
+}
+static void up_cdf46_i(pairs_t* p)
+{ /* cuUpdateCdf46_ad_i, cuWaveletCompute.cuh:983-1002 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = p->sd[j] * d[j] * WT_C4_scl_inv[1]; a[j] = p->sa[j] * a[j] * WT_C4_scl_inv[0]; }
+    FORJ a[j] -= WT_C6_fu_2[5] * UP(d, 3);
+    FORJ a[j] -= WT_C6_fu_2[4] * UP(d, 2);
+    FORJ a[j] -= WT_C6_fu_2[3] * UP(d, 1);
+    FORJ a[j] -= WT_C6_fu_2[2] * d[j];
+    FORJ a[j] -= WT_C6_fu_2[1] * DN(d, 1);
+    FORJ a[j] -= WT_C6_fu_2[0] * DN(d, 2);
+    FORJ { a[j] *= p->sa[j]; d[j] *= p->sd[j]; }
+    FORJ d[j] -= WT_C4_fp[1] * a[j] + WT_C4_fp[0] * DN(a, 1);
+    FORJ d[j] *= p->sd[j];
+    FORJ a[j] -= WT_C4_fu_1[1] * UP(d, 1) + WT_C4_fu_1[0] * d[j];
+}
+static void up_cdf20(pairs_t* p)
+{ /* cuUpdateCdf20_ad, cuWaveletCompute.cuh:1413-1418 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ d[j] += WT_L6_fp[0] * a[j] + WT_L6_fp[1] * DN(a, 1);
+    FORJ { a[j] *= WT_L6_scl[0]; d[j] *= -WT_L6_scl[1]; }
+}
+static void up_cdf20_i(pairs_t* p)
+{ /* cuUpdateCdf20_ad_i, cuWaveletCompute.cuh:1425-1430 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = -p->sd[j] * d[j] * WT_L6_scl[0]; a[j] = p->sa[j] * a[j] * WT_L6_scl[1]; }
+    FORJ d[j] -= WT_L6_fp[1] * a[j] + WT_L6_fp[0] * DN(a, 1);
+}
+static void up_cdf11(pairs_t* p)
+{ /* cuUpdateCdf11_ad, cuWaveletCompute.cuh:1451-1458 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ a[j] *= p->sa[j];
+    FORJ d[j] = p->sd[j] * d[j] + WT_T1_fp[0] * a[j];
+    FORJ a[j] += WT_T1_fu[0] * d[j];
+    FORJ { a[j] *= WT_T1_scl[0]; d[j] *= WT_T1_scl[1]; }
+}
+static void up_cdf11_i(pairs_t* p)
+{ /* cuUpdateCdf11_ad_i, cuWaveletCompute.cuh:1465-1471: signs unused */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = d[j] * WT_T1_scl[0]; a[j] = a[j] * WT_T1_scl[1]; }
+    FORJ a[j] -= WT_T1_fu[0] * d[j];
+    FORJ d[j] -= WT_T1_fp[0] * a[j];
+}
+static void up_cdf17(pairs_t* p)
+{ /* cuUpdateCdf17_ad, cuWaveletCompute.cuh:1504-1518 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ a[j] *= p->sa[j];
+    FORJ d[j] = p->sd[j] * d[j] + WT_T1_fp[0] * a[j];
+    FORJ a[j] += WT_T7_fu[6] * UP(d, 3);
+    FORJ a[j] += WT_T7_fu[5] * UP(d, 2);
+    FORJ a[j] += WT_T7_fu[4] * UP(d, 1);
+    FORJ a[j] += WT_T7_fu[3] * d[j];
+    FORJ a[j] += WT_T7_fu[2] * DN(d, 1);
+    FORJ a[j] += WT_T7_fu[1] * DN(d, 2);
+    FORJ a[j] += WT_T7_fu[0] * DN(d, 3);
+    FORJ { a[j] *= WT_T7_scl[0]; d[j] *= WT_T7_scl[1]; }
+}
+static void up_cdf17_i(pairs_t* p)
+{ /* cuUpdateCdf17_ad_i, cuWaveletCompute.cuh:1529-1542 */
+    int n = p->n; double *a = p->a, *d = p->d;
+    FORJ { d[j] = p->sd[j] * d[j] / WT_T7_scl[1]; a[j] = p->sa[j] * a[j] / WT_T7_scl[0]; }
+    FORJ a[j] -= WT_T7_fu[6] * UP(d, 3);
+    FORJ a[j] -= WT_T7_fu[5] * UP(d, 2);
+    FORJ a[j] -= WT_T7_fu[4] * UP(d, 1);
+    FORJ a[j] -= WT_T7_fu[3] * d[j];
+    FORJ a[j] -= WT_T7_fu[2] * DN(d, 1);
+    FORJ a[j] -= WT_T7_fu[1] * DN(d, 2);
+    FORJ a[j] -= WT_T7_fu[0] * DN(d, 3);
+    FORJ d[j] -= WT_T1_fp[0] * a[j];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Registered (type, bc) table, WaveletPotentials/cuWHHDForward.cu:744-1101. */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    idx_fn ix, ix_i;
+    upd_fn up, up_i;
+    const double *m, *mi; /* L10 and inverse L10, m10 x m10 row-major */
+    int m10;              /* 4 or 5 */
+    int halo;
+} wdesc_t;
+
+static int desc_for(int wt, char bc, wdesc_t* w)
+{
+    memset(w, 0, sizeof(*w));
+    switch (wt) {
+    case ORC_CDF3_5:
+        w->up = up_cdf35; w->up_i = up_cdf35_i; w->m10 = 4; w->halo = 3;
+        if (bc == 'n') { w->ix = ix_cdf35_n; w->ix_i = ix_cdf35_n_i; w->m = WT_QN5_L10; w->mi = WT_iQN5_L10; }
+        else if (bc == 'd' || bc == 'z') { w->ix = ix_cdf35_d; w->ix_i = ix_cdf35_d_i; w->m = WT_QD5_L10; w->mi = WT_iQD5_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF3_7:
+        w->up = up_cdf37; w->up_i = up_cdf37_i; w->m10 = 4; w->halo = 4;
+        if (bc == 'n') { w->ix = ix_cdf35_n; w->ix_i = ix_cdf35_n_i; w->m = WT_QN7_L10; w->mi = WT_iQN7_L10; }
+        else if (bc == 'd' || bc == 'z') { w->ix = ix_cdf35_d; w->ix_i = ix_cdf35_d_i; w->m = WT_QD7_L10; w->mi = WT_iQD7_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF2_6:
+        w->up = up_cdf26; w->up_i = up_cdf26_i; w->m10 = 5; w->halo = 3;
+        if (bc == 'n') { w->ix = ix_cdf26_n; w->ix_i = ix_cdf26_n_i; w->m = WT_LN6_L10; w->mi = WT_iLN6_L10; }
+        else if (bc == 'd') { w->ix = ix_cdf26_d; w->ix_i = ix_cdf26_d_i; w->m = WT_LD6_L10; w->mi = WT_iLD6_L10; }
+        else if (bc == 'z') { w->ix = ix_cdf26_z; w->ix_i = ix_cdf26_z_i; w->m = WT_LZ6_L10; w->mi = WT_iLZ6_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF2_8:
+        w->up = up_cdf28; w->up_i = up_cdf28_i; w->m10 = 5; w->halo = 4;
+        if (bc == 'n') { w->ix = ix_cdf26_n; w->ix_i = ix_cdf26_n_i; w->m = WT_LN8_L10; w->mi = WT_iLN8_L10; }
+        else if (bc == 'd') { w->ix = ix_cdf26_d; w->ix_i = ix_cdf26_d_i; w->m = WT_LD8_L10; w->mi = WT_iLD8_L10; }
+        else if (bc == 'z') { w->ix = ix_cdf26_z; w->ix_i = ix_cdf26_z_i; w->m = WT_LZ8_L10; w->mi = WT_iLZ8_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF4_4:
+        w->up = up_cdf44; w->up_i = up_cdf44_i; w->m10 = 5; w->halo = 3;
+        if (bc == 'n' || bc == 'z') { w->ix = ix_cdf26_n; w->ix_i = ix_cdf26_n_i; w->m = WT_CN4_L10; w->mi = WT_iCN4_L10; }
+        else if (bc == 'd') { w->ix = ix_cdf44_d; w->ix_i = ix_cdf44_d_i; w->m = WT_CD4_L10; w->mi = WT_iCD4_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF4_6:
+        w->up = up_cdf46; w->up_i = up_cdf46_i; w->m10 = 5; w->halo = 4;
+        if (bc == 'n' || bc == 'z') { w->ix = ix_cdf26_n; w->ix_i = ix_cdf26_n_i; w->m = WT_CN6_L10; w->mi = WT_iCN6_L10; }
+        else if (bc == 'd') { w->ix = ix_cdf44_d; w->ix_i = ix_cdf44_d_i; w->m = WT_CD6_L10; w->mi = WT_iCD6_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF2_0:
+        w->up = up_cdf20; w->up_i = up_cdf20_i; w->m10 = 5; w->halo = 1; w->m = WT_20_L10; w->mi = WT_i20_L10;
+        if (bc == 'n') { w->ix = ix_cdf26_n; w->ix_i = ix_cdf26_n_i; }
+        else if (bc == 'd') { w->ix = ix_cdf26_d; w->ix_i = ix_cdf26_d_i; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF1_1:
+        w->up = up_cdf11; w->up_i = up_cdf11_i; w->m10 = 4; w->halo = 0; w->m = WT_11_L10; w->mi = WT_i11_L10;
+        if (bc == 'n') { w->ix = ix_cdf35_n; w->ix_i = ix_cdf35_n_i; }
+        else if (bc == 'd') { w->ix = ix_cdf35_d; w->ix_i = ix_cdf35_d_i; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    case ORC_CDF1_7:
+        w->up = up_cdf17; w->up_i = up_cdf17_i; w->m10 = 4; w->halo = 3;
+        if (bc == 'n') { w->ix = ix_cdf35_n; w->ix_i = ix_cdf35_n_i; w->m = WT_TN7_L10; w->mi = WT_iTN7_L10; }
+        else if (bc == 'd') { w->ix = ix_cdf35_d; w->ix_i = ix_cdf35_d_i; w->m = WT_TD7_L10; w->mi = WT_iTD7_L10; }
+        else return ORC_UNSUPPORTED;
+        return ORC_OK;
+    default:
+        return ORC_UNSUPPORTED;
+    }
+}
+
+int orc_wavelet_supported(int wt, char bc)
+{
+    wdesc_t w;
+    return desc_for(wt, bc, &w) == ORC_OK;
+}
+
+int orc_wavelet_halo(int wt)
+{ /* halo map, WaveletPotentials/cuWHHDForward.cu:1082-1101 */
+    switch (wt) {
+    case ORC_CDF1_7: case ORC_CDF2_6: case ORC_CDF4_4: case ORC_CDF3_5: return 3;
+    case ORC_CDF3_7: case ORC_CDF4_6: case ORC_CDF2_8: return 4;
+    case ORC_CDF3_11: case ORC_CDF4_10: case ORC_CDF2_12: return 6;
+    case ORC_CDF26_44: case ORC_CDF44_26: return 3;
+    case ORC_CDF28_46: case ORC_CDF46_28: return 4;
+    case ORC_CDF2_0: return 1;
+    case ORC_CDF1_1: return 0;
+    default: return -1;
+    }
+}
+
+char orc_proj_bc(char bc)
+{ /* WaveletTypes.h:102-118 */
+    if (bc == 'n') return 'd';
+    if (bc == 'd') return 'n';
+    return bc;
+}
+
+int orc_proj_wavelet(int wt)
+{ /* WaveletTypes.h:120-143 */
+    switch (wt) {
+    case ORC_CDF3_9: return ORC_CDF2_10;
+    case ORC_CDF3_11: return ORC_CDF2_12;
+    case ORC_CDF3_5: return ORC_CDF2_6;
+    case ORC_CDF4_4: return ORC_CDF3_5;
+    case ORC_CDF4_6: return ORC_CDF3_7;
+    case ORC_CDF3_7: return ORC_CDF2_8;
+    case ORC_CDF4_8: return ORC_CDF3_9;
+    case ORC_CDF4_10: return ORC_CDF3_11;
+    case ORC_CDF2_0: return ORC_CDF1_1;
+    default: return -1;
+    }
+}
+
+static int ilog2(int n)
+{
+    int l = 0;
+    while ((2 << l) <= n) ++l;
+    return l;
+}
+
+/* ------------------------------------------------------------------------- */
+/* One line, clean restatement                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Forward multi-level transform of one line of length n, following
+ * cuWHHDForward_y_ml (WaveletPotentials/cuWHHDForward.cu:5-105): levels
+ * l = levels-1 .. 2 lift pairs of the first 2^(l+1)+ext samples, the
+ * approximations stay in front, the details of level l go to
+ * [2^l+ext, 2^(l+1)+ext), and the last 4+ext approximations get the dense
+ * L10 block.  A read outside the valid samples of a level marks the pair NaN
+ * (the reference reads stale shared memory there). */
+static int fwd_line(const wdesc_t* w, const double* in, double* out, int n, int levels)
+{
+    const int ext = n & 1, H = w->halo;
+    double* s = (double*)malloc(sizeof(double) * n);
+    double* buf = (double*)malloc(sizeof(double) * 4 * (n + 2 * H + 4));
+    memcpy(s, in, sizeof(double) * n);
+    for (int i = 0; i < n; ++i) out[i] = NAN;
+    int valid = n;
+    for (int l = levels - 1; l > 1; --l) {
+        const int L = 1 << (l + 1), W = L / 2 + ext, np = W + 2 * H;
+        pairs_t p = { np, buf, buf + np, buf + 2 * np, buf + 3 * np };
+        for (int j = 0; j < np; ++j) {
+            int k = j - H, ai = 2 * k, di = 2 * k + 1;
+            w->ix(&ai, &di, &p.sa[j], &p.sd[j], L);
+            if (di < 0) di = 0;
+            p.a[j] = (ai >= 0 && ai < valid) ? s[ai] : NAN;
+            p.d[j] = (di >= 0 && di < valid) ? s[di] : NAN;
+        }
+        w->up(&p);
+        for (int k = 0; k < W; ++k) {
+            s[k] = p.a[k + H];
+            if (k < W - ext) out[W + k] = p.d[k + H];
+        }
+        valid = W;
+    }
+    const int m = w->m10;
+    if (m > n) { free(s); free(buf); return ORC_INVALID; }
+    for (int r = 0; r < m; ++r) {
+        double acc = 0.;
+        for (int c = 0; c < m; ++c) acc += s[c] * w->m[r * m + c];
+        out[r] = acc;
+    }
+    free(s);
+    free(buf);
+    for (int i = 0; i < n; ++i)
+        if (isnan(out[i])) return ORC_UNDEFINED;
+    return ORC_OK;
+}
+
+/* Inverse transform of one line, following cuWHHDBackward_y_ml
+ * (WaveletPotentials/cuWHHDForward.cu:349-448): inverse L10 first, then
+ * levels l = 2 .. levels-1 with approximations [0, 2^l+ext) and details
+ * [2^l+ext, 2^(l+1)+ext); pairs beyond 2^l + halo are never loaded there. */
+static int inv_line(const wdesc_t* w, const double* in, double* out, int n, int levels)
+{
+    const int ext = n & 1, H = w->halo, m = w->m10;
+    if (m > n) return ORC_INVALID;
+    double* cur = (double*)malloc(sizeof(double) * n);
+    double* nxt = (double*)malloc(sizeof(double) * n);
+    double* buf = (double*)malloc(sizeof(double) * 4 * (n + 2 * H + 4));
+    for (int i = 0; i < n; ++i) cur[i] = NAN;
+    for (int r = 0; r < m; ++r) {
+        double acc = 0.;
+        for (int c = 0; c < m; ++c) acc += in[c] * w->mi[r * m + c];
+        cur[r] = acc;
+    }
+    for (int l = 2; l < levels; ++l) {
+        const int L = 1 << l, W = L + ext, np = W + 2 * H;
+        pairs_t p = { np, buf, buf + np, buf + 2 * np, buf + 3 * np };
+        for (int j = 0; j < np; ++j) {
+            int k = j - H, ai = k, di = k;
+            w->ix_i(&ai, &di, &p.sa[j], &p.sd[j], L);
+            if (k < L + H) {
+                p.a[j] = (ai >= 0 && ai < W) ? cur[ai] : NAN;
+                p.d[j] = (di >= 0 && di < L) ? in[L + ext + di] : NAN;
+            } else {
+                p.a[j] = NAN;
+                p.d[j] = NAN;
+            }
+        }
+        w->up_i(&p);
+        for (int i = 0; i < n; ++i) nxt[i] = NAN;
+        for (int k = 0; k < W; ++k) {
+            nxt[2 * k] = p.a[k + H];
+            if (k < L) nxt[2 * k + 1] = p.d[k + H];
+        }
+        double* t = cur; cur = nxt; nxt = t;
+    }
+    memcpy(out, cur, sizeof(double) * n);
+    free(cur);
+    free(nxt);
+    free(buf);
+    for (int i = 0; i < n; ++i)
+        if (isnan(out[i])) return ORC_UNDEFINED;
+    return ORC_OK;
+}
+
+int orc_transform(double* dst, const double* src, int nx, int ny, int nz, int levels,
+                  char axis, char bc, int wt, int forward)
+{ /* CuWHHDForward::Solve1D, WaveletPotentials/cuWHHDForward.cu:1122-1253 */
+    wdesc_t w;
+    if (desc_for(wt, bc, &w) != ORC_OK) return ORC_UNSUPPORTED;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return ORC_INVALID;
+    int n, stride, nlines;
+    if (axis == 'y' || axis == 'Y') { n = ny; stride = 1; }
+    else if (axis == 'x' || axis == 'X') { n = nx; stride = ny; }
+    else if (axis == 'z' || axis == 'Z') { n = nz; stride = nx * ny; }
+    else return ORC_INVALID;
+    nlines = nx * ny * nz / n;
+    if (levels != ilog2(n) || ((n >> levels) << levels) + (n & 1) != n || levels < 2)
+        return ORC_INVALID;
+    int rc = ORC_OK;
+#pragma omp parallel
+    {
+        double* lin = (double*)malloc(sizeof(double) * n);
+        double* lout = (double*)malloc(sizeof(double) * n);
+        int rc_local = ORC_OK;
+#pragma omp for schedule(static)
+        for (int li = 0; li < nlines; ++li) {
+            size_t base;
+            if (stride == 1) base = (size_t)li * n;
+            else if (stride == ny) { int y = li % ny, z = li / ny; base = y + (size_t)z * nx * ny; }
+            else base = li; /* z lines: li over the (y, x) slice */
+            for (int i = 0; i < n; ++i) lin[i] = src[base + (size_t)i * stride];
+            int r = forward ? fwd_line(&w, lin, lout, n, levels) : inv_line(&w, lin, lout, n, levels);
+            if (r != ORC_OK) rc_local = r;
+            for (int i = 0; i < n; ++i) dst[base + (size_t)i * stride] = lout[i];
+        }
+        free(lin);
+        free(lout);
+#pragma omp critical
+        if (rc_local != ORC_OK) rc = rc_local;
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Literal lane-level emulation of cuWHHDForward_y_ml / cuWHHDBackward_y_ml   */
+/* for one line (threadIdx.y == 0).  Lanes that the kernel leaves unloaded    */
+/* keep their previous register contents; never-written registers and shared */
+/* memory start as NaN.                                                       */
+/* ------------------------------------------------------------------------- */
+#define NL 32
+static void shfl_dn(double* v, int m) { double t[NL]; for (int i = 0; i < NL; ++i) t[i] = i + m < NL ? v[i + m] : v[i]; memcpy(v, t, sizeof t); }
+
+/* Run a pairs_t update on 32 lanes: neighbours outside the warp read the lane's
+ * own value, exactly what the pair-sequence form does at its ends. */
+static void lanes_update(upd_fn f, double* a, double* d, double* sa, double* sd)
+{
+    pairs_t p = { NL, a, d, sa, sd };
+    f(&p);
+}
+
+int orc_emulate_line(double* dst, const double* src, int n, int levels, char bc, int wt,
+                     int forward)
+{
+    wdesc_t w;
+    if (desc_for(wt, bc, &w) != ORC_OK) return ORC_UNSUPPORTED;
+    const int lh = w.halo, rh = w.halo, wl = NL - lh - rh, ext = n & 1, m10 = w.m10;
+    /* sm holds one line (dim.y entries) followed by sm_res; reads past the line
+     * would land in sm_res, reads before it are out of the allocation */
+    double* sm = (double*)malloc(sizeof(double) * 2 * n);
+    for (int i = 0; i < 2 * n; ++i) sm[i] = NAN;
+    double* sm_res = sm + n;
+    double a[NL], d[NL], sa[NL], sd[NL];
+    for (int t = 0; t < NL; ++t) { a[t] = d[t] = NAN; sa[t] = sd[t] = 1.; }
+    for (int i = 0; i < n; ++i) sm[i] = src[i];
+#define SMREAD(base, idx) (((idx) >= 0 && (idx) < n) ? (base)[idx] : NAN)
+    if (forward) {
+        if (levels > 2) {
+            for (int l = levels - 1; l > 1; --l) {
+                int L = 1 << (l + 1), W = (L >> 1) + ext;
+                int patches = (int)ceil((double)W / wl);
+                for (int i = 0; i < patches; ++i) {
+                    int ai_s[NL], k_s[NL];
+                    for (int t = 0; t < NL; ++t) {
+                        int k = i * wl + t - lh, ai = 2 * k, di = 2 * k + 1;
+                        w.ix(&ai, &di, &sa[t], &sd[t], L);
+                        if (di < 0) di = 0;
+                        if (k < W + rh) { d[t] = SMREAD(sm, di); a[t] = SMREAD(sm, ai); }
+                        ai_s[t] = ai; k_s[t] = k;
+                    }
+                    (void)ai_s;
+                    lanes_update(w.up, a, d, sa, sd);
+                    for (int t = 0; t < NL; ++t) {
+                        int k = k_s[t];
+                        if (t - lh >= 0 && t - lh < wl && k < W) {
+                            sm[k] = a[t];
+                            if (k < W - ext) sm_res[k + W] = d[t];
+                        }
+                    }
+                }
+            }
+            shfl_dn(a, lh);
+        } else {
+            for (int t = 0; t < NL; ++t)
+                if (t < 4 + ext) a[t] = sm[t];
+        }
+        double res[NL];
+        for (int t = 0; t < NL; ++t) {
+            int rc = t < m10 ? t : 0;
+            double acc = 0.;
+            for (int c = 0; c < m10; ++c) acc += a[c] * w.m[rc * m10 + c];
+            res[t] = acc;
+        }
+        for (int t = 0; t < m10; ++t) sm_res[t] = res[t];
+        for (int i = 0; i < n; ++i) dst[i] = sm_res[i];
+    } else {
+        double* sm_dst = sm_res;
+        double* sm_src = sm;
+        if (levels & 1) { sm_dst = sm; sm_src = sm_res; }
+        for (int t = 0; t < NL; ++t)
+            if (t < m10) a[t] = sm[t];
+        double res[NL];
+        for (int t = 0; t < NL; ++t) {
+            int rc = t < m10 ? t : 0;
+            double acc = 0.;
+            for (int c = 0; c < m10; ++c) acc += a[c] * w.mi[rc * m10 + c];
+            res[t] = acc;
+        }
+        for (int t = 0; t < m10; ++t) sm_dst[t] = res[t];
+        for (int l = 2; l < levels; ++l) {
+            int L = 1 << l, W = L + ext;
+            int patches = (int)ceil((double)W / wl);
+            double* tmp = sm_dst; sm_dst = sm_src; sm_src = tmp;
+            for (int i = 0; i < patches; ++i) {
+                int k_s[NL];
+                for (int t = 0; t < NL; ++t) {
+                    int k = i * wl + t - lh, ai = k, di = k;
+                    sa[t] = 1.; sd[t] = 1.;
+                    w.ix_i(&ai, &di, &sa[t], &sd[t], L);
+                    if (k < L + rh) {
+                        d[t] = SMREAD(sm, di + L + ext);
+                        a[t] = SMREAD(sm_src, ai);
+                    }
+                    k_s[t] = k;
+                }
+                lanes_update(w.up_i, a, d, sa, sd);
+                for (int t = 0; t < NL; ++t) {
+                    int k = k_s[t];
+                    if (t - lh >= 0 && t - rh < wl) {
+                        if (k < L + ext) sm_dst[2 * k] = a[t];
+                        if (k < L) sm_dst[2 * k + 1] = d[t];
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) dst[i] = sm_dst[i];
+    }
+#undef SMREAD
+    free(sm);
+    for (int i = 0; i < n; ++i)
+        if (isnan(dst[i])) return ORC_UNDEFINED;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gram tables, CuGetInnerProduct::GenerateIp + cuSelectIpPart                */
+/* (WaveletPotentials/cuGetInnerProduct.cu:11-126), in exact FP64.            */
+/* ------------------------------------------------------------------------- */
+int orc_gram_table(double* out, int wt, char bc, int n)
+{
+    wdesc_t w;
+    if (desc_for(wt, bc, &w) != ORC_OK) return ORC_UNSUPPORTED;
+    int levels = ilog2(n);
+    double* basis = (double*)malloc(sizeof(double) * (size_t)n * n);
+    double* e = (double*)calloc(n, sizeof(double));
+    int rc = ORC_OK;
+    for (int i = 0; i < n; ++i) {
+        e[i] = 1.;
+        int r = inv_line(&w, e, basis + (size_t)i * n, n, levels);
+        if (r != ORC_OK) rc = r;
+        e[i] = 0.;
+    }
+    free(e);
+    const int ext = n & 1;
+    for (int i = 0; i < 8 * n; ++i) out[i] = 0.;
+    for (int i = 0; i < n; ++i) {
+        double g;
+        /* G[i][c] = <psi_i, psi_c> */
+#define GRAM(c) ({ double s_ = 0.; for (int t_ = 0; t_ < n; ++t_) s_ += basis[(size_t)i * n + t_] * basis[(size_t)(c) * n + t_]; s_; })
+        out[i] = GRAM(i);
+        out[n + i] = i > 1 ? GRAM(i - 2) : 0.;
+        out[2 * n + i] = i > 0 ? GRAM(i - 1) : 0.;
+        out[3 * n + i] = i < n - 1 ? GRAM(i + 1) : 0.;
+        out[4 * n + i] = i < n - 2 ? GRAM(i + 2) : 0.;
+        int nei = ((i - ext) >> 1) + ext;
+        if (nei > 0) out[5 * n + i] = GRAM(nei - 1);
+        out[6 * n + i] = GRAM(nei);
+        g = (nei + 1 < n) ? GRAM(nei + 1) : 0.;
+        out[7 * n + i] = g;
+#undef GRAM
+    }
+    free(basis);
+    return rc;
+}
+
+static int gram_diag(double* out, int wt, char bc, int n)
+{
+    double* t = (double*)malloc(sizeof(double) * 8 * n);
+    int rc = orc_gram_table(t, wt, bc, n);
+    memcpy(out, t, sizeof(double) * n);
+    free(t);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 3D projection: cuProjectLocal_q_3d_dq_ccc                                  */
+/* (WaveletPotentials/cuDivLocalProject3D.cu:4013-4246; 9-argument form with  */
+/* v_out = s x q at :5717-5965; launcher :5273-5308, ctor :43-72).            */
+/* ------------------------------------------------------------------------- */
+static int bitlen(int v) { int b = 0; while (v) { ++b; v >>= 1; } return b; }
+
+int orc_project3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, double* zvo,
+                  const double* xv, const double* yv, const double* zv, int NX, int NY, int NZ,
+                  double dx, double dy, double dz, int wt_div, char bc_div)
+{
+    if (bc_div != 'd') return ORC_UNSUPPORTED; /* the 'n' (nq_ccc) kernel is not restated */
+    const int Lx = ilog2(NX), Ly = ilog2(NY), Lz = ilog2(NZ);
+    if ((1 << Lx) != NX || (1 << Ly) != NY || (1 << Lz) != NZ) return ORC_INVALID;
+    int wt_proj = orc_proj_wavelet(wt_div);
+    char bc_proj = orc_proj_bc(bc_div);
+    double *ipbx = malloc(sizeof(double) * (NX + 1)), *ipby = malloc(sizeof(double) * (NY + 1)),
+           *ipbz = malloc(sizeof(double) * (NZ + 1)), *ippx = malloc(sizeof(double) * NX),
+           *ippy = malloc(sizeof(double) * NY), *ippz = malloc(sizeof(double) * NZ);
+    int rc = ORC_OK, r;
+    if ((r = gram_diag(ipbx, wt_div, bc_div, NX + 1))) rc = r;
+    if ((r = gram_diag(ipby, wt_div, bc_div, NY + 1))) rc = r;
+    if ((r = gram_diag(ipbz, wt_div, bc_div, NZ + 1))) rc = r;
+    if ((r = gram_diag(ippx, wt_proj, bc_proj, NX))) rc = r;
+    if ((r = gram_diag(ippy, wt_proj, bc_proj, NY))) rc = r;
+    if ((r = gram_diag(ippz, wt_proj, bc_proj, NZ))) rc = r;
+    const double ix = 1. / dx, iy = 1. / dy, iz = 1. / dz;
+    const long sxv_x = NY, sxv_z = (long)(NX + 1) * NY;       /* xv: (NX+1, NY, NZ) */
+    const long syv_x = NY + 1, syv_z = (long)NX * (NY + 1);   /* yv: (NX, NY+1, NZ) */
+    const long szv_x = NY, szv_z = (long)NX * NY;             /* zv: (NX, NY, NZ+1) */
+    const long sqx_x = NY + 1, sqx_z = (long)NX * (NY + 1);   /* qx: (NX, NY+1, NZ+1) */
+    const long sqy_x = NY, sqy_z = (long)(NX + 1) * NY;       /* qy: (NX+1, NY, NZ+1) */
+    const long sqz_x = NY + 1, sqz_z = (long)(NX + 1) * (NY + 1); /* qz: (NX+1, NY+1, NZ) */
+    for (int z = 0; z < NZ; ++z)
+        for (int x = 0; x < NX; ++x)
+            for (int y = 0; y < NY; ++y) {
+                long ixv = y + (x + 1) * sxv_x + z * sxv_z;
+                long iyv = (y + 1) + x * syv_x + z * syv_z;
+                long izv = y + x * szv_x + (z + 1) * szv_z;
+                long iqx = (y + 1) + x * sqx_x + (z + 1) * sqx_z;
+                long iqy = y + (x + 1) * sqy_x + (z + 1) * sqy_z;
+                long iqz = (y + 1) + (x + 1) * sqz_x + z * sqz_z;
+                int jy = Ly - bitlen(y) + 1, jx = Lx - bitlen(x) + 1, jz = Lz - bitlen(z) + 1;
+                double sy = 4. * iy / ((double)(1 << jy) + EPS);
+                double sx = 4. * ix / ((double)(1 << jx) + EPS);
+                double sz = 4. * iz / ((double)(1 << jz) + EPS);
+                double ip0x = ippx[x], ip1x = ipbx[x + 1];
+                double ip0y = ippy[y], ip1y = ipby[y + 1];
+                double ip0z = ippz[z], ip1z = ipbz[z + 1];
+                double vx = xv[ixv], vy = yv[iyv], vz = zv[izv];
+                double sxBxBy = sx * ip0x * ip1y * ip0z;
+                double syBxBy = sy * ip1x * ip0y * ip0z;
+                double sxBxBz = sx * ip0x * ip0y * ip1z;
+                double szBxBz = sz * ip1x * ip0y * ip0z;
+                double syByBz = sy * ip0x * ip0y * ip1z;
+                double szByBz = sz * ip0x * ip1y * ip0z;
+                double mx = 1. / (sy * syByBz + sz * szByBz + EPS);
+                double my = 1. / (sx * sxBxBz + sz * szBxBz + EPS);
+                double mz = 1. / (sx * sxBxBy + sy * syBxBy + EPS);
+                double fx = vy * szByBz - vz * syByBz;
+                double fy = vz * sxBxBz - vx * szBxBz;
+                double fz = vx * syBxBy - vy * sxBxBy;
+                double lx = 0., ly = 0., lz = 0.;
+                if (x > 0 && y > 0 && z > 0) {
+                    double inv = 1. / (sy * sy + sx * sx + sz * sz);
+                    lz = (sy * vx - sx * vy) * inv;
+                    lx = (sz * vy - sy * vz) * inv;
+                    ly = (sx * vz - sz * vx) * inv;
+                } else if (y > 0 && z > 0 && x == 0) {
+                    lx = fx * mx;
+                    qz[iqz - sqz_x] = 0.;
+                    qy[iqy - sqy_x] = 0.;
+                } else if (x > 0 && z > 0 && y == 0) {
+                    ly = fy * my;
+                    qx[iqx - 1] = 0.;
+                    qz[iqz - 1] = 0.;
+                } else if (x > 0 && y > 0 && z == 0) {
+                    lz = fz * mz;
+                    qx[iqx - sqx_z] = 0.;
+                    qy[iqy - sqy_z] = 0.;
+                } else if (y == 0 && x > 0 && z == 0) {
+                    qy[iqy - sqy_z] = 0.;
+                    qz[iqz - 1] = 0.;
+                    qx[iqx - 1] = 0.;
+                    qx[iqx - sqx_z - 1] = 0.;
+                    qx[iqx - sqx_z] = 0.;
+                } else if (y > 0 && x == 0 && z == 0) {
+                    qx[iqx - sqx_z] = 0.;
+                    qz[iqz - sqz_x] = 0.;
+                    qy[iqy - sqy_x - sqy_z] = 0.;
+                    qy[iqy - sqy_z] = 0.;
+                    qy[iqy - sqy_x] = 0.;
+                } else if (y == 0 && x == 0 && z > 0) {
+                    qx[iqx - 1] = 0.;
+                    qy[iqy - sqy_x] = 0.;
+                    qz[iqz - sqz_x - 1] = 0.;
+                    qz[iqz - sqz_x] = 0.;
+                    qz[iqz - 1] = 0.;
+                } else {
+                    qz[iqz - 1 - sqz_x] = 0.;
+                    qz[iqz - 1] = 0.;
+                    qz[iqz - sqz_x] = 0.;
+                    qx[iqx - 1 - sqx_z] = 0.;
+                    qx[iqx - sqx_z] = 0.;
+                    qx[iqx - 1] = 0.;
+                    qy[iqy - sqy_x - sqy_z] = 0.;
+                    qy[iqy - sqy_z] = 0.;
+                    qy[iqy - sqy_x] = 0.;
+                }
+                qx[iqx] = lx;
+                qy[iqy] = ly;
+                qz[iqz] = lz;
+                if (xvo) {
+                    double oy = sz * lx, oz = -sy * lx, ox = -sz * ly;
+                    oz += sx * ly;
+                    ox += sy * lz;
+                    oy += -sx * lz;
+                    xvo[ixv] = ox;
+                    yvo[iyv] = oy;
+                    zvo[izv] = oz;
+                }
+            }
+    free(ipbx); free(ipby); free(ipbz); free(ippx); free(ippy); free(ippz);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 2D projection: cuProjectLocal_q_single (bc_base 'd')                       */
+/* (WaveletPotentials/cuDivLocalProject.cu:765-1045, launcher :2162-2198).    */
+/* The kernel evaluates the same node formula once per tile orientation and   */
+/* adds the halves; the neighbour terms it computes are not added.            */
+/* ------------------------------------------------------------------------- */
+int orc_project2d(double* qz, const double* xv, const double* yv, int NX, int NY, double dx,
+                  double dy, int wt_div, char bc_div)
+{
+    if (bc_div != 'd') return ORC_UNSUPPORTED; /* _single_n is not restated */
+    const int Lx = ilog2(NX), Ly = ilog2(NY);
+    if ((1 << Lx) != NX || (1 << Ly) != NY) return ORC_INVALID;
+    int wt_proj = orc_proj_wavelet(wt_div);
+    char bc_proj = orc_proj_bc(bc_div);
+    double *ipbx = malloc(sizeof(double) * (NX + 1)), *ipby = malloc(sizeof(double) * (NY + 1)),
+           *ippx = malloc(sizeof(double) * NX), *ippy = malloc(sizeof(double) * NY);
+    int rc = ORC_OK, r;
+    if ((r = gram_diag(ipbx, wt_div, bc_div, NX + 1))) rc = r;
+    if ((r = gram_diag(ipby, wt_div, bc_div, NY + 1))) rc = r;
+    if ((r = gram_diag(ippx, wt_proj, bc_proj, NX))) rc = r;
+    if ((r = gram_diag(ippy, wt_proj, bc_proj, NY))) rc = r;
+    const double ix = 1. / dx, iy = 1. / dy;
+    for (int X = 0; X <= NX; ++X)
+        for (int Y = 0; Y <= NY; ++Y) {
+            double q = 0.;
+            if (X > 1 && Y > 1) {
+                double vx = xv[(Y - 1) + (long)X * NY];       /* xv: (NX+1, NY) */
+                double vy = yv[Y + (long)(X - 1) * (NY + 1)]; /* yv: (NX, NY+1) */
+                int jy = Ly - bitlen(Y - 1) + 1, jx = Lx - bitlen(X - 1) + 1;
+                double sy = 4. * iy / ((double)(1 << jy) + EPS);
+                double sx = 4. * ix / ((double)(1 << jx) + EPS);
+                double ip0x = ippx[X - 1], ip1x = ipbx[X], ip0y = ippy[Y - 1], ip1y = ipby[Y];
+                double mid = sx * sx * ip0x * ip1y + sy * sy * ip1x * ip0y + EPS;
+                q = (vx * sy * ip1x * ip0y - vy * sx * ip0x * ip1y) / mid;
+            }
+            qz[Y + (long)X * (NY + 1)] = q;
+        }
+    free(ipbx); free(ipby); free(ippx); free(ippy);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Recovery drivers                                                            */
+/* ------------------------------------------------------------------------- */
+#define CHK(e) do { int r_ = (e); if (r_ != ORC_OK && rc == ORC_OK) rc = r_; } while (0)
+
+int orc_recover3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, double* zvo,
+                  const double* xv, const double* yv, const double* zv, int NX, int NY, int NZ,
+                  double dx, double dy, double dz, char bc, int wtc, int wtd)
+{ /* CuWaveletPotentialRecover3D::Solve, WaveletPotentials/cuWaveletPotentialRecover3D.cu:16-121 */
+    const char bd = orc_proj_bc(bc);
+    const int lx = ilog2(NX), ly = ilog2(NY), lz = ilog2(NZ);
+    if ((1 << lx) != NX || (1 << ly) != NY || (1 << lz) != NZ) return ORC_INVALID;
+    size_t sxv = (size_t)(NX + 1) * NY * NZ, syv = (size_t)NX * (NY + 1) * NZ,
+           szv = (size_t)NX * NY * (NZ + 1);
+    double* tx = malloc(sizeof(double) * sxv);
+    double* ty = malloc(sizeof(double) * syv);
+    double* tz = malloc(sizeof(double) * szv);
+    int rc = ORC_OK;
+    CHK(orc_transform(tx, xv, NX + 1, NY, NZ, lx, 'x', bd, wtd, 1));
+    CHK(orc_transform(tx, tx, NX + 1, NY, NZ, ly, 'y', bc, wtc, 1));
+    CHK(orc_transform(tx, tx, NX + 1, NY, NZ, lz, 'z', bc, wtc, 1));
+    CHK(orc_transform(ty, yv, NX, NY + 1, NZ, lx, 'x', bc, wtc, 1));
+    CHK(orc_transform(ty, ty, NX, NY + 1, NZ, ly, 'y', bd, wtd, 1));
+    CHK(orc_transform(ty, ty, NX, NY + 1, NZ, lz, 'z', bc, wtc, 1));
+    CHK(orc_transform(tz, zv, NX, NY, NZ + 1, lx, 'x', bc, wtc, 1));
+    CHK(orc_transform(tz, tz, NX, NY, NZ + 1, ly, 'y', bc, wtc, 1));
+    CHK(orc_transform(tz, tz, NX, NY, NZ + 1, lz, 'z', bd, wtd, 1));
+    if (xvo) {
+        /* 9-argument form: v_out is the transformed input, overwritten by s x q */
+        memcpy(xvo, tx, sizeof(double) * sxv);
+        memcpy(yvo, ty, sizeof(double) * syv);
+        memcpy(zvo, tz, sizeof(double) * szv);
+    }
+    CHK(orc_project3d(qx, qy, qz, xvo, yvo, zvo, tx, ty, tz, NX, NY, NZ, dx, dy, dz, wtd, bd));
+    CHK(orc_transform(qx, qx, NX, NY + 1, NZ + 1, lx, 'x', bc, wtc, 0));
+    CHK(orc_transform(qx, qx, NX, NY + 1, NZ + 1, ly, 'y', bd, wtd, 0));
+    CHK(orc_transform(qx, qx, NX, NY + 1, NZ + 1, lz, 'z', bd, wtd, 0));
+    CHK(orc_transform(qy, qy, NX + 1, NY, NZ + 1, lx, 'x', bd, wtd, 0));
+    CHK(orc_transform(qy, qy, NX + 1, NY, NZ + 1, ly, 'y', bc, wtc, 0));
+    CHK(orc_transform(qy, qy, NX + 1, NY, NZ + 1, lz, 'z', bd, wtd, 0));
+    CHK(orc_transform(qz, qz, NX + 1, NY + 1, NZ, lx, 'x', bd, wtd, 0));
+    CHK(orc_transform(qz, qz, NX + 1, NY + 1, NZ, ly, 'y', bd, wtd, 0));
+    CHK(orc_transform(qz, qz, NX + 1, NY + 1, NZ, lz, 'z', bc, wtc, 0));
+    if (xvo) {
+        CHK(orc_transform(xvo, xvo, NX + 1, NY, NZ, lx, 'x', bd, wtd, 0));
+        CHK(orc_transform(xvo, xvo, NX + 1, NY, NZ, ly, 'y', bc, wtc, 0));
+        CHK(orc_transform(xvo, xvo, NX + 1, NY, NZ, lz, 'z', bc, wtc, 0));
+        CHK(orc_transform(yvo, yvo, NX, NY + 1, NZ, lx, 'x', bc, wtc, 0));
+        CHK(orc_transform(yvo, yvo, NX, NY + 1, NZ, ly, 'y', bd, wtd, 0));
+        CHK(orc_transform(yvo, yvo, NX, NY + 1, NZ, lz, 'z', bc, wtc, 0));
+        CHK(orc_transform(zvo, zvo, NX, NY, NZ + 1, lx, 'x', bc, wtc, 0));
+        CHK(orc_transform(zvo, zvo, NX, NY, NZ + 1, ly, 'y', bc, wtc, 0));
+        CHK(orc_transform(zvo, zvo, NX, NY, NZ + 1, lz, 'z', bd, wtd, 0));
+    }
+    free(tx); free(ty); free(tz);
+    return rc;
+}
+
+int orc_recover2d(double* qz, const double* xv, const double* yv, int NX, int NY, double dx,
+                  double dy, char bc, int wtc, int wtd)
+{ /* CuWaveletPotentialRecover2D::Solve, WaveletPotentials/cuWaveletPotentialRecover2D.cu:16-37 */
+    const char bd = orc_proj_bc(bc);
+    const int lx = ilog2(NX), ly = ilog2(NY);
+    if ((1 << lx) != NX || (1 << ly) != NY) return ORC_INVALID;
+    double* tx = malloc(sizeof(double) * (NX + 1) * NY);
+    double* ty = malloc(sizeof(double) * NX * (NY + 1));
+    int rc = ORC_OK;
+    CHK(orc_transform(tx, xv, NX + 1, NY, 1, lx, 'x', bd, wtd, 1));
+    CHK(orc_transform(tx, tx, NX + 1, NY, 1, ly, 'y', bc, wtc, 1));
+    CHK(orc_transform(ty, yv, NX, NY + 1, 1, lx, 'x', bc, wtc, 1));
+    CHK(orc_transform(ty, ty, NX, NY + 1, 1, ly, 'y', bd, wtd, 1));
+    CHK(orc_project2d(qz, tx, ty, NX, NY, dx, dy, wtd, bd));
+    CHK(orc_transform(qz, qz, NX + 1, NY + 1, 1, lx, 'x', bd, wtd, 0));
+    CHK(orc_transform(qz, qz, NX + 1, NY + 1, 1, ly, 'y', bd, wtd, 0));
+    free(tx); free(ty);
+    return rc;
+}
+
+int orc_cut3d(double* qxo, double* qyo, double* qzo, const double* qx, const double* qy,
+              const double* qz, int NX, int NY, int NZ, char bc, int wtc, int wtd, int cut)
+{ /* CuWaveletPotentialRecover3D::CutCoefficients + cuCutCoefs,
+     WaveletPotentials/cuWaveletPotentialRecover3D.cu:123-239 */
+    const char bd = orc_proj_bc(bc);
+    const int lx = ilog2(NX), ly = ilog2(NY), lz = ilog2(NZ);
+    if ((1 << lx) != NX || (1 << ly) != NY || (1 << lz) != NZ) return ORC_INVALID;
+    int rc = ORC_OK;
+    CHK(orc_transform(qxo, qx, NX, NY + 1, NZ + 1, lx, 'x', bc, wtc, 1));
+    CHK(orc_transform(qxo, qxo, NX, NY + 1, NZ + 1, ly, 'y', bd, wtd, 1));
+    CHK(orc_transform(qxo, qxo, NX, NY + 1, NZ + 1, lz, 'z', bd, wtd, 1));
+    CHK(orc_transform(qyo, qy, NX + 1, NY, NZ + 1, lx, 'x', bd, wtd, 1));
+    CHK(orc_transform(qyo, qyo, NX + 1, NY, NZ + 1, ly, 'y', bc, wtc, 1));
+    CHK(orc_transform(qyo, qyo, NX + 1, NY, NZ + 1, lz, 'z', bd, wtd, 1));
+    CHK(orc_transform(qzo, qz, NX + 1, NY + 1, NZ, lx, 'x', bd, wtd, 1));
+    CHK(orc_transform(qzo, qzo, NX + 1, NY + 1, NZ, ly, 'y', bd, wtd, 1));
+    CHK(orc_transform(qzo, qzo, NX + 1, NY + 1, NZ, lz, 'z', bc, wtc, 1));
+    /* cuCutCoefs over the (NX+1, NY+1, NZ+1) node grid */
+    const int ex = NX + 1, ey = NY + 1, ez = NZ + 1;
+    const long sqx_x = NY + 1, sqx_z = (long)NX * (NY + 1);
+    const long sqy_x = NY, sqy_z = (long)(NX + 1) * NY;
+    const long sqz_x = NY + 1, sqz_z = (long)(NX + 1) * (NY + 1);
+    for (int z = 0; z < ez; ++z)
+        for (int x = 0; x < ex; ++x)
+            for (int y = 0; y < ey; ++y) {
+                /* INDEX3D with an index one below the extent's start can point
+                 * before the array; those writes are skipped by the guards of
+                 * the kernel's callers only where noted, so check bounds here */
+                long iqx = y + (long)(x - 1) * sqx_x + z * sqx_z;
+                long iqy = (y - 1) + (long)x * sqy_x + z * sqy_z;
+                long iqz = y + (long)x * sqz_x + (long)(z - 1) * sqz_z;
+                int jy = ly - bitlen(y - 1 > 0 ? y - 1 : 0);
+                int jx = lx - bitlen(x - 1 > 0 ? x - 1 : 0);
+                int jz = lz - bitlen(z - 1 > 0 ? z - 1 : 0);
+                long nqx = (long)NX * (NY + 1) * (NZ + 1), nqy = (long)(NX + 1) * NY * (NZ + 1),
+                     nqz = (long)(NX + 1) * (NY + 1) * NZ;
+#define ZX() do { if (iqx >= 0 && iqx < nqx) qxo[iqx] = 0.; } while (0)
+#define ZY() do { if (iqy >= 0 && iqy < nqy) qyo[iqy] = 0.; } while (0)
+#define ZZ() do { if (iqz >= 0 && iqz < nqz) qzo[iqz] = 0.; } while (0)
+                if (jx < cut) { ZX(); if (y > 0) ZY(); if (z > 0) ZZ(); }
+                if (jy < cut) { ZY(); if (x > 0) ZX(); if (z > 0) ZZ(); }
+                if (jz < cut) { ZZ(); if (x > 0) ZX(); if (y > 0) ZY(); }
+#undef ZX
+#undef ZY
+#undef ZZ
+            }
+    CHK(orc_transform(qxo, qxo, NX, NY + 1, NZ + 1, lx, 'x', bc, wtc, 0));
+    CHK(orc_transform(qxo, qxo, NX, NY + 1, NZ + 1, ly, 'y', bd, wtd, 0));
+    CHK(orc_transform(qxo, qxo, NX, NY + 1, NZ + 1, lz, 'z', bd, wtd, 0));
+    CHK(orc_transform(qyo, qyo, NX + 1, NY, NZ + 1, lx, 'x', bd, wtd, 0));
+    CHK(orc_transform(qyo, qyo, NX + 1, NY, NZ + 1, ly, 'y', bc, wtc, 0));
+    CHK(orc_transform(qyo, qyo, NX + 1, NY, NZ + 1, lz, 'z', bd, wtd, 0));
+    CHK(orc_transform(qzo, qzo, NX + 1, NY + 1, NZ, lx, 'x', bd, wtd, 0));
+    CHK(orc_transform(qzo, qzo, NX + 1, NY + 1, NZ, ly, 'y', bd, wtd, 0));
+    CHK(orc_transform(qzo, qzo, NX + 1, NY + 1, NZ, lz, 'z', bc, wtc, 0));
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Verification helpers                                                        */
+/* ------------------------------------------------------------------------- */
+void orc_curl3d(double* xv, double* yv, double* zv, const double* qx, const double* qy,
+                const double* qz, int NX, int NY, int NZ, double dx, double dy, double dz)
+{ /* StreamInitial::GradientStream 3D, WaveletPotentials/StreamInitial.cpp:87-143 */
+    const double ix = 1. / dx, iy = 1. / dy, iz = 1. / dz;
+    const long qx_x = NY + 1, qx_z = (long)NX * (NY + 1);
+    const long qy_x = NY, qy_z = (long)(NX + 1) * NY;
+    const long qz_x = NY + 1, qz_z = (long)(NX + 1) * (NY + 1);
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j <= NX; ++j)
+            for (int i = 0; i < NY; ++i) {
+                long o = i + (long)j * NY + (long)k * (NX + 1) * NY;
+                long a = i + j * qz_x + k * qz_z, b = i + j * qy_x + k * qy_z;
+                xv[o] = (qz[a + 1] - qz[a]) * iy - (qy[b + qy_z] - qy[b]) * iz;
+            }
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NX; ++j)
+            for (int i = 0; i <= NY; ++i) {
+                long o = i + (long)j * (NY + 1) + (long)k * NX * (NY + 1);
+                long a = i + j * qz_x + k * qz_z, b = i + j * qx_x + k * qx_z;
+                yv[o] = (qx[b + qx_z] - qx[b]) * iz - (qz[a + qz_x] - qz[a]) * ix;
+            }
+    for (int k = 0; k <= NZ; ++k)
+        for (int j = 0; j < NX; ++j)
+            for (int i = 0; i < NY; ++i) {
+                long o = i + (long)j * NY + (long)k * NX * NY;
+                long a = i + j * qx_x + k * qx_z, b = i + j * qy_x + k * qy_z;
+                zv[o] = (qy[b + qy_x] - qy[b]) * ix - (qx[a + 1] - qx[a]) * iy;
+            }
+}
+
+void orc_curl2d(double* xv, double* yv, const double* qz, int NX, int NY, double dx, double dy)
+{ /* StreamInitial::GradientStream 2D, WaveletPotentials/StreamInitial.cpp:145-175 */
+    for (int j = 0; j <= NX; ++j)
+        for (int i = 0; i < NY; ++i) {
+            long q = i + (long)j * (NY + 1);
+            xv[i + (long)j * NY] = (qz[q + 1] - qz[q]) / dy;
+        }
+    for (int j = 0; j < NX; ++j)
+        for (int i = 0; i <= NY; ++i) {
+            long q = i + (long)j * (NY + 1);
+            yv[i + (long)j * (NY + 1)] = -(qz[q + NY + 1] - qz[q]) / dx;
+        }
+}
+
+void orc_div3d(double* div, const double* xv, const double* yv, const double* zv, int NX, int NY,
+               int NZ, double dx, double dy, double dz)
+{
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NX; ++j)
+            for (int i = 0; i < NY; ++i) {
+                long c = i + (long)j * NY + (long)k * NX * NY;
+                long a = i + (long)j * NY + (long)k * (NX + 1) * NY;
+                long b = i + (long)j * (NY + 1) + (long)k * NX * (NY + 1);
+                div[c] = (xv[a + NY] - xv[a]) / dx + (yv[b + 1] - yv[b]) / dy +
+                         (zv[c + (long)NX * NY] - zv[c]) / dz;
+            }
+}

@@ -1,0 +1,94 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * CPU FP64 restatement of the reference's wavelet potential-recovery path
+ * (yours321dog/WaveletPotentials).  It is the checker for the HIP library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load or call it.  Nothing in the shipped library depends on it.
+ *
+ * Layout convention (WaveletPotentials/Utils.h:46-47): a 3D array of extents
+ * (nx, ny, nz) is stored y-fastest: index(i=y, j=x, k=z) = i + j*ny + k*nx*ny.
+ *
+ * Every function returns 0 on success, a negative code on invalid input, and
+ * ORC_UNDEFINED when the reference's algorithm would read outside the line
+ * for that (wavelet type, boundary condition, length) -- the reference gives
+ * no defined result there.
+ */
+#ifndef WHHD_ORACLE_H
+#define WHHD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_OK 0
+#define ORC_INVALID (-1)
+#define ORC_UNSUPPORTED (-2)
+#define ORC_UNDEFINED (-3)
+
+/* WaveletType enumerators in the order of WaveletPotentials/WaveletTypes.h:7-28 */
+enum orc_wavelet {
+    ORC_CDF3_9 = 0, ORC_CDF3_11, ORC_CDF3_5, ORC_CDF2_10, ORC_CDF2_12, ORC_CDF2_6,
+    ORC_CDF4_4, ORC_CDF4_6, ORC_CDF4_8, ORC_CDF4_10, ORC_CDF2_8, ORC_CDF3_7,
+    ORC_CDF26_44, ORC_CDF28_46, ORC_CDF44_26, ORC_CDF46_28, ORC_CDF2_0, ORC_CDF1_1, ORC_CDF1_7
+};
+
+int orc_wavelet_supported(int wt, char bc);
+int orc_wavelet_halo(int wt);
+char orc_proj_bc(char bc);
+int orc_proj_wavelet(int wt);
+
+/* One multi-level transform of every line along `axis` ('x','y','z') of a
+ * (nx,ny,nz) array.  dst may equal src.  levels = floor(log2(line length)). */
+int orc_transform(double* dst, const double* src, int nx, int ny, int nz, int levels,
+                  char axis, char bc, int wt, int forward);
+
+/* Literal emulation of the reference's 32-lane y-axis kernels (window, halo,
+ * shuffle and shared-memory ping-pong semantics), for a single line. */
+int orc_emulate_line(double* dst, const double* src, int n, int levels, char bc, int wt,
+                     int forward);
+
+/* Gram table of the synthesis basis of (wt, bc) on n samples: 8 rows of n,
+ * rows as WaveletPotentials/cuGetInnerProduct.cu:11-77 (row 5 is 0 where the
+ * reference leaves it unwritten). */
+int orc_gram_table(double* out8n, int wt, char bc, int n);
+
+/* 3D per-cell projection, velocity BC 'n' (bc_div 'd'): cuProjectLocal_q_3d_dq_ccc.
+ * N = 2^L cells per axis (possibly different per axis). ip_* are Gram diagonals. */
+int orc_project3d(double* qx, double* qy, double* qz, double* xv_out, double* yv_out,
+                  double* zv_out, const double* xv, const double* yv, const double* zv,
+                  int nx, int ny, int nz, double dx, double dy, double dz, int wt_div,
+                  char bc_div);
+
+/* 2D node projection cuProjectLocal_q_single (bc_base 'd'). */
+int orc_project2d(double* qz, const double* xv, const double* yv, int nx, int ny,
+                  double dx, double dy, int wt_div, char bc_div);
+
+/* CuWaveletPotentialRecover3D::Solve (6- and 9-argument forms; v_out may be NULL). */
+int orc_recover3d(double* qx, double* qy, double* qz, double* xv_out, double* yv_out,
+                  double* zv_out, const double* xv, const double* yv, const double* zv,
+                  int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                  int wt_curl, int wt_div);
+
+/* CuWaveletPotentialRecover2D::Solve */
+int orc_recover2d(double* qz, const double* xv, const double* yv, int nx, int ny,
+                  double dx, double dy, char bc, int wt_curl, int wt_div);
+
+/* CuWaveletPotentialRecover3D::CutCoefficients */
+int orc_cut3d(double* qx_out, double* qy_out, double* qz_out, const double* qx,
+              const double* qy, const double* qz, int nx, int ny, int nz, char bc,
+              int wt_curl, int wt_div, int level);
+
+/* Discrete curl of an edge potential to face velocities (StreamInitial.cpp:87-175). */
+void orc_curl3d(double* xv, double* yv, double* zv, const double* qx, const double* qy,
+                const double* qz, int nx, int ny, int nz, double dx, double dy, double dz);
+void orc_curl2d(double* xv, double* yv, const double* qz, int nx, int ny, double dx,
+                double dy);
+/* Cell divergence of face velocities (nx*ny*nz cells). */
+void orc_div3d(double* div, const double* xv, const double* yv, const double* zv, int nx,
+               int ny, int nz, double dx, double dy, double dz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+"""ctypes binding of libwhhd.so (the HIP library; C ABI declared in include/whhd.h).
+
+The library is built in-tree by ``waveletpotentials_amd.build``.  Loading it
+does not need a GPU; calling a compute entry point does.  There is no CPU
+fallback: if the library is missing, :func:`lib` raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwhhd.so")
+
+STATUS = {0: "WHHD_OK", 1: "WHHD_INVALID_ARG", 2: "WHHD_NOT_POW2", 3: "WHHD_UNSUPPORTED_WAVELET",
+          4: "WHHD_HIP_ERROR", 5: "WHHD_OOM", 6: "WHHD_NONCONVERGED"}
+
+# every function include/whhd.h declares, with (restype, argtypes)
+_P, _I, _C, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_char, ctypes.c_double
+_S = ctypes.c_char_p
+SIGNATURES = {
+    "whhd_status_string": (_S, [_I]),
+    "whhd_last_error": (_S, []),
+    "whhd_version": (_S, []),
+    "whhd_proj_bc": (_C, [_C]),
+    "whhd_proj_wavelet": (_I, [_I]),
+    "whhd_wavelet_supported": (_I, [_I, _C]),
+    "whhd_transform1d": (_I, [_P, _P, _I, _I, _I, _I, _C, _C, _I, _I, _P]),
+    "whhd_transform1d_f64": (_I, [_P, _P, _I, _I, _I, _I, _C, _C, _I, _I, _P]),
+    "whhd_gram_table": (_I, [_P, _I, _C, _I, _P]),
+    "whhd_project3d_create": (_I, [_I, _I, _I, _D, _D, _D, _I, _C, ctypes.POINTER(_P)]),
+    "whhd_project3d_run": (_I, [_P] * 10 + [_P]),
+    "whhd_project3d_destroy": (None, [_P]),
+    "whhd_project2d_create": (_I, [_I, _I, _D, _D, _I, _C, ctypes.POINTER(_P)]),
+    "whhd_project2d_run": (_I, [_P, _P, _P, _P, _P]),
+    "whhd_project2d_destroy": (None, [_P]),
+    "whhd_recover3d_create": (_I, [_I, _I, _I, _D, _D, _D, _C, _I, _I, ctypes.POINTER(_P)]),
+    "whhd_recover3d_solve": (_I, [_P] * 7 + [_P]),
+    "whhd_recover3d_solve_proj": (_I, [_P] * 10 + [_P]),
+    "whhd_recover3d_cut": (_I, [_P] * 7 + [_I, _P]),
+    "whhd_recover3d_destroy": (None, [_P]),
+    "whhd_recover2d_create": (_I, [_I, _I, _D, _D, _C, _I, _I, ctypes.POINTER(_P)]),
+    "whhd_recover2d_solve": (_I, [_P, _P, _P, _P, _P]),
+    "whhd_recover2d_destroy": (None, [_P]),
+}
+
+_lib = None
+
+
+class WhhdError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = ""
+        if _lib is not None:
+            msg = (_lib.whhd_last_error() or b"").decode(errors="replace")
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {msg}")
+
+
+def lib():
+    """Load libwhhd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m waveletpotentials_amd.build` "
+                               "(or __graft_entry__.build()) first; there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, where):
+    if status != 0:
+        raise WhhdError(status, where)
+    return status

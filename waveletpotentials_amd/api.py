@@ -1,0 +1,320 @@
+"""Python mirror of the reference's recovery-path classes over libwhhd.so.
+
+Names, argument order and meaning follow the reference's C++ interface so a
+caller of ``CuWaveletPotentialRecover3D`` & co. finds the same surface:
+
+* ``WaveletType`` / ``proj_bc`` / ``proj_WaveletType`` / ``type_from_string`` /
+  ``string_from_type``  (WaveletPotentials/WaveletTypes.h:7-143, WaveletTypes.cpp:4-67)
+* ``CuWHHDForward.GetInstance().Solve1D``  (WaveletPotentials/cuWHHDForward.cuh:17-25)
+* ``CuGetInnerProduct.GetInstance().GenerateIp``  (WaveletPotentials/cuGetInnerProduct.cuh:21-22)
+* ``CuDivLocalProject3D`` / ``CuDivLocalProject``  (WaveletPotentials/cuDivLocalProject3D.cuh:7-47,
+  WaveletPotentials/cuDivLocalProject.cuh:7-33)
+* ``CuWaveletPotentialRecover3D`` / ``CuWaveletPotentialRecover2D``
+  (WaveletPotentials/cuWaveletPotentialRecover3D.cuh:8-31, cuWaveletPotentialRecover2D.cuh)
+
+Fields are torch tensors on the GPU (float32; Solve1D also takes float64),
+stored y-fastest: a field of extents (nx, ny, nz) is a contiguous tensor of
+shape (nz, nx, ny) (2D: (nx, ny)).  Where the reference aborts the process on
+an error, these raise :class:`WhhdError`.  Work is enqueued on torch's current
+stream.
+"""
+import ctypes
+import enum
+
+import torch
+
+from ._lib import WhhdError, check, lib
+
+__all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "string_from_type",
+           "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
+           "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "WhhdError",
+           "mac_shapes3d", "mac_shapes2d"]
+
+
+class WaveletType(enum.IntEnum):
+    """WaveletPotentials/WaveletTypes.h:7-28 (same enumerator order)."""
+    WT_CDF3_9 = 0
+    WT_CDF3_11 = 1
+    WT_CDF3_5 = 2
+    WT_CDF2_10 = 3
+    WT_CDF2_12 = 4
+    WT_CDF2_6 = 5
+    WT_CDF4_4 = 6
+    WT_CDF4_6 = 7
+    WT_CDF4_8 = 8
+    WT_CDF4_10 = 9
+    WT_CDF2_8 = 10
+    WT_CDF3_7 = 11
+    WT_CDF26_44 = 12
+    WT_CDF28_46 = 13
+    WT_CDF44_26 = 14
+    WT_CDF46_28 = 15
+    WT_CDF2_0 = 16
+    WT_CDF1_1 = 17
+    WT_CDF1_7 = 18
+
+
+_NAMES = {WaveletType.WT_CDF3_5: "cdf3.5", WaveletType.WT_CDF3_7: "cdf3.7",
+          WaveletType.WT_CDF3_9: "cdf3.9", WaveletType.WT_CDF3_11: "cdf3.11",
+          WaveletType.WT_CDF2_6: "cdf2.6", WaveletType.WT_CDF2_8: "cdf2.8",
+          WaveletType.WT_CDF2_10: "cdf2.10", WaveletType.WT_CDF2_12: "cdf2.12",
+          WaveletType.WT_CDF4_4: "cdf4.4", WaveletType.WT_CDF4_6: "cdf4.6",
+          WaveletType.WT_CDF4_8: "cdf4.8", WaveletType.WT_CDF4_10: "cdf4.10",
+          WaveletType.WT_CDF2_0: "cdf2.0", WaveletType.WT_CDF1_1: "cdf1.1",
+          WaveletType.WT_CDF1_7: "cdf1.7"}
+
+
+def proj_bc(bc):
+    """WaveletPotentials/WaveletTypes.h:102-118: 'n' <-> 'd', others unchanged."""
+    return {"n": "d", "d": "n"}.get(bc, bc)
+
+
+def proj_WaveletType(t):
+    """WaveletPotentials/WaveletTypes.h:120-143."""
+    r = lib().whhd_proj_wavelet(int(t))
+    return None if r < 0 else WaveletType(r)
+
+
+def type_from_string(s):
+    """WaveletPotentials/WaveletTypes.cpp:4-67: 'n_cdf3.7' -> WT_CDF3_7 (default WT_CDF3_5)."""
+    tail = s[2:]
+    for t, name in _NAMES.items():
+        if name == tail:
+            return t
+    return WaveletType.WT_CDF3_5
+
+
+def string_from_type(t, bc):
+    """WaveletPotentials/WaveletTypes.h:67-100."""
+    return f"{bc}_{_NAMES[WaveletType(t)]}"
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t, dtype=torch.float32, numel=None, name="tensor"):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name}: expected a CUDA/HIP torch tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: expected dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} elements, got {t.numel()}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _b(ch):
+    return ch.encode() if isinstance(ch, str) else ch
+
+
+def mac_shapes3d(nx, ny, nz):
+    """Tensor shapes (nz, nx, ny)-ordered of xv, yv, zv, qx, qy, qz for nx*ny*nz cells."""
+    return {"xv": (nz, nx + 1, ny), "yv": (nz, nx, ny + 1), "zv": (nz + 1, nx, ny),
+            "qx": (nz + 1, nx, ny + 1), "qy": (nz + 1, nx + 1, ny), "qz": (nz, nx + 1, ny + 1)}
+
+
+def mac_shapes2d(nx, ny):
+    return {"xv": (nx + 1, ny), "yv": (nx, ny + 1), "qz": (nx + 1, ny + 1)}
+
+
+class CuWHHDForward:
+    """CuWHHDForward (WaveletPotentials/cuWHHDForward.cuh:11-37): line transforms."""
+    _instance = None
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def Solve1D(self, dst, src, dim, level, direction, bc="n", rsw_type=WaveletType.WT_CDF3_5,
+                is_forward=True):
+        """Multi-level transform of every line of the (dim.x, dim.y, dim.z) array
+        along `direction`; src None means in place (cuWHHDForward.cu:1111-1120)."""
+        if src is None:
+            src = dst
+        nx, ny, nz = dim
+        n = nx * ny * nz
+        if dst.dtype == torch.float64:
+            f = lib().whhd_transform1d_f64
+            dt = torch.float64
+        else:
+            f = lib().whhd_transform1d
+            dt = torch.float32
+        check(f(_ptr(dst, dt, n, "dst"), _ptr(src, dt, n, "src"), nx, ny, nz, int(level),
+                _b(direction), _b(bc), int(rsw_type), int(bool(is_forward)), _stream()),
+              "Solve1D")
+
+
+class CuGetInnerProduct:
+    """CuGetInnerProduct (WaveletPotentials/cuGetInnerProduct.cuh:8-28); tables are cached
+    per (type, bc, n) like the reference's ips_map_."""
+    _instance = None
+
+    def __init__(self):
+        self._cache = {}
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def GenerateIp(self, type_or_str, bc_or_dim, dim=None):
+        """GenerateIp(type, bc, n) or GenerateIp("d_cdf4.6", n): (8, n) float64 tensor."""
+        if isinstance(type_or_str, str):
+            t, bc, n = type_from_string(type_or_str), type_or_str[0], bc_or_dim
+        else:
+            t, bc, n = type_or_str, bc_or_dim, dim
+        key = (int(t), bc, int(n))
+        if key not in self._cache:
+            out = torch.empty((8, n), dtype=torch.float64, device="cuda")
+            check(lib().whhd_gram_table(_ptr(out, torch.float64), int(t), _b(bc), int(n),
+                                        _stream()), "GenerateIp")
+            self._cache[key] = out
+        return self._cache[key]
+
+    def GetIp(self, type_or_str, bc_or_dim, dim=None):
+        if isinstance(type_or_str, str):
+            key = (int(type_from_string(type_or_str)), type_or_str[0], int(bc_or_dim))
+        else:
+            key = (int(type_or_str), bc_or_dim, int(dim))
+        return self._cache.get(key)
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and self._h.value and self._destroy:
+                getattr(lib(), self._destroy)(self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:
+            pass
+
+
+class CuDivLocalProject3D(_Handle):
+    """CuDivLocalProject3D(int3 dim, int3 levels, DTYPE3 dx, WaveletType type, char bc)
+    (WaveletPotentials/cuDivLocalProject3D.cuh:11).  As in the reference, `dim` is
+    the node extent (cells + 1) and `type`/`bc` the divergence type and bc."""
+    _destroy = "whhd_project3d_destroy"
+
+    def __init__(self, dim, levels, dx, type, bc):
+        super().__init__()
+        self.levels = tuple(int(v) for v in levels)
+        self.n = tuple(1 << v for v in self.levels)
+        check(lib().whhd_project3d_create(*self.n, *[float(v) for v in dx], int(type), _b(bc),
+                                          ctypes.byref(self._h)), "CuDivLocalProject3D")
+
+    def ProjectLocal_q_ccc(self, q_lwt_x, q_lwt_y, q_lwt_z, *rest):
+        """(q..., v..., levels) or (q..., v_out..., v..., levels)
+        (WaveletPotentials/cuDivLocalProject3D.cuh:17-20)."""
+        args = [a for a in rest if isinstance(a, torch.Tensor)]
+        nx, ny, nz = self.n
+        s = mac_shapes3d(nx, ny, nz)
+        size = {k: int(torch.Size(v).numel()) for k, v in s.items()}
+        q = [_ptr(q_lwt_x, numel=size["qx"], name="qx"), _ptr(q_lwt_y, numel=size["qy"], name="qy"),
+             _ptr(q_lwt_z, numel=size["qz"], name="qz")]
+        if len(args) == 6:
+            vo = [_ptr(args[0], numel=size["xv"]), _ptr(args[1], numel=size["yv"]),
+                  _ptr(args[2], numel=size["zv"])]
+            v = args[3:]
+        elif len(args) == 3:
+            vo = [None, None, None]
+            v = args
+        else:
+            raise TypeError("ProjectLocal_q_ccc expects 3 potential and 3 or 6 velocity tensors")
+        vp = [_ptr(v[0], numel=size["xv"], name="xv"), _ptr(v[1], numel=size["yv"], name="yv"),
+              _ptr(v[2], numel=size["zv"], name="zv")]
+        check(lib().whhd_project3d_run(self._h, *q, *vo, *vp, _stream()), "ProjectLocal_q_ccc")
+
+
+class CuDivLocalProject(_Handle):
+    """CuDivLocalProject(int2 dim, int2 levels, DTYPE2 dx, WaveletType, char bc)
+    (WaveletPotentials/cuDivLocalProject.cuh:10); `dim` is the node extent."""
+    _destroy = "whhd_project2d_destroy"
+
+    def __init__(self, dim, levels, dx, type, bc):
+        super().__init__()
+        self.levels = tuple(int(v) for v in levels)
+        self.n = tuple(1 << v for v in self.levels)
+        check(lib().whhd_project2d_create(*self.n, float(dx[0]), float(dx[1]), int(type), _b(bc),
+                                          ctypes.byref(self._h)), "CuDivLocalProject")
+
+    def ProjectSingle(self, p_lwt, xv_lwt, yv_lwt):
+        nx, ny = self.n
+        check(lib().whhd_project2d_run(self._h, _ptr(p_lwt, numel=(nx + 1) * (ny + 1)),
+                                       _ptr(xv_lwt, numel=(nx + 1) * ny),
+                                       _ptr(yv_lwt, numel=nx * (ny + 1)), _stream()),
+              "ProjectSingle")
+
+
+class CuWaveletPotentialRecover3D(_Handle):
+    """CuWaveletPotentialRecover3D(int3 dim, DTYPE3 dx, char bc, WaveletType curl,
+    WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover3D.cuh:12); `dim`
+    is the cell count per axis."""
+    _destroy = "whhd_recover3d_destroy"
+
+    def __init__(self, dim, dx, bc, type_w_curl, type_w_div):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        self.dx = tuple(float(v) for v in dx)
+        self.bc = bc
+        check(lib().whhd_recover3d_create(*self.dim, *self.dx, _b(bc), int(type_w_curl),
+                                          int(type_w_div), ctypes.byref(self._h)),
+              "CuWaveletPotentialRecover3D")
+        s = mac_shapes3d(*self.dim)
+        self._size = {k: int(torch.Size(v).numel()) for k, v in s.items()}
+
+    def _q(self, qx, qy, qz):
+        s = self._size
+        return [_ptr(qx, numel=s["qx"], name="qx"), _ptr(qy, numel=s["qy"], name="qy"),
+                _ptr(qz, numel=s["qz"], name="qz")]
+
+    def _v(self, xv, yv, zv):
+        s = self._size
+        return [_ptr(xv, numel=s["xv"], name="xv"), _ptr(yv, numel=s["yv"], name="yv"),
+                _ptr(zv, numel=s["zv"], name="zv")]
+
+    def Solve(self, qx, qy, qz, *v):
+        """Solve(qx, qy, qz, xv, yv, zv) or Solve(qx, qy, qz, xv_out, yv_out, zv_out, xv, yv, zv)
+        (WaveletPotentials/cuWaveletPotentialRecover3D.cuh:15-16)."""
+        if len(v) == 3:
+            check(lib().whhd_recover3d_solve(self._h, *self._q(qx, qy, qz), *self._v(*v),
+                                             _stream()), "Solve")
+        elif len(v) == 6:
+            check(lib().whhd_recover3d_solve_proj(self._h, *self._q(qx, qy, qz), *self._v(*v[:3]),
+                                                  *self._v(*v[3:]), _stream()), "Solve")
+        else:
+            raise TypeError("Solve expects 3 or 6 velocity tensors")
+
+    def CutCoefficients(self, qx_out, qy_out, qz_out, qx, qy, qz, level):
+        check(lib().whhd_recover3d_cut(self._h, *self._q(qx_out, qy_out, qz_out),
+                                       *self._q(qx, qy, qz), int(level), _stream()),
+              "CutCoefficients")
+
+
+class CuWaveletPotentialRecover2D(_Handle):
+    """CuWaveletPotentialRecover2D(int2 dim, DTYPE2 dx, char bc, WaveletType curl,
+    WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover2D.cuh)."""
+    _destroy = "whhd_recover2d_destroy"
+
+    def __init__(self, dim, dx, bc, type_w_curl, type_w_div):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        check(lib().whhd_recover2d_create(*self.dim, float(dx[0]), float(dx[1]), _b(bc),
+                                          int(type_w_curl), int(type_w_div),
+                                          ctypes.byref(self._h)), "CuWaveletPotentialRecover2D")
+
+    def Solve(self, qz, xv, yv):
+        nx, ny = self.dim
+        check(lib().whhd_recover2d_solve(self._h, _ptr(qz, numel=(nx + 1) * (ny + 1), name="qz"),
+                                         _ptr(xv, numel=(nx + 1) * ny, name="xv"),
+                                         _ptr(yv, numel=nx * (ny + 1), name="yv"), _stream()),
+              "Solve")

@@ -1,0 +1,371 @@
+// Per-cell local projection in wavelet space.
+//
+// 3D: cuProjectLocal_q_3d_dq_ccc (WaveletPotentials/cuDivLocalProject3D.cu:
+// 4013-4246; 9-argument form with v_out = s x q :5717-5965).  For cell
+// (x, y, z) the kernel reads xv(x+1,y,z), yv(x,y+1,z), zv(x,y,z+1) and writes
+// qx(x,y+1,z+1), qy(x+1,y,z+1), qz(x+1,y+1,z): interior cells get
+// q = (s x v)/|s|^2, cells on one index-0 plane get one Gram-weighted
+// component, and the index-0 lines and corner write zeros, covering every
+// potential coefficient exactly once.
+//
+// 2D: cuProjectLocal_q_single (WaveletPotentials/cuDivLocalProject.cu:765-1045):
+// node (X, Y) with X, Y > 1 gets
+// (xv*sy*B1x*B0y - yv*sx*B0x*B1y) / (sx^2*B0x*B1y + sy^2*B1x*B0y + eps);
+// the kernel there evaluates this in two tile orientations and adds halves.
+//
+// Both are single streaming passes, one thread per cell, y fastest so that
+// every wavefront touches contiguous rows.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "whhd_internal.h"
+
+namespace whhd {
+
+constexpr double kEps = 1e-12;  // eps<DTYPE>, WaveletPotentials/Utils.h:52-53
+
+static int bitlen(int v) {
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+// Per-axis coefficient rows: s[c], B0[c] (projection-type Gram diagonal at c),
+// B1[c] (divergence-type Gram diagonal at c+1).
+struct AxisCoef {
+    const float* s;
+    const float* b0;
+    const float* b1;
+};
+
+struct Proj3Args {
+    float *qx, *qy, *qz, *xo, *yo, *zo;
+    const float *xv, *yv, *zv;
+    AxisCoef cx, cy, cz;
+    int nx, ny, nz;
+};
+
+__global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= p.ny || x >= p.nx) return;
+    const int64_t NY = p.ny, NX = p.nx;
+    const int64_t sxv_z = (NX + 1) * NY, syv_z = NX * (NY + 1), szv_z = NX * NY;
+    const int64_t sqx_x = NY + 1, sqx_z = NX * (NY + 1);
+    const int64_t sqy_x = NY, sqy_z = (NX + 1) * NY;
+    const int64_t sqz_x = NY + 1, sqz_z = (NX + 1) * (NY + 1);
+    const int64_t ixv = y + (x + 1) * NY + z * sxv_z;
+    const int64_t iyv = (y + 1) + x * (NY + 1) + z * syv_z;
+    const int64_t izv = y + x * NY + (z + 1) * szv_z;
+    const int64_t iqx = (y + 1) + x * sqx_x + (z + 1) * sqx_z;
+    const int64_t iqy = y + (x + 1) * sqy_x + (z + 1) * sqy_z;
+    const int64_t iqz = (y + 1) + (x + 1) * sqz_x + z * sqz_z;
+
+    const float sx = p.cx.s[x], sy = p.cy.s[y], sz = p.cz.s[z];
+    const float vx = p.xv[ixv], vy = p.yv[iyv], vz = p.zv[izv];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    if (x > 0 && y > 0 && z > 0) {
+        const float inv = 1.f / (sy * sy + sx * sx + sz * sz);
+        lz = (sy * vx - sx * vy) * inv;
+        lx = (sz * vy - sy * vz) * inv;
+        ly = (sx * vz - sz * vx) * inv;
+    } else {
+        const float ip0x = p.cx.b0[x], ip1x = p.cx.b1[x];
+        const float ip0y = p.cy.b0[y], ip1y = p.cy.b1[y];
+        const float ip0z = p.cz.b0[z], ip1z = p.cz.b1[z];
+        const float eps = (float)kEps;
+        if (y > 0 && z > 0 && x == 0) {
+            const float syByBz = sy * ip0x * ip0y * ip1z;
+            const float szByBz = sz * ip0x * ip1y * ip0z;
+            const float mx = 1.f / (sy * syByBz + sz * szByBz + eps);
+            lx = (vy * szByBz - vz * syByBz) * mx;
+            p.qz[iqz - sqz_x] = 0.f;
+            p.qy[iqy - sqy_x] = 0.f;
+        } else if (x > 0 && z > 0 && y == 0) {
+            const float sxBxBz = sx * ip0x * ip0y * ip1z;
+            const float szBxBz = sz * ip1x * ip0y * ip0z;
+            const float my = 1.f / (sx * sxBxBz + sz * szBxBz + eps);
+            ly = (vz * sxBxBz - vx * szBxBz) * my;
+            p.qx[iqx - 1] = 0.f;
+            p.qz[iqz - 1] = 0.f;
+        } else if (x > 0 && y > 0 && z == 0) {
+            const float sxBxBy = sx * ip0x * ip1y * ip0z;
+            const float syBxBy = sy * ip1x * ip0y * ip0z;
+            const float mz = 1.f / (sx * sxBxBy + sy * syBxBy + eps);
+            lz = (vx * syBxBy - vy * sxBxBy) * mz;
+            p.qx[iqx - sqx_z] = 0.f;
+            p.qy[iqy - sqy_z] = 0.f;
+        } else if (y == 0 && x > 0 && z == 0) {
+            p.qy[iqy - sqy_z] = 0.f;
+            p.qz[iqz - 1] = 0.f;
+            p.qx[iqx - 1] = 0.f;
+            p.qx[iqx - sqx_z - 1] = 0.f;
+            p.qx[iqx - sqx_z] = 0.f;
+        } else if (y > 0 && x == 0 && z == 0) {
+            p.qx[iqx - sqx_z] = 0.f;
+            p.qz[iqz - sqz_x] = 0.f;
+            p.qy[iqy - sqy_x - sqy_z] = 0.f;
+            p.qy[iqy - sqy_z] = 0.f;
+            p.qy[iqy - sqy_x] = 0.f;
+        } else if (y == 0 && x == 0 && z > 0) {
+            p.qx[iqx - 1] = 0.f;
+            p.qy[iqy - sqy_x] = 0.f;
+            p.qz[iqz - sqz_x - 1] = 0.f;
+            p.qz[iqz - sqz_x] = 0.f;
+            p.qz[iqz - 1] = 0.f;
+        } else {
+            p.qz[iqz - 1 - sqz_x] = 0.f;
+            p.qz[iqz - 1] = 0.f;
+            p.qz[iqz - sqz_x] = 0.f;
+            p.qx[iqx - 1 - sqx_z] = 0.f;
+            p.qx[iqx - sqx_z] = 0.f;
+            p.qx[iqx - 1] = 0.f;
+            p.qy[iqy - sqy_x - sqy_z] = 0.f;
+            p.qy[iqy - sqy_z] = 0.f;
+            p.qy[iqy - sqy_x] = 0.f;
+        }
+    }
+    p.qx[iqx] = lx;
+    p.qy[iqy] = ly;
+    p.qz[iqz] = lz;
+    if (p.xo) {
+        p.xo[ixv] = sy * lz - sz * ly;
+        p.yo[iyv] = sz * lx - sx * lz;
+        p.zo[izv] = sx * ly - sy * lx;
+    }
+}
+
+struct Proj2Args {
+    float* qz;
+    const float *xv, *yv;
+    AxisCoef cx, cy;  // indexed by node X (resp. Y): s from X-1, B0[X-1], B1[X]
+    int nx, ny;
+};
+
+__global__ void __launch_bounds__(256) project2d_kernel(Proj2Args p) {
+    const int Y = blockIdx.x * 64 + threadIdx.x;
+    const int X = blockIdx.y * 4 + threadIdx.y;
+    if (Y > p.ny || X > p.nx) return;
+    float q = 0.f;
+    if (X > 1 && Y > 1) {
+        const float vx = p.xv[(Y - 1) + (int64_t)X * p.ny];
+        const float vy = p.yv[Y + (int64_t)(X - 1) * (p.ny + 1)];
+        const float sx = p.cx.s[X], sy = p.cy.s[Y];
+        const float ip0x = p.cx.b0[X], ip1x = p.cx.b1[X];
+        const float ip0y = p.cy.b0[Y], ip1y = p.cy.b1[Y];
+        const float mid = sx * sx * ip0x * ip1y + sy * sy * ip1x * ip0y + (float)kEps;
+        q = (vx * sy * ip1x * ip0y - vy * sx * ip0x * ip1y) / mid;
+    }
+    p.qz[Y + (int64_t)X * (p.ny + 1)] = q;
+}
+
+// ---------------------------------------------------------------------------
+// Host: coefficient tables
+// ---------------------------------------------------------------------------
+static int gram_diag_host(std::vector<double>& diag, int wt, char bc, int n) {
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * 8 * (size_t)n) != hipSuccess)
+        return fail(WHHD_OOM, "gram diagonal allocation failed");
+    int st = gram_table_device(d, wt, bc, n, nullptr);
+    if (st == WHHD_OK) {
+        diag.resize(n);
+        hipError_t e = hipMemcpy(diag.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "gram diagonal copy");
+    }
+    hipFree(d);
+    return st;
+}
+
+// rows: [s | B0 | B1], each of `len` floats.  mode3d: index c is the cell
+// (s from bitlen(c), B0[c], B1[c+1]); otherwise index X is the node
+// (s from bitlen(X-1), B0[X-1], B1[X]).
+static int axis_coef(std::vector<float>& rows, int N, double h, int wt_div, char bc_div,
+                     bool mode3d) {
+    const int L = ilog2(N);
+    std::vector<double> base, proj;
+    int st = gram_diag_host(base, wt_div, bc_div, N + 1);
+    if (st != WHHD_OK) return st;
+    st = gram_diag_host(proj, proj_wavelet(wt_div), proj_bc(bc_div), N);
+    if (st != WHHD_OK) return st;
+    const int len = N + 1;
+    rows.assign(3 * (size_t)len, 0.f);
+    const double hinv = 1.0 / h;
+    for (int c = 0; c < len; ++c) {
+        int bl, i0, i1;
+        if (mode3d) { bl = bitlen(c); i0 = c; i1 = c + 1; }
+        else { bl = c >= 1 ? bitlen(c - 1) : 0; i0 = c - 1; i1 = c; }
+        const int j = L - bl + 1;
+        const double s = 4.0 * hinv / (std::ldexp(1.0, j) + kEps);
+        rows[c] = (float)s;
+        rows[len + c] = (i0 >= 0 && i0 < N) ? (float)proj[i0] : 0.f;
+        rows[2 * len + c] = (i1 >= 0 && i1 <= N) ? (float)base[i1] : 0.f;
+    }
+    return WHHD_OK;
+}
+
+}  // namespace whhd
+
+struct whhd_project3d_s {
+    int nx, ny, nz;
+    float* coef = nullptr;  // 3 axes x 3 rows x (n+1)
+    whhd::AxisCoef cx, cy, cz;
+};
+
+struct whhd_project2d_s {
+    int nx, ny;
+    float* coef = nullptr;
+    whhd::AxisCoef cx, cy;
+};
+
+namespace whhd {
+
+static bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
+
+int project3d_create(int nx, int ny, int nz, double dx, double dy, double dz, int wt, char bc,
+                     whhd_project3d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "project3d_create: null handle pointer");
+    *out = nullptr;
+    if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))
+        return fail(WHHD_NOT_POW2, "project3d: cell counts must be powers of two (>= 4)");
+    if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "project3d: spacing must be > 0");
+    if (bc != 'd')
+        return fail(WHHD_UNSUPPORTED_WAVELET,
+                    "project3d: only the divergence bc 'd' (velocity bc 'n') is implemented");
+    int st = check_wavelet(wt, bc);
+    if (st != WHHD_OK) return st;
+    if (proj_wavelet(wt) < 0) return fail(WHHD_UNSUPPORTED_WAVELET, "project3d: no projection type");
+    std::vector<float> rx, ry, rz;
+    if ((st = axis_coef(rx, nx, dx, wt, bc, true)) != WHHD_OK) return st;
+    if ((st = axis_coef(ry, ny, dy, wt, bc, true)) != WHHD_OK) return st;
+    if ((st = axis_coef(rz, nz, dz, wt, bc, true)) != WHHD_OK) return st;
+    auto* h = new whhd_project3d_s;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    const size_t tot = rx.size() + ry.size() + rz.size();
+    if (hipMalloc(&h->coef, tot * sizeof(float)) != hipSuccess) {
+        delete h;
+        return fail(WHHD_OOM, "project3d: coefficient allocation failed");
+    }
+    std::vector<float> all;
+    all.insert(all.end(), rx.begin(), rx.end());
+    all.insert(all.end(), ry.begin(), ry.end());
+    all.insert(all.end(), rz.begin(), rz.end());
+    hipError_t e = hipMemcpy(h->coef, all.data(), tot * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(h->coef);
+        delete h;
+        return hip_fail(e, "project3d: coefficient upload");
+    }
+    const float* b = h->coef;
+    h->cx = {b, b + (nx + 1), b + 2 * (nx + 1)};
+    b += 3 * (nx + 1);
+    h->cy = {b, b + (ny + 1), b + 2 * (ny + 1)};
+    b += 3 * (ny + 1);
+    h->cz = {b, b + (nz + 1), b + 2 * (nz + 1)};
+    *out = h;
+    return WHHD_OK;
+}
+
+int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, float* xo, float* yo,
+                  float* zo, const float* xv, const float* yv, const float* zv, hipStream_t st) {
+    if (!h || !qx || !qy || !qz || !xv || !yv || !zv)
+        return fail(WHHD_INVALID_ARG, "project3d_run: null argument");
+    if ((xo || yo || zo) && !(xo && yo && zo))
+        return fail(WHHD_INVALID_ARG, "project3d_run: give all three or none of the v_out arrays");
+    Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz};
+    dim3 block(64, 4, 1);
+    dim3 grid((h->ny + 63) / 64, (h->nx + 3) / 4, h->nz);
+    hipLaunchKernelGGL(project3d_kernel, grid, block, 0, st, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "project3d kernel");
+}
+
+int project2d_create(int nx, int ny, double dx, double dy, int wt, char bc, whhd_project2d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "project2d_create: null handle pointer");
+    *out = nullptr;
+    if (!is_pow2(nx) || !is_pow2(ny))
+        return fail(WHHD_NOT_POW2, "project2d: cell counts must be powers of two (>= 4)");
+    if (!(dx > 0 && dy > 0)) return fail(WHHD_INVALID_ARG, "project2d: spacing must be > 0");
+    if (bc != 'd')
+        return fail(WHHD_UNSUPPORTED_WAVELET,
+                    "project2d: only the divergence bc 'd' (velocity bc 'n') is implemented");
+    int st = check_wavelet(wt, bc);
+    if (st != WHHD_OK) return st;
+    if (proj_wavelet(wt) < 0) return fail(WHHD_UNSUPPORTED_WAVELET, "project2d: no projection type");
+    std::vector<float> rx, ry;
+    if ((st = axis_coef(rx, nx, dx, wt, bc, false)) != WHHD_OK) return st;
+    if ((st = axis_coef(ry, ny, dy, wt, bc, false)) != WHHD_OK) return st;
+    auto* h = new whhd_project2d_s;
+    h->nx = nx; h->ny = ny;
+    const size_t tot = rx.size() + ry.size();
+    if (hipMalloc(&h->coef, tot * sizeof(float)) != hipSuccess) {
+        delete h;
+        return fail(WHHD_OOM, "project2d: coefficient allocation failed");
+    }
+    std::vector<float> all(rx);
+    all.insert(all.end(), ry.begin(), ry.end());
+    hipError_t e = hipMemcpy(h->coef, all.data(), tot * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(h->coef);
+        delete h;
+        return hip_fail(e, "project2d: coefficient upload");
+    }
+    const float* b = h->coef;
+    h->cx = {b, b + (nx + 1), b + 2 * (nx + 1)};
+    b += 3 * (nx + 1);
+    h->cy = {b, b + (ny + 1), b + 2 * (ny + 1)};
+    *out = h;
+    return WHHD_OK;
+}
+
+int project2d_run(const whhd_project2d_s* h, float* qz, const float* xv, const float* yv,
+                  hipStream_t st) {
+    if (!h || !qz || !xv || !yv) return fail(WHHD_INVALID_ARG, "project2d_run: null argument");
+    Proj2Args a{qz, xv, yv, h->cx, h->cy, h->nx, h->ny};
+    dim3 block(64, 4, 1);
+    dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, 1);
+    hipLaunchKernelGGL(project2d_kernel, grid, block, 0, st, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "project2d kernel");
+}
+
+}  // namespace whhd
+
+extern "C" {
+
+WHHD_API int whhd_project3d_create(int nx, int ny, int nz, double dx, double dy, double dz,
+                                   int wavelet_div, char bc_div, whhd_project3d_t* out) {
+    return whhd::project3d_create(nx, ny, nz, dx, dy, dz, wavelet_div, bc_div, out);
+}
+
+WHHD_API int whhd_project3d_run(whhd_project3d_t h, float* qx, float* qy, float* qz,
+                                float* xv_out, float* yv_out, float* zv_out, const float* xv,
+                                const float* yv, const float* zv, void* stream) {
+    return whhd::project3d_run(h, qx, qy, qz, xv_out, yv_out, zv_out, xv, yv, zv,
+                               (hipStream_t)stream);
+}
+
+WHHD_API void whhd_project3d_destroy(whhd_project3d_t h) {
+    if (!h) return;
+    hipFree(h->coef);
+    delete h;
+}
+
+WHHD_API int whhd_project2d_create(int nx, int ny, double dx, double dy, int wavelet_div,
+                                   char bc_div, whhd_project2d_t* out) {
+    return whhd::project2d_create(nx, ny, dx, dy, wavelet_div, bc_div, out);
+}
+
+WHHD_API int whhd_project2d_run(whhd_project2d_t h, float* qz, const float* xv, const float* yv,
+                                void* stream) {
+    return whhd::project2d_run(h, qz, xv, yv, (hipStream_t)stream);
+}
+
+WHHD_API void whhd_project2d_destroy(whhd_project2d_t h) {
+    if (!h) return;
+    hipFree(h->coef);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,296 @@
+// Potential-recovery drivers: CuWaveletPotentialRecover3D / 2D
+// (WaveletPotentials/cuWaveletPotentialRecover3D.cu:6-239,
+//  WaveletPotentials/cuWaveletPotentialRecover2D.cu:6-37).
+//
+// Per component, the velocity's own axis uses the divergence wavelet with
+// bc_div = proj_bc(bc) and the other axes the curl wavelet with bc; the
+// potential's own axis uses the curl wavelet and the other axes the
+// divergence wavelet.  The handle owns its scratch (one array per velocity
+// component, allocated at create) and enqueues everything on the caller's
+// stream without synchronizing, so a solve can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include "whhd_internal.h"
+
+struct whhd_project3d_s;
+struct whhd_project2d_s;
+
+namespace whhd {
+int project3d_create(int, int, int, double, double, double, int, char, whhd_project3d_s**);
+int project3d_run(const whhd_project3d_s*, float*, float*, float*, float*, float*, float*,
+                  const float*, const float*, const float*, hipStream_t);
+int project2d_create(int, int, double, double, int, char, whhd_project2d_s**);
+int project2d_run(const whhd_project2d_s*, float*, const float*, const float*, hipStream_t);
+}  // namespace whhd
+
+extern "C" void whhd_project3d_destroy(whhd_project3d_s* h);
+extern "C" void whhd_project2d_destroy(whhd_project2d_s* h);
+
+struct whhd_recover3d_s {
+    int nx, ny, nz;
+    int lx, ly, lz;
+    char bc, bd;
+    int wc, wd;
+    whhd_project3d_s* proj = nullptr;
+    float* tmp[3] = {nullptr, nullptr, nullptr};
+};
+
+struct whhd_recover2d_s {
+    int nx, ny, lx, ly;
+    char bc, bd;
+    int wc, wd;
+    whhd_project2d_s* proj = nullptr;
+    float* tmp[2] = {nullptr, nullptr};
+};
+
+namespace whhd {
+
+#define TRY(e) do { int s_ = (e); if (s_ != WHHD_OK) return s_; } while (0)
+
+// cuCutCoefs (WaveletPotentials/cuWaveletPotentialRecover3D.cu:123-197) over the
+// (nx+1, ny+1, nz+1) node grid; writes that would fall outside an array are
+// skipped.
+__global__ void cut_kernel(float* qx, float* qy, float* qz, int nx, int ny, int nz, int lx, int ly,
+                           int lz, int cut) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y > ny || x > nx) return;
+    const int64_t NX = nx, NY = ny, NZ = nz;
+    const int64_t iqx = y + (x - 1) * (NY + 1) + z * NX * (NY + 1);
+    const int64_t iqy = (y - 1) + x * NY + z * (NX + 1) * NY;
+    const int64_t iqz = y + x * (NY + 1) + (z - 1) * (NX + 1) * (NY + 1);
+    const int64_t nqx = NX * (NY + 1) * (NZ + 1), nqy = (NX + 1) * NY * (NZ + 1),
+                  nqz = (NX + 1) * (NY + 1) * NZ;
+    auto bl = [](int v) { return v > 0 ? 32 - __clz(v) : 0; };
+    const int jy = ly - bl(y - 1 > 0 ? y - 1 : 0);
+    const int jx = lx - bl(x - 1 > 0 ? x - 1 : 0);
+    const int jz = lz - bl(z - 1 > 0 ? z - 1 : 0);
+    bool zx = false, zy = false, zz = false;
+    if (jx < cut) { zx = true; if (y > 0) zy = true; if (z > 0) zz = true; }
+    if (jy < cut) { zy = true; if (x > 0) zx = true; if (z > 0) zz = true; }
+    if (jz < cut) { zz = true; if (x > 0) zx = true; if (y > 0) zy = true; }
+    if (zx && iqx >= 0 && iqx < nqx) qx[iqx] = 0.f;
+    if (zy && iqy >= 0 && iqy < nqy) qy[iqy] = 0.f;
+    if (zz && iqz >= 0 && iqz < nqz) qz[iqz] = 0.f;
+}
+
+static int fwd3(const whhd_recover3d_s* h, float* t[3], const float* xv, const float* yv,
+                const float* zv, hipStream_t s) {
+    const int X = h->nx, Y = h->ny, Z = h->nz;
+    TRY(transform_lines<float>(t[0], xv, X + 1, Y, Z, h->lx, 'x', h->bd, h->wd, true, s));
+    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->ly, 'y', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lz, 'z', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[1], yv, X, Y + 1, Z, h->lx, 'x', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->ly, 'y', h->bd, h->wd, true, s));
+    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lz, 'z', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[2], zv, X, Y, Z + 1, h->lx, 'x', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->ly, 'y', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lz, 'z', h->bd, h->wd, true, s));
+    return WHHD_OK;
+}
+
+// Forward (fwd=true) or inverse transform of a potential (qx, qy, qz).
+static int pot3(const whhd_recover3d_s* h, float* dx, float* dy, float* dz, const float* sx,
+                const float* sy, const float* sz, bool fwd, hipStream_t s) {
+    const int X = h->nx, Y = h->ny, Z = h->nz;
+    TRY(transform_lines<float>(dx, sx, X, Y + 1, Z + 1, h->lx, 'x', h->bc, h->wc, fwd, s));
+    TRY(transform_lines<float>(dx, dx, X, Y + 1, Z + 1, h->ly, 'y', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dx, dx, X, Y + 1, Z + 1, h->lz, 'z', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dy, sy, X + 1, Y, Z + 1, h->lx, 'x', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dy, dy, X + 1, Y, Z + 1, h->ly, 'y', h->bc, h->wc, fwd, s));
+    TRY(transform_lines<float>(dy, dy, X + 1, Y, Z + 1, h->lz, 'z', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dz, sz, X + 1, Y + 1, Z, h->lx, 'x', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dz, dz, X + 1, Y + 1, Z, h->ly, 'y', h->bd, h->wd, fwd, s));
+    TRY(transform_lines<float>(dz, dz, X + 1, Y + 1, Z, h->lz, 'z', h->bc, h->wc, fwd, s));
+    return WHHD_OK;
+}
+
+static int inv_vel3(const whhd_recover3d_s* h, float* t[3], hipStream_t s) {
+    const int X = h->nx, Y = h->ny, Z = h->nz;
+    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lx, 'x', h->bd, h->wd, false, s));
+    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->ly, 'y', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lz, 'z', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lx, 'x', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->ly, 'y', h->bd, h->wd, false, s));
+    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lz, 'z', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lx, 'x', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->ly, 'y', h->bc, h->wc, false, s));
+    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lz, 'z', h->bd, h->wd, false, s));
+    return WHHD_OK;
+}
+
+static bool pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
+
+int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc, int wc,
+                     int wd, whhd_recover3d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "recover3d_create: null handle pointer");
+    *out = nullptr;
+    if (!pow2(nx) || !pow2(ny) || !pow2(nz))
+        return fail(WHHD_NOT_POW2, "recover3d: cell counts must be powers of two (>= 4)");
+    const char bd = proj_bc(bc);
+    TRY(check_wavelet(wc, bc));
+    TRY(check_wavelet(wd, bd));
+    TRY(check_line(nx, ilog2(nx), wc));
+    TRY(check_line(nx + 1, ilog2(nx + 1), wd));
+    TRY(check_line(ny, ilog2(ny), wc));
+    TRY(check_line(ny + 1, ilog2(ny + 1), wd));
+    TRY(check_line(nz, ilog2(nz), wc));
+    TRY(check_line(nz + 1, ilog2(nz + 1), wd));
+    auto* h = new whhd_recover3d_s;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    h->lx = ilog2(nx); h->ly = ilog2(ny); h->lz = ilog2(nz);
+    h->bc = bc; h->bd = bd; h->wc = wc; h->wd = wd;
+    int st = project3d_create(nx, ny, nz, dx, dy, dz, wd, bd, &h->proj);
+    if (st != WHHD_OK) {
+        delete h;
+        return st;
+    }
+    const size_t sz[3] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz,
+                          (size_t)nx * ny * (nz + 1)};
+    for (int i = 0; i < 3; ++i)
+        if (hipMalloc(&h->tmp[i], sz[i] * sizeof(float)) != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipFree(h->tmp[j]);
+            whhd_project3d_destroy(h->proj);
+            delete h;
+            return fail(WHHD_OOM, "recover3d: scratch allocation failed");
+        }
+    *out = h;
+    return WHHD_OK;
+}
+
+int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float* xo, float* yo,
+                    float* zo, const float* xv, const float* yv, const float* zv, hipStream_t s) {
+    if (!h || !qx || !qy || !qz || !xv || !yv || !zv)
+        return fail(WHHD_INVALID_ARG, "recover3d_solve: null argument");
+    const bool vout = xo || yo || zo;
+    if (vout && !(xo && yo && zo))
+        return fail(WHHD_INVALID_ARG, "recover3d_solve_proj: all three v_out arrays are needed");
+    float* t[3] = {h->tmp[0], h->tmp[1], h->tmp[2]};
+    if (vout) { t[0] = xo; t[1] = yo; t[2] = zo; }
+    TRY(fwd3(h, t, xv, yv, zv, s));
+    if (vout) TRY(project3d_run(h->proj, qx, qy, qz, t[0], t[1], t[2], t[0], t[1], t[2], s));
+    else TRY(project3d_run(h->proj, qx, qy, qz, nullptr, nullptr, nullptr, t[0], t[1], t[2], s));
+    TRY(pot3(h, qx, qy, qz, qx, qy, qz, false, s));
+    if (vout) TRY(inv_vel3(h, t, s));
+    return WHHD_OK;
+}
+
+int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const float* qx,
+                  const float* qy, const float* qz, int level, hipStream_t s) {
+    if (!h || !ox || !oy || !oz || !qx || !qy || !qz)
+        return fail(WHHD_INVALID_ARG, "recover3d_cut: null argument");
+    TRY(pot3(h, ox, oy, oz, qx, qy, qz, true, s));
+    dim3 block(64, 4, 1);
+    dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, h->nz + 1);
+    hipLaunchKernelGGL(cut_kernel, grid, block, 0, s, ox, oy, oz, h->nx, h->ny, h->nz, h->lx, h->ly,
+                       h->lz, level);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "cut kernel");
+    TRY(pot3(h, ox, oy, oz, ox, oy, oz, false, s));
+    return WHHD_OK;
+}
+
+int recover2d_create(int nx, int ny, double dx, double dy, char bc, int wc, int wd,
+                     whhd_recover2d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "recover2d_create: null handle pointer");
+    *out = nullptr;
+    if (!pow2(nx) || !pow2(ny))
+        return fail(WHHD_NOT_POW2, "recover2d: cell counts must be powers of two (>= 4)");
+    const char bd = proj_bc(bc);
+    TRY(check_wavelet(wc, bc));
+    TRY(check_wavelet(wd, bd));
+    TRY(check_line(nx, ilog2(nx), wc));
+    TRY(check_line(nx + 1, ilog2(nx + 1), wd));
+    TRY(check_line(ny, ilog2(ny), wc));
+    TRY(check_line(ny + 1, ilog2(ny + 1), wd));
+    auto* h = new whhd_recover2d_s;
+    h->nx = nx; h->ny = ny; h->lx = ilog2(nx); h->ly = ilog2(ny);
+    h->bc = bc; h->bd = bd; h->wc = wc; h->wd = wd;
+    int st = project2d_create(nx, ny, dx, dy, wd, bd, &h->proj);
+    if (st != WHHD_OK) {
+        delete h;
+        return st;
+    }
+    const size_t sz[2] = {(size_t)(nx + 1) * ny, (size_t)nx * (ny + 1)};
+    for (int i = 0; i < 2; ++i)
+        if (hipMalloc(&h->tmp[i], sz[i] * sizeof(float)) != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipFree(h->tmp[j]);
+            whhd_project2d_destroy(h->proj);
+            delete h;
+            return fail(WHHD_OOM, "recover2d: scratch allocation failed");
+        }
+    *out = h;
+    return WHHD_OK;
+}
+
+int recover2d_solve(whhd_recover2d_s* h, float* qz, const float* xv, const float* yv, hipStream_t s) {
+    if (!h || !qz || !xv || !yv) return fail(WHHD_INVALID_ARG, "recover2d_solve: null argument");
+    const int X = h->nx, Y = h->ny;
+    float* tx = h->tmp[0];
+    float* ty = h->tmp[1];
+    TRY(transform_lines<float>(tx, xv, X + 1, Y, 1, h->lx, 'x', h->bd, h->wd, true, s));
+    TRY(transform_lines<float>(tx, tx, X + 1, Y, 1, h->ly, 'y', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(ty, yv, X, Y + 1, 1, h->lx, 'x', h->bc, h->wc, true, s));
+    TRY(transform_lines<float>(ty, ty, X, Y + 1, 1, h->ly, 'y', h->bd, h->wd, true, s));
+    TRY(project2d_run(h->proj, qz, tx, ty, s));
+    TRY(transform_lines<float>(qz, qz, X + 1, Y + 1, 1, h->lx, 'x', h->bd, h->wd, false, s));
+    TRY(transform_lines<float>(qz, qz, X + 1, Y + 1, 1, h->ly, 'y', h->bd, h->wd, false, s));
+    return WHHD_OK;
+}
+
+}  // namespace whhd
+
+extern "C" {
+
+WHHD_API int whhd_recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                                   int wavelet_curl, int wavelet_div, whhd_recover3d_t* out) {
+    return whhd::recover3d_create(nx, ny, nz, dx, dy, dz, bc, wavelet_curl, wavelet_div, out);
+}
+
+WHHD_API int whhd_recover3d_solve(whhd_recover3d_t h, float* qx, float* qy, float* qz,
+                                  const float* xv, const float* yv, const float* zv, void* stream) {
+    return whhd::recover3d_solve(h, qx, qy, qz, nullptr, nullptr, nullptr, xv, yv, zv,
+                                 (hipStream_t)stream);
+}
+
+WHHD_API int whhd_recover3d_solve_proj(whhd_recover3d_t h, float* qx, float* qy, float* qz,
+                                       float* xv_out, float* yv_out, float* zv_out, const float* xv,
+                                       const float* yv, const float* zv, void* stream) {
+    if (!xv_out || !yv_out || !zv_out)
+        return whhd::fail(WHHD_INVALID_ARG, "recover3d_solve_proj: v_out arrays are required");
+    return whhd::recover3d_solve(h, qx, qy, qz, xv_out, yv_out, zv_out, xv, yv, zv,
+                                 (hipStream_t)stream);
+}
+
+WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out, float* qz_out,
+                                const float* qx, const float* qy, const float* qz, int level,
+                                void* stream) {
+    return whhd::recover3d_cut(h, qx_out, qy_out, qz_out, qx, qy, qz, level, (hipStream_t)stream);
+}
+
+WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h) {
+    if (!h) return;
+    for (float* p : h->tmp) hipFree(p);
+    whhd_project3d_destroy(h->proj);
+    delete h;
+}
+
+WHHD_API int whhd_recover2d_create(int nx, int ny, double dx, double dy, char bc, int wavelet_curl,
+                                   int wavelet_div, whhd_recover2d_t* out) {
+    return whhd::recover2d_create(nx, ny, dx, dy, bc, wavelet_curl, wavelet_div, out);
+}
+
+WHHD_API int whhd_recover2d_solve(whhd_recover2d_t h, float* qz, const float* xv, const float* yv,
+                                  void* stream) {
+    return whhd::recover2d_solve(h, qz, xv, yv, (hipStream_t)stream);
+}
+
+WHHD_API void whhd_recover2d_destroy(whhd_recover2d_t h) {
+    if (!h) return;
+    for (float* p : h->tmp) hipFree(p);
+    whhd_project2d_destroy(h->proj);
+    delete h;
+}
+
+}  // extern "C"

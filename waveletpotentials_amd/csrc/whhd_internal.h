@@ -1,0 +1,60 @@
+// Internal declarations shared by the HIP translation units of libwhhd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/whhd.h"
+
+namespace whhd {
+
+enum {
+    WT_CDF3_9 = 0, WT_CDF3_11, WT_CDF3_5, WT_CDF2_10, WT_CDF2_12, WT_CDF2_6, WT_CDF4_4,
+    WT_CDF4_6, WT_CDF4_8, WT_CDF4_10, WT_CDF2_8, WT_CDF3_7, WT_CDF26_44, WT_CDF28_46,
+    WT_CDF44_26, WT_CDF46_28, WT_CDF2_0, WT_CDF1_1, WT_CDF1_7
+};
+
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
+// Geometry of one line-transform launch.
+struct LineGeom {
+    int n = 0;        // samples per line
+    int levels = 0;   // floor(log2(n))
+    int lds_row = 0;  // padded LDS row length
+    int tile = 0;     // lines per workgroup
+    int64_t lines = 0;          // contiguous: total lines; strided: lines per outer index (ny)
+    int outer = 1;              // strided: number of outer indices
+    int64_t sample_stride = 1;  // strided: distance between consecutive samples of a line
+    int64_t outer_stride = 0;   // strided: distance between outer indices
+};
+
+// Launch shape of the line-transform kernels.
+struct TileConfig {
+    int contig_tile = 4;    // lines per workgroup, contiguous (y) lines
+    int strided_tile = 16;  // consecutive y per workgroup, strided (x, z) lines
+    int threads = 256;
+};
+
+TileConfig default_tile(int n, size_t elem, bool strided);
+
+// Error plumbing: set the thread's last-error message and return `status`.
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+int check_wavelet(int wt, char bc);
+int check_line(int n, int levels, int wt);
+int ilog2(int n);
+
+template <class T>
+int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, char axis, char bc,
+                    int wt, bool fwd, hipStream_t stream, const TileConfig* cfg = nullptr);
+
+// Gram table (8 rows of n) of the synthesis basis of (wt, bc) into device memory.
+int gram_table_device(double* out, int wt, char bc, int n, hipStream_t stream);
+
+char proj_bc(char bc);
+int proj_wavelet(int wt);
+
+}  // namespace whhd
