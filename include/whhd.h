@@ -61,6 +61,11 @@ WHHD_API char whhd_proj_bc(char bc);
 WHHD_API int whhd_proj_wavelet(int wavelet);
 /* 1 if (wavelet, bc) has a kernel (WaveletPotentials/cuWHHDForward.cu:744-1080) */
 WHHD_API int whhd_wavelet_supported(int wavelet, char bc);
+/* Process-wide choice of line-transform kernels (no reference counterpart; the
+ * reference has a single kernel family): 0 = line-parallel register kernels
+ * where one exists for the (wavelet, bc, length) and dst != src, windowed
+ * LDS-tile kernels otherwise (default); 1 = windowed kernels only. */
+WHHD_API int whhd_set_line_kernel(int mode);
 
 /* ---- 1-D line transforms ---------------------------------------------- */
 /* CuWHHDForward::Solve1D(dst, src, int3 dim, level, direction, bc, type, is_forward)
@@ -114,7 +119,8 @@ WHHD_API void whhd_project2d_destroy(whhd_project2d_t h);
 typedef struct whhd_recover3d_s* whhd_recover3d_t;
 /* CuWaveletPotentialRecover3D(int3 dim, DTYPE3 dx, char bc, WaveletType curl,
  * WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover3D.cuh:12,
- * .cu:6-12).  Allocates the handle's scratch (three velocity-sized arrays). */
+ * .cu:6-12).  Allocates the handle's scratch: one velocity-sized and one
+ * potential-sized array per component (passes run out of place). */
 WHHD_API int whhd_recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz,
                                    char bc, int wavelet_curl, int wavelet_div,
                                    whhd_recover3d_t* out);

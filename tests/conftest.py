@@ -27,3 +27,13 @@ def whhd():
     import waveletpotentials_amd as W
     W.lib()  # raises if libwhhd.so is missing: there is no fallback path
     return W
+
+
+@pytest.fixture(params=[0, 1], ids=["line_parallel", "windowed"])
+def kernel_mode(request, whhd):
+    """Run a GPU test with the line-parallel kernels enabled (where they exist)
+    and again with the windowed tile kernels only."""
+    L = whhd.lib()
+    assert L.whhd_set_line_kernel(request.param) == 0
+    yield request.param
+    L.whhd_set_line_kernel(0)

@@ -38,8 +38,9 @@ def alloc_q(n):
             torch.full((nz, nx + 1, ny + 1), np.nan, device="cuda"))
 
 
-@pytest.mark.parametrize("n", [(8, 8, 8), (16, 16, 16), (32, 32, 32), (16, 8, 32), (64, 64, 64)])
-def test_recover3d_matches_oracle(whhd, orc, n):
+@pytest.mark.parametrize("n", [(8, 8, 8), (16, 16, 16), (32, 32, 32), (16, 8, 32), (64, 64, 64),
+                               (128, 32, 64)])
+def test_recover3d_matches_oracle(whhd, orc, kernel_mode, n):
     dx = tuple(1.0 / v for v in n)
     q = I.spectral_potential3d(n, max(2, min(n) // 4), I.seed_for(1, n[0] + n[2]))
     xv, yv, zv = I.curl3d(*q, dx)
@@ -56,7 +57,7 @@ def test_recover3d_matches_oracle(whhd, orc, n):
 
 
 @pytest.mark.parametrize("N", [16, 32])
-def test_recover3d_proj_variant(whhd, orc, N):
+def test_recover3d_proj_variant(whhd, orc, kernel_mode, N):
     n = (N, N, N)
     dx = (1.0 / N,) * 3
     rng = np.random.default_rng(N)
@@ -96,7 +97,7 @@ def test_project3d_matches_oracle(whhd, orc):
         np.testing.assert_allclose(g, r, rtol=2e-5, atol=2e-6 * np.abs(r).max())
 
 
-def test_cut_matches_oracle(whhd, orc):
+def test_cut_matches_oracle(whhd, orc, kernel_mode):
     N = 16
     n = (N, N, N)
     (xv, yv, zv), _, dx = I.velocity3d(n, 4, 7)
@@ -126,7 +127,7 @@ def test_recover2d_matches_oracle(whhd, orc, N):
     assert rel([got], [psi]) < TOL_Q
 
 
-def test_recover3d_full_size_properties(whhd, orc):
+def test_recover3d_full_size_properties(whhd, orc, kernel_mode):
     """256^3 (the headline workload): FP64 oracle parity plus size-independent checks."""
     N = 256
     n = (N, N, N)

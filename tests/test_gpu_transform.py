@@ -67,14 +67,15 @@ def test_transform_matches_oracle(whhd, orc, wt, bc, axis):
 
 @pytest.mark.parametrize("wt,bc", [("cdf3.7", "n"), ("cdf4.6", "d")])
 @pytest.mark.parametrize("axis", ["y", "x", "z"])
-def test_transform_config_sizes(whhd, orc, wt, bc, axis):
-    """The two (type, bc) pairs of the recovery path at lengths up to 1025,
-    with ragged line counts so partial tiles are exercised."""
+def test_transform_config_sizes(whhd, orc, kernel_mode, wt, bc, axis):
+    """The two (type, bc) pairs of the recovery path at lengths 2^2..2^10 (+1),
+    out of place (line-parallel kernels for 2^5..2^8) with ragged line counts
+    so partial tiles and partial wavefronts are exercised."""
     rng = np.random.default_rng(5)
     T = whhd.WaveletType
     wtype = T.WT_CDF3_7 if wt == "cdf3.7" else T.WT_CDF4_6
     fw = whhd.CuWHHDForward.GetInstance()
-    for levels in (7, 8, 10):
+    for levels in (2, 3, 4, 5, 6, 7, 8, 10):
         n = line_len(wt, levels)
         shape = shape_for(axis, n, 37, 3)
         x = make(rng, shape, axis, wt, bc)

@@ -23,6 +23,7 @@ SIGNATURES = {
     "whhd_proj_bc": (_C, [_C]),
     "whhd_proj_wavelet": (_I, [_I]),
     "whhd_wavelet_supported": (_I, [_I, _C]),
+    "whhd_set_line_kernel": (_I, [_I]),
     "whhd_transform1d": (_I, [_P, _P, _I, _I, _I, _I, _C, _C, _I, _I, _P]),
     "whhd_transform1d_f64": (_I, [_P, _P, _I, _I, _I, _I, _C, _C, _I, _I, _P]),
     "whhd_gram_table": (_I, [_P, _I, _C, _I, _P]),

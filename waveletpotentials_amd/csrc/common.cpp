@@ -1,4 +1,5 @@
 // Status codes, argument checks and small shared helpers of libwhhd.so.
+#include <atomic>
 #include <cstdio>
 #include <string>
 
@@ -9,6 +10,8 @@ namespace whhd {
 namespace {
 thread_local std::string g_last_error;
 }
+
+std::atomic<int> g_line_kernel_mode{0};
 
 int fail(int status, const std::string& msg) {
     g_last_error = msg;
@@ -167,6 +170,12 @@ WHHD_API const char* whhd_version(void) { return "whhd-mi355x 0.1 (gfx950)"; }
 WHHD_API char whhd_proj_bc(char bc) { return whhd::proj_bc(bc); }
 
 WHHD_API int whhd_proj_wavelet(int wt) { return whhd::proj_wavelet(wt); }
+
+WHHD_API int whhd_set_line_kernel(int mode) {
+    if (mode != 0 && mode != 1) return whhd::fail(WHHD_INVALID_ARG, "line kernel mode must be 0 or 1");
+    whhd::g_line_kernel_mode.store(mode);
+    return WHHD_OK;
+}
 
 WHHD_API int whhd_wavelet_supported(int wt, char bc) {
     return whhd::check_wavelet(wt, bc) == WHHD_OK ? 1 : 0;

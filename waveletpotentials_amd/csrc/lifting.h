@@ -17,12 +17,47 @@ namespace whhd {
 
 #define WHHD_DEV __device__ __forceinline__
 
+// Whole-wavefront shifts by one lane on the VALU (DPP wave_shr:1 / wave_shl:1);
+// a lane without a source keeps its own value.  Shifts by m lanes chain m of
+// them, so edge lanes may hold values that differ from a direct m-lane
+// shuffle; they are halo lanes, whose results are never written.
+WHHD_DEV int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+WHHD_DEV int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+
+WHHD_DEV float shr1(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+WHHD_DEV float shl1(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+WHHD_DEV double shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shr1((int)(b & 0xffffffffLL)), hi = dpp_shr1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+WHHD_DEV double shl1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shl1((int)(b & 0xffffffffLL)), hi = dpp_shl1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// lane i receives lane i-m (lane_up) / lane i+m (lane_dn)
 template <class T>
-WHHD_DEV T lane_up(T v, int m) { return __shfl_up(v, (unsigned)m, 64); }
+WHHD_DEV T lane_up(T v, int m) {
+    for (int i = 0; i < m; ++i) v = shr1(v);
+    return v;
+}
 template <class T>
-WHHD_DEV T lane_dn(T v, int m) { return __shfl_down(v, (unsigned)m, 64); }
-template <class T>
-WHHD_DEV T lane_get(T v, int src) { return __shfl(v, src, 64); }
+WHHD_DEV T lane_dn(T v, int m) {
+    for (int i = 0; i < m; ++i) v = shl1(v);
+    return v;
+}
+// value of lane `src` (wave-uniform, compile-time) broadcast to every lane
+WHHD_DEV float lane_get(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+WHHD_DEV double lane_get(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 // ---------------------------------------------------------------------------
 // Index / sign reflections.  Forward forms receive a = 2k, d = 2k+1; inverse

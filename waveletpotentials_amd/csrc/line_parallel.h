@@ -1,0 +1,295 @@
+// Line-parallel multi-level transforms: one lane owns one whole line, a
+// wavefront advances 64 lines in lockstep.  Every sample position is a
+// compile-time constant (the level loop, the pair blocks and the reflections
+// are unrolled for a fixed line length), so a level's approximations live in
+// VGPRs, the boundary reflections and signs fold into constants, and the
+// lifting statements become straight FMA chains.  Global traffic is one read
+// and one write per sample: strided lines (x and z axes) give each load a
+// full row of 64 consecutive floats, contiguous lines (y axis) give each lane
+// a run of consecutive samples of its own line.
+//
+// The lifting statements themselves are the same templates as the windowed
+// kernel (lifting.h), instantiated on `Arr`, a static array of pairs whose
+// neighbour shift reads the pair's own value at the array ends.  Pairs are
+// processed in blocks of B with `halo` extra pairs on each side, so every
+// written pair only depends on pairs inside its block (the same cone argument
+// as the 64-lane windows).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+
+#include "lifting.h"
+
+namespace whhd {
+namespace lp {
+
+template <class S, int NP>
+struct Arr {
+    S v[NP];
+    WHHD_DEV Arr() {}
+    WHHD_DEV Arr(double c) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) v[i] = S(c);
+    }
+    WHHD_DEV S& operator[](int i) { return v[i]; }
+    WHHD_DEV const S& operator[](int i) const { return v[i]; }
+#define WHHD_ARR_OP(op)                                                       \
+    WHHD_DEV Arr& operator op##=(const Arr& o) {                              \
+        _Pragma("unroll") for (int i = 0; i < NP; ++i) v[i] op## = o.v[i];    \
+        return *this;                                                         \
+    }                                                                         \
+    WHHD_DEV friend Arr operator op(const Arr& x, const Arr& y) {             \
+        Arr r;                                                                \
+        _Pragma("unroll") for (int i = 0; i < NP; ++i) r.v[i] = x.v[i] op y.v[i]; \
+        return r;                                                             \
+    }
+    WHHD_ARR_OP(+)
+    WHHD_ARR_OP(-)
+    WHHD_ARR_OP(*)
+    WHHD_ARR_OP(/)
+#undef WHHD_ARR_OP
+    WHHD_DEV friend Arr operator-(const Arr& x) {
+        Arr r;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) r.v[i] = -x.v[i];
+        return r;
+    }
+};
+
+// pair i receives pair i-m / i+m; pairs without a source keep their own value
+// (found by argument-dependent lookup from the lifting templates)
+template <class S, int NP>
+WHHD_DEV Arr<S, NP> lane_up(const Arr<S, NP>& x, int m) {
+    Arr<S, NP> r;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) r.v[i] = (i - m >= 0) ? x.v[i - m] : x.v[i];
+    return r;
+}
+template <class S, int NP>
+WHHD_DEV Arr<S, NP> lane_dn(const Arr<S, NP>& x, int m) {
+    Arr<S, NP> r;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) r.v[i] = (i + m < NP) ? x.v[i + m] : x.v[i];
+    return r;
+}
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <int B, int E, class F>
+WHHD_DEV void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+constexpr int kBlock = 16;  // pairs written per block
+
+// ---- I/O of one lane's line ----------------------------------------------
+// sample i of the lane's line is at base + i * step
+struct LineIO {
+    const float* src;
+    float* dst;
+    int64_t base;
+    int64_t step;
+    bool valid;
+    WHHD_DEV float load(int i) const { return src[base + (int64_t)i * step]; }
+    WHHD_DEV void store(int i, float v) const {
+        if (valid) dst[base + (int64_t)i * step] = v;
+    }
+};
+
+// ---- forward --------------------------------------------------------------
+// One forward level: the 2^(l+1)+EXT samples come from `in` (or the line when
+// TOP), approximations go to `out`, details to the line at [W, 2W-EXT).
+template <class W, int l, int EXT, bool TOP>
+WHHD_DEV void fwd_level(const LineIO& io, const float* in, float* out) {
+    constexpr int H = W::halo;
+    constexpr int L = 1 << (l + 1);
+    constexpr int Wd = L / 2 + EXT;
+    constexpr int NP = kBlock + 2 * H;
+    constexpr int NB = (Wd + kBlock - 1) / kBlock;
+    static_for<0, NB>([&](auto bi) {
+        constexpr int b0 = decltype(bi)::value * kBlock;
+        Arr<float, NP> A, D, SA, SD;
+        static_for<0, NP>([&](auto ji) {
+            constexpr int j = decltype(ji)::value;
+            constexpr int k = b0 - H + j;
+            int ai = 2 * k, di = 2 * k + 1;
+            float sa, sd;
+            W::ix::fwd(ai, di, sa, sd, L);
+            di = di < 0 ? 0 : di;
+            ai = ai < 0 ? 0 : (ai > L + EXT - 1 ? L + EXT - 1 : ai);
+            di = di > L + EXT - 1 ? L + EXT - 1 : di;
+            SA[j] = sa;
+            SD[j] = sd;
+            if constexpr (k < Wd + H) {
+                if constexpr (TOP) {
+                    A[j] = io.load(ai);
+                    D[j] = io.load(di);
+                } else {
+                    A[j] = in[ai];
+                    D[j] = in[di];
+                }
+            } else {
+                A[j] = 0.f;
+                D[j] = 0.f;
+            }
+        });
+        W::up::fwd(A, D, SA, SD);
+        static_for<H, H + kBlock>([&](auto ji) {
+            constexpr int j = decltype(ji)::value;
+            constexpr int k = b0 - H + j;
+            if constexpr (k < Wd) {
+                out[k] = A[j];
+                if constexpr (k < Wd - EXT) io.store(Wd + k, D[j]);
+            }
+        });
+    });
+}
+
+template <class W, int l, int EXT>
+WHHD_DEV void fwd_levels_below(const LineIO& io, const float* in) {
+    // `in` holds the 2^(l+1)+EXT approximations produced by level l+1
+    if constexpr (l >= 2) {
+        float out[(1 << l) + EXT];
+        fwd_level<W, l, EXT, false>(io, in, out);
+        fwd_levels_below<W, l - 1, EXT>(io, out);
+    } else {
+        constexpr int M = W::m10;
+        static_for<0, M>([&](auto ri) {
+            constexpr int r = decltype(ri)::value;
+            float acc = 0.f;
+            static_for<0, M>([&](auto ci) {
+                constexpr int c = decltype(ci)::value;
+                acc += in[c] * float(W::l10[r * M + c]);
+            });
+            io.store(r, acc);
+        });
+    }
+}
+
+template <class W, int LEV, int EXT>
+WHHD_DEV void forward_line_lp(const LineIO& io) {
+    if constexpr (LEV > 2) {
+        float out[(1 << (LEV - 1)) + EXT];
+        fwd_level<W, LEV - 1, EXT, true>(io, nullptr, out);
+        fwd_levels_below<W, LEV - 2, EXT>(io, out);
+    } else {
+        float in[(1 << LEV) + EXT];
+        static_for<0, (1 << LEV) + EXT>([&](auto i) { in[decltype(i)::value] = io.load(decltype(i)::value); });
+        fwd_levels_below<W, 1, EXT>(io, in);
+    }
+}
+
+// ---- inverse --------------------------------------------------------------
+// One inverse level l: approximations [0, 2^l+EXT) from `in`, details from the
+// line at [2^l+EXT, 2^(l+1)+EXT); reconstructed samples to `out`, or to the
+// line when LAST.
+template <class W, int l, int EXT, bool LAST>
+WHHD_DEV void inv_level(const LineIO& io, const float* in, float* out) {
+    constexpr int H = W::halo;
+    constexpr int L = 1 << l;
+    constexpr int Wd = L + EXT;
+    constexpr int NP = kBlock + 2 * H;
+    constexpr int NB = (Wd + kBlock - 1) / kBlock;
+    static_for<0, NB>([&](auto bi) {
+        constexpr int b0 = decltype(bi)::value * kBlock;
+        Arr<float, NP> A, D, SA, SD;
+        static_for<0, NP>([&](auto ji) {
+            constexpr int j = decltype(ji)::value;
+            constexpr int k = b0 - H + j;
+            int ai = k, di = k;
+            float sa, sd;
+            W::ix::inv(ai, di, sa, sd, L);
+            ai = ai < 0 ? 0 : (ai > Wd - 1 ? Wd - 1 : ai);
+            di = di < 0 ? 0 : (di > L - 1 ? L - 1 : di);
+            SA[j] = sa;
+            SD[j] = sd;
+            if constexpr (k < L + H) {
+                A[j] = in[ai];
+                D[j] = io.load(L + EXT + di);
+            } else {
+                A[j] = 0.f;
+                D[j] = 0.f;
+            }
+        });
+        W::up::inv(A, D, SA, SD);
+        static_for<H, H + kBlock>([&](auto ji) {
+            constexpr int j = decltype(ji)::value;
+            constexpr int k = b0 - H + j;
+            if constexpr (k < L + EXT) {
+                if constexpr (LAST) io.store(2 * k, A[j]);
+                else out[2 * k] = A[j];
+            }
+            if constexpr (k < L) {
+                if constexpr (LAST) io.store(2 * k + 1, D[j]);
+                else out[2 * k + 1] = D[j];
+            }
+        });
+    });
+}
+
+template <class W, int l, int LEV, int EXT>
+WHHD_DEV void inv_levels_from(const LineIO& io, const float* in) {
+    // `in` holds the 2^l+EXT approximations of level l
+    if constexpr (l == LEV - 1) {
+        inv_level<W, l, EXT, true>(io, in, nullptr);
+    } else {
+        float out[(1 << (l + 1)) + EXT];
+        inv_level<W, l, EXT, false>(io, in, out);
+        inv_levels_from<W, l + 1, LEV, EXT>(io, out);
+    }
+}
+
+template <class W, int LEV, int EXT>
+WHHD_DEV void inverse_line_lp(const LineIO& io) {
+    constexpr int M = W::m10;
+    float c[M], a[M];
+    static_for<0, M>([&](auto i) { c[decltype(i)::value] = io.load(decltype(i)::value); });
+    static_for<0, M>([&](auto ri) {
+        constexpr int r = decltype(ri)::value;
+        float acc = 0.f;
+        static_for<0, M>([&](auto ci) {
+            constexpr int cc = decltype(ci)::value;
+            acc += c[cc] * float(W::il10[r * M + cc]);
+        });
+        a[r] = acc;
+    });
+    if constexpr (LEV > 2) {
+        inv_levels_from<W, 2, LEV, EXT>(io, a);
+    } else {
+        static_for<0, M>([&](auto i) { io.store(decltype(i)::value, a[decltype(i)::value]); });
+    }
+}
+
+// ---- kernel -----------------------------------------------------------------
+// Lines are numbered li = outer * lines_inner + inner; line li starts at
+// inner * inner_stride + outer * outer_stride and its samples are `step` apart.
+struct LpGeom {
+    int64_t total_lines;
+    int lines_inner;
+    int64_t inner_stride;
+    int64_t outer_stride;
+    int64_t step;
+};
+
+template <class W, int LEV, int EXT, bool FWD>
+__global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                                                 LpGeom g) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = li < g.total_lines;
+    const int64_t l = valid ? li : 0;
+    const int64_t outer = l / g.lines_inner;
+    const int64_t inner = l - outer * g.lines_inner;
+    LineIO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid};
+    if (FWD) forward_line_lp<W, LEV, EXT>(io);
+    else inverse_line_lp<W, LEV, EXT>(io);
+}
+
+}  // namespace lp
+}  // namespace whhd

@@ -269,6 +269,13 @@ int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, ch
     }
     st = check_line(n, levels, wt);
     if (st != WHHD_OK) return st;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if ((!cfg || cfg->allow_line_parallel) && g_line_kernel_mode.load() == 0) {
+            const int r = lp_transform((float*)dst, (const float*)src, nx, ny, nz, levels, axis, bc,
+                                       wt, fwd, stream);
+            if (r != -1) return r;
+        }
+    }
     g.n = n;
     g.levels = levels;
     g.lds_row = (n & 1) ? n : n + 1;

@@ -1,0 +1,62 @@
+// Instantiations and dispatch of the line-parallel kernels (line_parallel.h):
+// the two (wavelet, bc) pairs of the recovery path -- curl CDF3.7 'n' on 2^L
+// samples and divergence CDF4.6 'd' on 2^L+1 samples -- for L = 5..8.
+#include <hip/hip_runtime.h>
+
+#include "line_parallel.h"
+#include "whhd_internal.h"
+
+namespace whhd {
+
+template <class W, int EXT, bool FWD>
+static bool lp_launch_lev(int levels, float* dst, const float* src, const lp::LpGeom& g,
+                          hipStream_t s) {
+    const dim3 block(256);
+    const dim3 grid((unsigned)((g.total_lines + 255) / 256));
+    switch (levels) {
+#define LP_CASE(LEV) \
+    case LEV: hipLaunchKernelGGL((lp::lp_kernel<W, LEV, EXT, FWD>), grid, block, 0, s, dst, src, g); return true;
+    LP_CASE(5)
+    LP_CASE(6)
+    LP_CASE(7)
+    LP_CASE(8)
+#undef LP_CASE
+    default: return false;
+    }
+}
+
+// Returns WHHD_OK when a line-parallel kernel was launched, -1 when none fits
+// (the caller then uses the tile kernel).  Needs dst != src: lines are read
+// and written in one pass without staging.
+int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int levels, char axis,
+                 char bc, int wt, bool fwd, hipStream_t stream) {
+    if (dst == src) return -1;
+    lp::LpGeom g{};
+    int n;
+    switch (axis) {
+    case 'y': case 'Y':
+        n = ny; g.total_lines = (int64_t)nx * nz; g.lines_inner = (int)g.total_lines;
+        g.inner_stride = ny; g.outer_stride = 0; g.step = 1; break;
+    case 'x': case 'X':
+        n = nx; g.total_lines = (int64_t)ny * nz; g.lines_inner = ny; g.inner_stride = 1;
+        g.outer_stride = (int64_t)nx * ny; g.step = ny; break;
+    case 'z': case 'Z':
+        n = nz; g.total_lines = (int64_t)ny * nx; g.lines_inner = ny; g.inner_stride = 1;
+        g.outer_stride = ny; g.step = (int64_t)nx * ny; break;
+    default: return -1;
+    }
+    if (g.total_lines > (int64_t)1 << 31) return -1;
+    bool ok = false;
+    if (wt == WT_CDF3_7 && bc == 'n' && n == (1 << levels)) {
+        ok = fwd ? lp_launch_lev<W_Cdf37N, 0, true>(levels, dst, src, g, stream)
+                 : lp_launch_lev<W_Cdf37N, 0, false>(levels, dst, src, g, stream);
+    } else if (wt == WT_CDF4_6 && bc == 'd' && n == (1 << levels) + 1) {
+        ok = fwd ? lp_launch_lev<W_Cdf46D, 1, true>(levels, dst, src, g, stream)
+                 : lp_launch_lev<W_Cdf46D, 1, false>(levels, dst, src, g, stream);
+    }
+    if (!ok) return -1;
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "line-parallel kernel");
+}
+
+}  // namespace whhd

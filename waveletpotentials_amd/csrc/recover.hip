@@ -5,9 +5,10 @@
 // Per component, the velocity's own axis uses the divergence wavelet with
 // bc_div = proj_bc(bc) and the other axes the curl wavelet with bc; the
 // potential's own axis uses the curl wavelet and the other axes the
-// divergence wavelet.  The handle owns its scratch (one array per velocity
-// component, allocated at create) and enqueues everything on the caller's
-// stream without synchronizing, so a solve can be captured in a HIP graph.
+// divergence wavelet.  The handle owns its scratch (one velocity-sized and one
+// potential-sized array per component, allocated at create: passes run out of
+// place) and enqueues everything on the caller's stream without
+// synchronizing, so a solve can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
 
 #include "whhd_internal.h"
@@ -32,7 +33,8 @@ struct whhd_recover3d_s {
     char bc, bd;
     int wc, wd;
     whhd_project3d_s* proj = nullptr;
-    float* tmp[3] = {nullptr, nullptr, nullptr};
+    float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
+    float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
 };
 
 struct whhd_recover2d_s {
@@ -75,49 +77,48 @@ __global__ void cut_kernel(float* qx, float* qy, float* qz, int nx, int ny, int 
     if (zz && iqz >= 0 && iqz < nqz) qz[iqz] = 0.f;
 }
 
-static int fwd3(const whhd_recover3d_s* h, float* t[3], const float* xv, const float* yv,
-                const float* zv, hipStream_t s) {
-    const int X = h->nx, Y = h->ny, Z = h->nz;
-    TRY(transform_lines<float>(t[0], xv, X + 1, Y, Z, h->lx, 'x', h->bd, h->wd, true, s));
-    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->ly, 'y', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lz, 'z', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[1], yv, X, Y + 1, Z, h->lx, 'x', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->ly, 'y', h->bd, h->wd, true, s));
-    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lz, 'z', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[2], zv, X, Y, Z + 1, h->lx, 'x', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->ly, 'y', h->bc, h->wc, true, s));
-    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lz, 'z', h->bd, h->wd, true, s));
+// Three line passes (x, then y, then z) of one array.  The line-parallel
+// kernels read and write different arrays, so the passes ping-pong:
+// in -> out -> scratch -> out.  `scratch` may be `in` (consumed by the first
+// pass) but not `out`.
+struct Pass {
+    char bc;
+    int wt;
+};
+static int three_pass(const float* in, float* scratch, float* out, int nx, int ny, int nz,
+                      const int lv[3], Pass px, Pass py, Pass pz, bool fwd, hipStream_t s) {
+    TRY(transform_lines<float>(out, in, nx, ny, nz, lv[0], 'x', px.bc, px.wt, fwd, s));
+    TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s));
+    TRY(transform_lines<float>(out, scratch, nx, ny, nz, lv[2], 'z', pz.bc, pz.wt, fwd, s));
     return WHHD_OK;
 }
 
-// Forward (fwd=true) or inverse transform of a potential (qx, qy, qz).
-static int pot3(const whhd_recover3d_s* h, float* dx, float* dy, float* dz, const float* sx,
-                const float* sy, const float* sz, bool fwd, hipStream_t s) {
+// Velocity component c (0: xv, 1: yv, 2: zv): its own axis uses the divergence
+// wavelet, the others the curl wavelet.
+static int vel_pass(const whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
+                    bool fwd, hipStream_t s) {
+    const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
+    const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
-    TRY(transform_lines<float>(dx, sx, X, Y + 1, Z + 1, h->lx, 'x', h->bc, h->wc, fwd, s));
-    TRY(transform_lines<float>(dx, dx, X, Y + 1, Z + 1, h->ly, 'y', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dx, dx, X, Y + 1, Z + 1, h->lz, 'z', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dy, sy, X + 1, Y, Z + 1, h->lx, 'x', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dy, dy, X + 1, Y, Z + 1, h->ly, 'y', h->bc, h->wc, fwd, s));
-    TRY(transform_lines<float>(dy, dy, X + 1, Y, Z + 1, h->lz, 'z', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dz, sz, X + 1, Y + 1, Z, h->lx, 'x', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dz, dz, X + 1, Y + 1, Z, h->ly, 'y', h->bd, h->wd, fwd, s));
-    TRY(transform_lines<float>(dz, dz, X + 1, Y + 1, Z, h->lz, 'z', h->bc, h->wc, fwd, s));
-    return WHHD_OK;
+    switch (c) {
+    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s);
+    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s);
+    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s);
+    }
 }
 
-static int inv_vel3(const whhd_recover3d_s* h, float* t[3], hipStream_t s) {
+// Potential component c (0: qx, 1: qy, 2: qz): its own axis uses the curl
+// wavelet, the others the divergence wavelet.
+static int pot_pass(const whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
+                    bool fwd, hipStream_t s) {
+    const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
+    const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
-    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lx, 'x', h->bd, h->wd, false, s));
-    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->ly, 'y', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[0], t[0], X + 1, Y, Z, h->lz, 'z', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lx, 'x', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->ly, 'y', h->bd, h->wd, false, s));
-    TRY(transform_lines<float>(t[1], t[1], X, Y + 1, Z, h->lz, 'z', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lx, 'x', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->ly, 'y', h->bc, h->wc, false, s));
-    TRY(transform_lines<float>(t[2], t[2], X, Y, Z + 1, h->lz, 'z', h->bd, h->wd, false, s));
-    return WHHD_OK;
+    switch (c) {
+    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s);
+    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s);
+    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s);
+    }
 }
 
 static bool pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
@@ -146,11 +147,13 @@ int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, ch
         delete h;
         return st;
     }
-    const size_t sz[3] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz,
-                          (size_t)nx * ny * (nz + 1)};
-    for (int i = 0; i < 3; ++i)
-        if (hipMalloc(&h->tmp[i], sz[i] * sizeof(float)) != hipSuccess) {
-            for (int j = 0; j < i; ++j) hipFree(h->tmp[j]);
+    const size_t sz[6] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz,
+                          (size_t)nx * ny * (nz + 1), (size_t)nx * (ny + 1) * (nz + 1),
+                          (size_t)(nx + 1) * ny * (nz + 1), (size_t)(nx + 1) * (ny + 1) * nz};
+    float** slot[6] = {&h->tmp[0], &h->tmp[1], &h->tmp[2], &h->sq[0], &h->sq[1], &h->sq[2]};
+    for (int i = 0; i < 6; ++i)
+        if (hipMalloc(slot[i], sz[i] * sizeof(float)) != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipFree(*slot[j]);
             whhd_project3d_destroy(h->proj);
             delete h;
             return fail(WHHD_OOM, "recover3d: scratch allocation failed");
@@ -166,13 +169,19 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
     const bool vout = xo || yo || zo;
     if (vout && !(xo && yo && zo))
         return fail(WHHD_INVALID_ARG, "recover3d_solve_proj: all three v_out arrays are needed");
-    float* t[3] = {h->tmp[0], h->tmp[1], h->tmp[2]};
-    if (vout) { t[0] = xo; t[1] = yo; t[2] = zo; }
-    TRY(fwd3(h, t, xv, yv, zv, s));
-    if (vout) TRY(project3d_run(h->proj, qx, qy, qz, t[0], t[1], t[2], t[0], t[1], t[2], s));
-    else TRY(project3d_run(h->proj, qx, qy, qz, nullptr, nullptr, nullptr, t[0], t[1], t[2], s));
-    TRY(pot3(h, qx, qy, qz, qx, qy, qz, false, s));
-    if (vout) TRY(inv_vel3(h, t, s));
+    float* const* t = h->tmp;
+    float* const* S = h->sq;
+    const float* v[3] = {xv, yv, zv};
+    float* q[3] = {qx, qy, qz};
+    float* vo[3] = {xo, yo, zo};
+    for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, v[c], S[c], t[c], true, s));
+    // q coefficients land in the scratch; with v_out, s x q overwrites the
+    // transformed velocity in place at the projected coefficients
+    if (vout) TRY(project3d_run(h->proj, S[0], S[1], S[2], t[0], t[1], t[2], t[0], t[1], t[2], s));
+    else TRY(project3d_run(h->proj, S[0], S[1], S[2], nullptr, nullptr, nullptr, t[0], t[1], t[2], s));
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], q[c], false, s));
+    if (vout)
+        for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, t[c], S[c], vo[c], false, s));
     return WHHD_OK;
 }
 
@@ -180,14 +189,17 @@ int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const fl
                   const float* qy, const float* qz, int level, hipStream_t s) {
     if (!h || !ox || !oy || !oz || !qx || !qy || !qz)
         return fail(WHHD_INVALID_ARG, "recover3d_cut: null argument");
-    TRY(pot3(h, ox, oy, oz, qx, qy, qz, true, s));
+    float* const* S = h->sq;
+    const float* q[3] = {qx, qy, qz};
+    float* o[3] = {ox, oy, oz};
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, q[c], o[c], S[c], true, s));
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, h->nz + 1);
-    hipLaunchKernelGGL(cut_kernel, grid, block, 0, s, ox, oy, oz, h->nx, h->ny, h->nz, h->lx, h->ly,
-                       h->lz, level);
+    hipLaunchKernelGGL(cut_kernel, grid, block, 0, s, S[0], S[1], S[2], h->nx, h->ny, h->nz, h->lx,
+                       h->ly, h->lz, level);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "cut kernel");
-    TRY(pot3(h, ox, oy, oz, ox, oy, oz, false, s));
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], o[c], false, s));
     return WHHD_OK;
 }
 
@@ -272,6 +284,7 @@ WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out
 WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h) {
     if (!h) return;
     for (float* p : h->tmp) hipFree(p);
+    for (float* p : h->sq) hipFree(p);
     whhd_project3d_destroy(h->proj);
     delete h;
 }

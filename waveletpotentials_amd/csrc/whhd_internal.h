@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 
@@ -35,9 +36,13 @@ struct TileConfig {
     int contig_tile = 4;    // lines per workgroup, contiguous (y) lines
     int strided_tile = 16;  // consecutive y per workgroup, strided (x, z) lines
     int threads = 256;
+    bool allow_line_parallel = true;  // use the line-parallel kernels where one exists
 };
 
 TileConfig default_tile(int n, size_t elem, bool strided);
+
+// 0: line-parallel kernels where one exists, 1: windowed tile kernels only
+extern std::atomic<int> g_line_kernel_mode;
 
 // Error plumbing: set the thread's last-error message and return `status`.
 int fail(int status, const std::string& msg);
@@ -50,6 +55,11 @@ int ilog2(int n);
 template <class T>
 int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, char axis, char bc,
                     int wt, bool fwd, hipStream_t stream, const TileConfig* cfg = nullptr);
+
+// Line-parallel transform (dst != src, curl CDF3.7 'n' / divergence CDF4.6 'd',
+// 2^5..2^8 samples); returns -1 when no such kernel applies.
+int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int levels, char axis,
+                 char bc, int wt, bool fwd, hipStream_t stream);
 
 // Gram table (8 rows of n) of the synthesis basis of (wt, bc) into device memory.
 int gram_table_device(double* out, int wt, char bc, int n, hipStream_t stream);
