@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include "lifting.h"
@@ -90,24 +91,54 @@ WHHD_DEV void static_for(F&& f) {
 constexpr int kBlock = 16;  // pairs written per block
 
 // ---- I/O of one lane's line ----------------------------------------------
-// sample i of the lane's line is at base + i * step
-struct LineIO {
+// Strided lines (x and z axes): sample i of the lane's line is at
+// base + i * step; a load of sample i across the wavefront reads 64
+// consecutive floats.
+struct StridedIO {
     const float* src;
     float* dst;
     int64_t base;
     int64_t step;
     bool valid;
+    int n;
     WHHD_DEV float load(int i) const { return src[base + (int64_t)i * step]; }
     WHHD_DEV void store(int i, float v) const {
         if (valid) dst[base + (int64_t)i * step] = v;
     }
 };
 
+// Contiguous lines (y axis): the lane's line is n consecutive floats.  Loads
+// fetch 16-byte runs of four samples (dword alignment suffices on gfx950) and
+// the compiler folds the four element reads of a run into one load; the last
+// run of a line is read sample by sample so nothing past the array is touched.
+struct __attribute__((packed, aligned(4))) Quad {
+    float v[4];
+};
+struct ContigIO {
+    const float* src;
+    float* dst;
+    int64_t base;
+    int64_t step;
+    bool valid;
+    int n;
+    WHHD_DEV float load(int i) const {
+        const int q = i & ~3;
+        if (q + 3 < n) {
+            const Quad r = *reinterpret_cast<const Quad*>(src + base + q);
+            return r.v[i & 3];
+        }
+        return src[base + i];
+    }
+    WHHD_DEV void store(int i, float v) const {
+        if (valid) dst[base + i] = v;
+    }
+};
+
 // ---- forward --------------------------------------------------------------
 // One forward level: the 2^(l+1)+EXT samples come from `in` (or the line when
 // TOP), approximations go to `out`, details to the line at [W, 2W-EXT).
-template <class W, int l, int EXT, bool TOP>
-WHHD_DEV void fwd_level(const LineIO& io, const float* in, float* out) {
+template <class W, int l, int EXT, bool TOP, class IO>
+WHHD_DEV void fwd_level(const IO& io, const float* in, float* out) {
     constexpr int H = W::halo;
     constexpr int L = 1 << (l + 1);
     constexpr int Wd = L / 2 + EXT;
@@ -152,8 +183,8 @@ WHHD_DEV void fwd_level(const LineIO& io, const float* in, float* out) {
     });
 }
 
-template <class W, int l, int EXT>
-WHHD_DEV void fwd_levels_below(const LineIO& io, const float* in) {
+template <class W, int l, int EXT, class IO>
+WHHD_DEV void fwd_levels_below(const IO& io, const float* in) {
     // `in` holds the 2^(l+1)+EXT approximations produced by level l+1
     if constexpr (l >= 2) {
         float out[(1 << l) + EXT];
@@ -173,8 +204,8 @@ WHHD_DEV void fwd_levels_below(const LineIO& io, const float* in) {
     }
 }
 
-template <class W, int LEV, int EXT>
-WHHD_DEV void forward_line_lp(const LineIO& io) {
+template <class W, int LEV, int EXT, class IO>
+WHHD_DEV void forward_line_lp(const IO& io) {
     if constexpr (LEV > 2) {
         float out[(1 << (LEV - 1)) + EXT];
         fwd_level<W, LEV - 1, EXT, true>(io, nullptr, out);
@@ -190,8 +221,8 @@ WHHD_DEV void forward_line_lp(const LineIO& io) {
 // One inverse level l: approximations [0, 2^l+EXT) from `in`, details from the
 // line at [2^l+EXT, 2^(l+1)+EXT); reconstructed samples to `out`, or to the
 // line when LAST.
-template <class W, int l, int EXT, bool LAST>
-WHHD_DEV void inv_level(const LineIO& io, const float* in, float* out) {
+template <class W, int l, int EXT, bool LAST, class IO>
+WHHD_DEV void inv_level(const IO& io, const float* in, float* out) {
     constexpr int H = W::halo;
     constexpr int L = 1 << l;
     constexpr int Wd = L + EXT;
@@ -234,8 +265,8 @@ WHHD_DEV void inv_level(const LineIO& io, const float* in, float* out) {
     });
 }
 
-template <class W, int l, int LEV, int EXT>
-WHHD_DEV void inv_levels_from(const LineIO& io, const float* in) {
+template <class W, int l, int LEV, int EXT, class IO>
+WHHD_DEV void inv_levels_from(const IO& io, const float* in) {
     // `in` holds the 2^l+EXT approximations of level l
     if constexpr (l == LEV - 1) {
         inv_level<W, l, EXT, true>(io, in, nullptr);
@@ -246,8 +277,8 @@ WHHD_DEV void inv_levels_from(const LineIO& io, const float* in) {
     }
 }
 
-template <class W, int LEV, int EXT>
-WHHD_DEV void inverse_line_lp(const LineIO& io) {
+template <class W, int LEV, int EXT, class IO>
+WHHD_DEV void inverse_line_lp(const IO& io) {
     constexpr int M = W::m10;
     float c[M], a[M];
     static_for<0, M>([&](auto i) { c[decltype(i)::value] = io.load(decltype(i)::value); });
@@ -278,7 +309,7 @@ struct LpGeom {
     int64_t step;
 };
 
-template <class W, int LEV, int EXT, bool FWD>
+template <class W, int LEV, int EXT, bool FWD, bool CONTIG>
 __global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const float* __restrict__ src,
                                                  LpGeom g) {
     const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,10 +317,37 @@ __global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const 
     const int64_t l = valid ? li : 0;
     const int64_t outer = l / g.lines_inner;
     const int64_t inner = l - outer * g.lines_inner;
-    LineIO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid};
+    using IO = std::conditional_t<CONTIG, ContigIO, StridedIO>;
+    IO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid,
+          (1 << LEV) + EXT};
     if (FWD) forward_line_lp<W, LEV, EXT>(io);
     else inverse_line_lp<W, LEV, EXT>(io);
 }
 
+// Launch the line-parallel kernel for `levels` (5..8); false if none exists.
+template <class W, int EXT, bool FWD, bool CONTIG>
+bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hipStream_t s) {
+    const dim3 block(256);
+    const dim3 grid((unsigned)((g.total_lines + 255) / 256));
+    switch (levels) {
+    case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
+    case 6: hipLaunchKernelGGL((lp_kernel<W, 6, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
+    case 7: hipLaunchKernelGGL((lp_kernel<W, 7, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
+    case 8: hipLaunchKernelGGL((lp_kernel<W, 8, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
+    default: return false;
+    }
+}
+
 }  // namespace lp
+
+template <class W, int EXT>
+bool lp_launch(int levels, bool fwd, bool contig, float* dst, const float* src,
+               const lp::LpGeom& g, hipStream_t s) {
+    if (contig)
+        return fwd ? lp::lp_launch_one<W, EXT, true, true>(levels, dst, src, g, s)
+                   : lp::lp_launch_one<W, EXT, false, true>(levels, dst, src, g, s);
+    return fwd ? lp::lp_launch_one<W, EXT, true, false>(levels, dst, src, g, s)
+               : lp::lp_launch_one<W, EXT, false, false>(levels, dst, src, g, s);
+}
+
 }  // namespace whhd

@@ -1,0 +1,7 @@
+// Line-parallel kernels for CDF3.7 bc 'n' (see line_parallel.h).
+#include "line_parallel.h"
+
+namespace whhd {
+template bool lp_launch<W_Cdf37N, 0>(int, bool, bool, float*, const float*, const lp::LpGeom&,
+                                   hipStream_t);
+}  // namespace whhd

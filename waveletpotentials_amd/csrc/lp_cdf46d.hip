@@ -1,0 +1,7 @@
+// Line-parallel kernels for CDF4.6 bc 'd' (see line_parallel.h).
+#include "line_parallel.h"
+
+namespace whhd {
+template bool lp_launch<W_Cdf46D, 1>(int, bool, bool, float*, const float*, const lp::LpGeom&,
+                                   hipStream_t);
+}  // namespace whhd

@@ -1,6 +1,7 @@
-// Instantiations and dispatch of the line-parallel kernels (line_parallel.h):
-// the two (wavelet, bc) pairs of the recovery path -- curl CDF3.7 'n' on 2^L
-// samples and divergence CDF4.6 'd' on 2^L+1 samples -- for L = 5..8.
+// Dispatch of the line-parallel kernels (line_parallel.h): the two (wavelet,
+// bc) pairs of the recovery path -- curl CDF3.7 'n' on 2^L samples and
+// divergence CDF4.6 'd' on 2^L+1 samples -- for L = 5..8, instantiated in
+// lp_cdf37n.hip and lp_cdf46d.hip.
 #include <hip/hip_runtime.h>
 
 #include "line_parallel.h"
@@ -8,22 +9,10 @@
 
 namespace whhd {
 
-template <class W, int EXT, bool FWD>
-static bool lp_launch_lev(int levels, float* dst, const float* src, const lp::LpGeom& g,
-                          hipStream_t s) {
-    const dim3 block(256);
-    const dim3 grid((unsigned)((g.total_lines + 255) / 256));
-    switch (levels) {
-#define LP_CASE(LEV) \
-    case LEV: hipLaunchKernelGGL((lp::lp_kernel<W, LEV, EXT, FWD>), grid, block, 0, s, dst, src, g); return true;
-    LP_CASE(5)
-    LP_CASE(6)
-    LP_CASE(7)
-    LP_CASE(8)
-#undef LP_CASE
-    default: return false;
-    }
-}
+extern template bool lp_launch<W_Cdf37N, 0>(int, bool, bool, float*, const float*,
+                                            const lp::LpGeom&, hipStream_t);
+extern template bool lp_launch<W_Cdf46D, 1>(int, bool, bool, float*, const float*,
+                                            const lp::LpGeom&, hipStream_t);
 
 // Returns WHHD_OK when a line-parallel kernel was launched, -1 when none fits
 // (the caller then uses the tile kernel).  Needs dst != src: lines are read
@@ -47,13 +36,11 @@ int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int level
     }
     if (g.total_lines > (int64_t)1 << 31) return -1;
     bool ok = false;
-    if (wt == WT_CDF3_7 && bc == 'n' && n == (1 << levels)) {
-        ok = fwd ? lp_launch_lev<W_Cdf37N, 0, true>(levels, dst, src, g, stream)
-                 : lp_launch_lev<W_Cdf37N, 0, false>(levels, dst, src, g, stream);
-    } else if (wt == WT_CDF4_6 && bc == 'd' && n == (1 << levels) + 1) {
-        ok = fwd ? lp_launch_lev<W_Cdf46D, 1, true>(levels, dst, src, g, stream)
-                 : lp_launch_lev<W_Cdf46D, 1, false>(levels, dst, src, g, stream);
-    }
+    const bool contig = (axis == 'y' || axis == 'Y');
+    if (wt == WT_CDF3_7 && bc == 'n' && n == (1 << levels))
+        ok = lp_launch<W_Cdf37N, 0>(levels, fwd, contig, dst, src, g, stream);
+    else if (wt == WT_CDF4_6 && bc == 'd' && n == (1 << levels) + 1)
+        ok = lp_launch<W_Cdf46D, 1>(levels, fwd, contig, dst, src, g, stream);
     if (!ok) return -1;
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "line-parallel kernel");
