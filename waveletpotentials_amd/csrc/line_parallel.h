@@ -89,21 +89,40 @@ WHHD_DEV void static_for(F&& f) {
 }
 
 constexpr int kBlock = 16;  // pairs written per block
+// Lines per wavefront.  Fewer than 64 leaves lanes idle but doubles the
+// number of wavefronts, which the memory system needs to overlap latency:
+// a 256^3 pass has only ~65k lines, i.e. one 64-line wavefront per SIMD.
+constexpr int kLinesPerWave = 64;
 
 // ---- I/O of one lane's line ----------------------------------------------
 // Strided lines (x and z axes): sample i of the lane's line is at
 // base + i * step; a load of sample i across the wavefront reads 64
-// consecutive floats.
+// consecutive floats.  Accesses go through buffer resources: the lane's line
+// start is the per-lane byte offset and i * step the scalar offset, so no
+// vector instructions are spent on addresses.  Lanes that must not store get
+// an offset past the end of the buffer, which the hardware discards.
+constexpr uint32_t kBufWord3 = 0x00020000;  // raw dword buffer, gfx9 family
+constexpr uint32_t kOutOfRange = 0xF0000000u;
+
 struct StridedIO {
-    const float* src;
-    float* dst;
-    int64_t base;
-    int64_t step;
-    bool valid;
+    __amdgpu_buffer_rsrc_t rs;
+    __amdgpu_buffer_rsrc_t rd;
+    uint32_t voff_ld;  // byte offset of the lane's line start (loads)
+    uint32_t voff_st;  // same, or out of range for lanes that do not store
+    int step_bytes;
     int n;
-    WHHD_DEV float load(int i) const { return src[base + (int64_t)i * step]; }
+    WHHD_DEV float load(int i) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff_ld, i * step_bytes, 0));
+    }
     WHHD_DEV void store(int i, float v) const {
-        if (valid) dst[base + (int64_t)i * step] = v;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rd, voff_st,
+                                              i * step_bytes, 0);
+    }
+    // samples [START, START+CNT) of the line; each store is already one row
+    template <int START, int CNT>
+    WHHD_DEV void store_run(const float* vals) const {
+#pragma unroll
+        for (int v = 0; v < CNT; ++v) store(START + v, vals[v]);
     }
 };
 
@@ -114,6 +133,9 @@ struct StridedIO {
 struct __attribute__((packed, aligned(4))) Quad {
     float v[4];
 };
+constexpr int kStageRun = 32;       // longest run written at once
+constexpr int kStageRow = 64 + 1;   // LDS row: one value of each of the 64 lines
+
 struct ContigIO {
     const float* src;
     float* dst;
@@ -121,6 +143,10 @@ struct ContigIO {
     int64_t step;
     bool valid;
     int n;
+    float* stage;          // this wavefront's LDS staging area, kStageRun rows
+    int64_t first_line;    // line of lane 0
+    int64_t total_lines;
+    int lane;
     WHHD_DEV float load(int i) const {
         const int q = i & ~3;
         if (q + 3 < n) {
@@ -129,8 +155,48 @@ struct ContigIO {
         }
         return src[base + i];
     }
-    WHHD_DEV void store(int i, float v) const {
-        if (valid) dst[base + i] = v;
+    // Samples [START, START+CNT) of every lane's line.  The run is transposed
+    // through LDS so that consecutive lanes store consecutive 16-byte pieces
+    // of one line (full 64-128 byte segments per line) instead of 64 lines
+    // touched 16 bytes at a time.
+    template <int START, int CNT>
+    WHHD_DEV void store_run(const float* vals) const {
+        static_assert(CNT <= kStageRun, "run too long");
+        constexpr int NQ = (CNT + 3) / 4;          // 16-byte pieces per line
+        constexpr int LPI = 64 / NQ;               // lines per pass (NQ divides 64 here)
+        static_assert(64 % NQ == 0, "pieces per line must divide 64");
+#pragma unroll
+        for (int v = 0; v < CNT; ++v) stage[v * kStageRow + lane] = vals[v];
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        const int q = lane % NQ;
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) {
+            const int l = lane / NQ + r * LPI;
+            const int64_t line = first_line + l;
+            const bool mine = l < kLinesPerWave;
+            float w[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int idx = 4 * q + e;
+                w[e] = idx < CNT ? stage[idx * kStageRow + l] : 0.f;
+            }
+            if (mine && line < total_lines) {
+                float* out = dst + line * n + START + 4 * q;
+                if (4 * q + 3 < CNT) {
+                    Quad qv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) qv.v[e] = w[e];
+                    *reinterpret_cast<Quad*>(out) = qv;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (4 * q + e < CNT) out[e] = w[e];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
     }
 };
 
@@ -172,14 +238,17 @@ WHHD_DEV void fwd_level(const IO& io, const float* in, float* out) {
             }
         });
         W::up::fwd(A, D, SA, SD);
+        constexpr int nd = (Wd - EXT - b0) < kBlock ? (Wd - EXT - b0) : kBlock;  // details here
+        float run[kBlock];
         static_for<H, H + kBlock>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
             constexpr int k = b0 - H + j;
             if constexpr (k < Wd) {
                 out[k] = A[j];
-                if constexpr (k < Wd - EXT) io.store(Wd + k, D[j]);
+                if constexpr (k < Wd - EXT) run[k - b0] = D[j];
             }
         });
+        if constexpr (nd > 0) io.template store_run<Wd + b0, nd>(run);
     });
 }
 
@@ -192,6 +261,7 @@ WHHD_DEV void fwd_levels_below(const IO& io, const float* in) {
         fwd_levels_below<W, l - 1, EXT>(io, out);
     } else {
         constexpr int M = W::m10;
+        float run[M];
         static_for<0, M>([&](auto ri) {
             constexpr int r = decltype(ri)::value;
             float acc = 0.f;
@@ -199,8 +269,9 @@ WHHD_DEV void fwd_levels_below(const IO& io, const float* in) {
                 constexpr int c = decltype(ci)::value;
                 acc += in[c] * float(W::l10[r * M + c]);
             });
-            io.store(r, acc);
+            run[r] = acc;
         });
+        io.template store_run<0, M>(run);
     }
 }
 
@@ -250,18 +321,22 @@ WHHD_DEV void inv_level(const IO& io, const float* in, float* out) {
             }
         });
         W::up::inv(A, D, SA, SD);
+        // samples [2*b0, 2*b0 + ns) are produced by this block
+        constexpr int ns = (2 * L + EXT - 2 * b0) < 2 * kBlock ? (2 * L + EXT - 2 * b0) : 2 * kBlock;
+        float run[2 * kBlock];
         static_for<H, H + kBlock>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
             constexpr int k = b0 - H + j;
             if constexpr (k < L + EXT) {
-                if constexpr (LAST) io.store(2 * k, A[j]);
+                if constexpr (LAST) run[2 * (k - b0)] = A[j];
                 else out[2 * k] = A[j];
             }
             if constexpr (k < L) {
-                if constexpr (LAST) io.store(2 * k + 1, D[j]);
+                if constexpr (LAST) run[2 * (k - b0) + 1] = D[j];
                 else out[2 * k + 1] = D[j];
             }
         });
+        if constexpr (LAST && ns > 0) io.template store_run<2 * b0, ns>(run);
     });
 }
 
@@ -294,7 +369,7 @@ WHHD_DEV void inverse_line_lp(const IO& io) {
     if constexpr (LEV > 2) {
         inv_levels_from<W, 2, LEV, EXT>(io, a);
     } else {
-        static_for<0, M>([&](auto i) { io.store(decltype(i)::value, a[decltype(i)::value]); });
+        io.template store_run<0, M>(a);
     }
 }
 
@@ -303,6 +378,7 @@ WHHD_DEV void inverse_line_lp(const IO& io) {
 // inner * inner_stride + outer * outer_stride and its samples are `step` apart.
 struct LpGeom {
     int64_t total_lines;
+    int64_t total_elems;  // elements in the array (buffer range)
     int lines_inner;
     int64_t inner_stride;
     int64_t outer_stride;
@@ -312,23 +388,39 @@ struct LpGeom {
 template <class W, int LEV, int EXT, bool FWD, bool CONTIG>
 __global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const float* __restrict__ src,
                                                  LpGeom g) {
-    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = li < g.total_lines;
-    const int64_t l = valid ? li : 0;
+    const int lane_ = threadIdx.x & 63;
+    const int64_t wave_first = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kLinesPerWave;
+    const int64_t li = wave_first + (lane_ % kLinesPerWave);
+    const bool in_range = li < g.total_lines;
+    const bool valid = in_range && lane_ < kLinesPerWave;  // lanes past kLinesPerWave mirror, never store
+    const int64_t l = in_range ? li : 0;
     const int64_t outer = l / g.lines_inner;
     const int64_t inner = l - outer * g.lines_inner;
-    using IO = std::conditional_t<CONTIG, ContigIO, StridedIO>;
-    IO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid,
-          (1 << LEV) + EXT};
-    if (FWD) forward_line_lp<W, LEV, EXT>(io);
-    else inverse_line_lp<W, LEV, EXT>(io);
+    constexpr int n = (1 << LEV) + EXT;
+    if constexpr (CONTIG) {
+        __shared__ float stage[4][kStageRun * kStageRow];
+        const int lane = threadIdx.x & 63;
+        ContigIO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid, n,
+                    stage[threadIdx.x >> 6], wave_first, g.total_lines, lane};
+        if (FWD) forward_line_lp<W, LEV, EXT>(io);
+        else inverse_line_lp<W, LEV, EXT>(io);
+    } else {
+        const uint32_t bytes = (uint32_t)(g.total_elems * 4);
+        const uint32_t voff = (uint32_t)((inner * g.inner_stride + outer * g.outer_stride) * 4);
+        StridedIO io{__builtin_amdgcn_make_buffer_rsrc((void*)src, 0, bytes, kBufWord3),
+                     __builtin_amdgcn_make_buffer_rsrc((void*)dst, 0, bytes, kBufWord3),
+                     voff, valid ? voff : kOutOfRange, (int)(g.step * 4), n};
+        if (FWD) forward_line_lp<W, LEV, EXT>(io);
+        else inverse_line_lp<W, LEV, EXT>(io);
+    }
 }
 
 // Launch the line-parallel kernel for `levels` (5..8); false if none exists.
 template <class W, int EXT, bool FWD, bool CONTIG>
 bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hipStream_t s) {
     const dim3 block(256);
-    const dim3 grid((unsigned)((g.total_lines + 255) / 256));
+    const int64_t lines_per_block = 4 * kLinesPerWave;
+    const dim3 grid((unsigned)((g.total_lines + lines_per_block - 1) / lines_per_block));
     switch (levels) {
     case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
     case 6: hipLaunchKernelGGL((lp_kernel<W, 6, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;

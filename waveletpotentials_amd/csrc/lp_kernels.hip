@@ -34,7 +34,9 @@ int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int level
         g.outer_stride = ny; g.step = (int64_t)nx * ny; break;
     default: return -1;
     }
-    if (g.total_lines > (int64_t)1 << 31) return -1;
+    g.total_elems = (int64_t)nx * ny * nz;
+    // buffer offsets are 32-bit; keep clear of the out-of-range marker
+    if (g.total_lines > (int64_t)1 << 31 || g.total_elems * 4 >= (int64_t)lp::kOutOfRange) return -1;
     bool ok = false;
     const bool contig = (axis == 'y' || axis == 'Y');
     if (wt == WT_CDF3_7 && bc == 'n' && n == (1 << levels))

@@ -46,94 +46,103 @@ struct Proj3Args {
     int nx, ny, nz;
 };
 
+// One thread per (x, y) column walking ZB consecutive z cells; offsets in
+// index type I (32-bit whenever every array fits).
+template <class I, int ZB>
 __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
-    const int z = blockIdx.z;
+    const int z0 = blockIdx.z * ZB;
     if (y >= p.ny || x >= p.nx) return;
-    const int64_t NY = p.ny, NX = p.nx;
-    const int64_t sxv_z = (NX + 1) * NY, syv_z = NX * (NY + 1), szv_z = NX * NY;
-    const int64_t sqx_x = NY + 1, sqx_z = NX * (NY + 1);
-    const int64_t sqy_x = NY, sqy_z = (NX + 1) * NY;
-    const int64_t sqz_x = NY + 1, sqz_z = (NX + 1) * (NY + 1);
-    const int64_t ixv = y + (x + 1) * NY + z * sxv_z;
-    const int64_t iyv = (y + 1) + x * (NY + 1) + z * syv_z;
-    const int64_t izv = y + x * NY + (z + 1) * szv_z;
-    const int64_t iqx = (y + 1) + x * sqx_x + (z + 1) * sqx_z;
-    const int64_t iqy = y + (x + 1) * sqy_x + (z + 1) * sqy_z;
-    const int64_t iqz = (y + 1) + (x + 1) * sqz_x + z * sqz_z;
-
-    const float sx = p.cx.s[x], sy = p.cy.s[y], sz = p.cz.s[z];
-    const float vx = p.xv[ixv], vy = p.yv[iyv], vz = p.zv[izv];
-    float lx = 0.f, ly = 0.f, lz = 0.f;
-    if (x > 0 && y > 0 && z > 0) {
-        const float inv = 1.f / (sy * sy + sx * sx + sz * sz);
-        lz = (sy * vx - sx * vy) * inv;
-        lx = (sz * vy - sy * vz) * inv;
-        ly = (sx * vz - sz * vx) * inv;
-    } else {
-        const float ip0x = p.cx.b0[x], ip1x = p.cx.b1[x];
-        const float ip0y = p.cy.b0[y], ip1y = p.cy.b1[y];
-        const float ip0z = p.cz.b0[z], ip1z = p.cz.b1[z];
-        const float eps = (float)kEps;
-        if (y > 0 && z > 0 && x == 0) {
-            const float syByBz = sy * ip0x * ip0y * ip1z;
-            const float szByBz = sz * ip0x * ip1y * ip0z;
-            const float mx = 1.f / (sy * syByBz + sz * szByBz + eps);
-            lx = (vy * szByBz - vz * syByBz) * mx;
-            p.qz[iqz - sqz_x] = 0.f;
-            p.qy[iqy - sqy_x] = 0.f;
-        } else if (x > 0 && z > 0 && y == 0) {
-            const float sxBxBz = sx * ip0x * ip0y * ip1z;
-            const float szBxBz = sz * ip1x * ip0y * ip0z;
-            const float my = 1.f / (sx * sxBxBz + sz * szBxBz + eps);
-            ly = (vz * sxBxBz - vx * szBxBz) * my;
-            p.qx[iqx - 1] = 0.f;
-            p.qz[iqz - 1] = 0.f;
-        } else if (x > 0 && y > 0 && z == 0) {
-            const float sxBxBy = sx * ip0x * ip1y * ip0z;
-            const float syBxBy = sy * ip1x * ip0y * ip0z;
-            const float mz = 1.f / (sx * sxBxBy + sy * syBxBy + eps);
-            lz = (vx * syBxBy - vy * sxBxBy) * mz;
-            p.qx[iqx - sqx_z] = 0.f;
-            p.qy[iqy - sqy_z] = 0.f;
-        } else if (y == 0 && x > 0 && z == 0) {
-            p.qy[iqy - sqy_z] = 0.f;
-            p.qz[iqz - 1] = 0.f;
-            p.qx[iqx - 1] = 0.f;
-            p.qx[iqx - sqx_z - 1] = 0.f;
-            p.qx[iqx - sqx_z] = 0.f;
-        } else if (y > 0 && x == 0 && z == 0) {
-            p.qx[iqx - sqx_z] = 0.f;
-            p.qz[iqz - sqz_x] = 0.f;
-            p.qy[iqy - sqy_x - sqy_z] = 0.f;
-            p.qy[iqy - sqy_z] = 0.f;
-            p.qy[iqy - sqy_x] = 0.f;
-        } else if (y == 0 && x == 0 && z > 0) {
-            p.qx[iqx - 1] = 0.f;
-            p.qy[iqy - sqy_x] = 0.f;
-            p.qz[iqz - sqz_x - 1] = 0.f;
-            p.qz[iqz - sqz_x] = 0.f;
-            p.qz[iqz - 1] = 0.f;
+    const I NY = p.ny, NX = p.nx;
+    const I sxv_z = (NX + 1) * NY, syv_z = NX * (NY + 1), szv_z = NX * NY;
+    const I sqx_x = NY + 1, sqx_z = NX * (NY + 1);
+    const I sqy_x = NY, sqy_z = (NX + 1) * NY;
+    const I sqz_x = NY + 1, sqz_z = (NX + 1) * (NY + 1);
+    I ixv = y + (x + 1) * NY + z0 * sxv_z;
+    I iyv = (y + 1) + x * (NY + 1) + z0 * syv_z;
+    I izv = y + x * NY + (z0 + 1) * szv_z;
+    I iqx = (y + 1) + x * sqx_x + (z0 + 1) * sqx_z;
+    I iqy = y + (x + 1) * sqy_x + (z0 + 1) * sqy_z;
+    I iqz = (y + 1) + (x + 1) * sqz_x + z0 * sqz_z;
+    const float sx = p.cx.s[x], sy = p.cy.s[y];
+    const float sxx_syy = sx * sx + sy * sy;
+    const int zend = min(z0 + ZB, p.nz);
+    for (int z = z0; z < zend; ++z) {
+        const float sz = p.cz.s[z];
+        const float vx = p.xv[ixv], vy = p.yv[iyv], vz = p.zv[izv];
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        if (x > 0 && y > 0 && z > 0) {
+            const float inv = 1.f / (sxx_syy + sz * sz);
+            lz = (sy * vx - sx * vy) * inv;
+            lx = (sz * vy - sy * vz) * inv;
+            ly = (sx * vz - sz * vx) * inv;
         } else {
-            p.qz[iqz - 1 - sqz_x] = 0.f;
-            p.qz[iqz - 1] = 0.f;
-            p.qz[iqz - sqz_x] = 0.f;
-            p.qx[iqx - 1 - sqx_z] = 0.f;
-            p.qx[iqx - sqx_z] = 0.f;
-            p.qx[iqx - 1] = 0.f;
-            p.qy[iqy - sqy_x - sqy_z] = 0.f;
-            p.qy[iqy - sqy_z] = 0.f;
-            p.qy[iqy - sqy_x] = 0.f;
+            const float ip0x = p.cx.b0[x], ip1x = p.cx.b1[x];
+            const float ip0y = p.cy.b0[y], ip1y = p.cy.b1[y];
+            const float ip0z = p.cz.b0[z], ip1z = p.cz.b1[z];
+            const float eps = (float)kEps;
+            if (y > 0 && z > 0 && x == 0) {
+                const float syByBz = sy * ip0x * ip0y * ip1z;
+                const float szByBz = sz * ip0x * ip1y * ip0z;
+                const float mx = 1.f / (sy * syByBz + sz * szByBz + eps);
+                lx = (vy * szByBz - vz * syByBz) * mx;
+                p.qz[iqz - sqz_x] = 0.f;
+                p.qy[iqy - sqy_x] = 0.f;
+            } else if (x > 0 && z > 0 && y == 0) {
+                const float sxBxBz = sx * ip0x * ip0y * ip1z;
+                const float szBxBz = sz * ip1x * ip0y * ip0z;
+                const float my = 1.f / (sx * sxBxBz + sz * szBxBz + eps);
+                ly = (vz * sxBxBz - vx * szBxBz) * my;
+                p.qx[iqx - 1] = 0.f;
+                p.qz[iqz - 1] = 0.f;
+            } else if (x > 0 && y > 0 && z == 0) {
+                const float sxBxBy = sx * ip0x * ip1y * ip0z;
+                const float syBxBy = sy * ip1x * ip0y * ip0z;
+                const float mz = 1.f / (sx * sxBxBy + sy * syBxBy + eps);
+                lz = (vx * syBxBy - vy * sxBxBy) * mz;
+                p.qx[iqx - sqx_z] = 0.f;
+                p.qy[iqy - sqy_z] = 0.f;
+            } else if (y == 0 && x > 0 && z == 0) {
+                p.qy[iqy - sqy_z] = 0.f;
+                p.qz[iqz - 1] = 0.f;
+                p.qx[iqx - 1] = 0.f;
+                p.qx[iqx - sqx_z - 1] = 0.f;
+                p.qx[iqx - sqx_z] = 0.f;
+            } else if (y > 0 && x == 0 && z == 0) {
+                p.qx[iqx - sqx_z] = 0.f;
+                p.qz[iqz - sqz_x] = 0.f;
+                p.qy[iqy - sqy_x - sqy_z] = 0.f;
+                p.qy[iqy - sqy_z] = 0.f;
+                p.qy[iqy - sqy_x] = 0.f;
+            } else if (y == 0 && x == 0 && z > 0) {
+                p.qx[iqx - 1] = 0.f;
+                p.qy[iqy - sqy_x] = 0.f;
+                p.qz[iqz - sqz_x - 1] = 0.f;
+                p.qz[iqz - sqz_x] = 0.f;
+                p.qz[iqz - 1] = 0.f;
+            } else {
+                p.qz[iqz - 1 - sqz_x] = 0.f;
+                p.qz[iqz - 1] = 0.f;
+                p.qz[iqz - sqz_x] = 0.f;
+                p.qx[iqx - 1 - sqx_z] = 0.f;
+                p.qx[iqx - sqx_z] = 0.f;
+                p.qx[iqx - 1] = 0.f;
+                p.qy[iqy - sqy_x - sqy_z] = 0.f;
+                p.qy[iqy - sqy_z] = 0.f;
+                p.qy[iqy - sqy_x] = 0.f;
+            }
         }
-    }
-    p.qx[iqx] = lx;
-    p.qy[iqy] = ly;
-    p.qz[iqz] = lz;
-    if (p.xo) {
-        p.xo[ixv] = sy * lz - sz * ly;
-        p.yo[iyv] = sz * lx - sx * lz;
-        p.zo[izv] = sx * ly - sy * lx;
+        p.qx[iqx] = lx;
+        p.qy[iqy] = ly;
+        p.qz[iqz] = lz;
+        if (p.xo) {
+            p.xo[ixv] = sy * lz - sz * ly;
+            p.yo[iyv] = sz * lx - sx * lz;
+            p.zo[izv] = sx * ly - sy * lx;
+        }
+        ixv += sxv_z; iyv += syv_z; izv += szv_z;
+        iqx += sqx_z; iqy += sqy_z; iqz += sqz_z;
     }
 }
 
@@ -274,9 +283,14 @@ int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, fl
     if ((xo || yo || zo) && !(xo && yo && zo))
         return fail(WHHD_INVALID_ARG, "project3d_run: give all three or none of the v_out arrays");
     Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz};
+    constexpr int ZB = 4;
     dim3 block(64, 4, 1);
-    dim3 grid((h->ny + 63) / 64, (h->nx + 3) / 4, h->nz);
-    hipLaunchKernelGGL(project3d_kernel, grid, block, 0, st, a);
+    dim3 grid((h->ny + 63) / 64, (h->nx + 3) / 4, (h->nz + ZB - 1) / ZB);
+    const int64_t biggest = (int64_t)(h->nx + 1) * (h->ny + 1) * (h->nz + 1);
+    if (biggest < ((int64_t)1 << 31))
+        hipLaunchKernelGGL((project3d_kernel<int, ZB>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((project3d_kernel<int64_t, ZB>), grid, block, 0, st, a);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "project3d kernel");
 }
