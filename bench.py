@@ -5,13 +5,14 @@ the per-cell projection, 9 inverse line transforms) over a synthetic
 divergence-free velocity (curl of a seeded Fourier potential, |k| <= 16,
 |k|^(-11/6) amplitudes) that is resident in HBM before timing starts.
 
-Prints one JSON line.  `value` is Mcells/s of the whole solve; `roofline`
-describes the kernel with the largest share of the solve, timed per launch
-with HIP events on the stream the library launches on; `cpu_baseline` times
-the FP64 C oracle (a port of the reference algorithm) on this host.
+Prints one JSON line.  `value` is Mcells/s of the whole solve (K solves
+between device synchronizations).  `roofline` describes the kernel with the
+largest share of the solve: a second run of K solves records HIP events after
+every launch on the stream the library launches on (the library's stage
+trace), giving each launch's duration in context.  `cpu_baseline` times the
+FP64 C oracle (a port of the reference algorithm) on this host.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -30,24 +31,42 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=256, help="cells per axis")
-    p.add_argument("--stage-reps", type=int, default=20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-runs", type=int, default=5)
     return p.parse_args()
 
 
-def stage_plan(N):
-    """The 19 launches of one solve: (name, array, dims (nx,ny,nz), kind, args)."""
-    xv, yv, zv = (N + 1, N, N), (N, N + 1, N), (N, N, N + 1)
-    qx, qy, qz = (N, N + 1, N + 1), (N + 1, N, N + 1), (N + 1, N + 1, N)
-    C, D = ("n", 11), ("d", 7)  # (bc, wavelet): curl CDF3.7/'n', div CDF4.6/'d'
-    fwd = [("xv", xv, "x", D), ("xv", xv, "y", C), ("xv", xv, "z", C),
-           ("yv", yv, "x", C), ("yv", yv, "y", D), ("yv", yv, "z", C),
-           ("zv", zv, "x", C), ("zv", zv, "y", C), ("zv", zv, "z", D)]
-    inv = [("qx", qx, "x", C), ("qx", qx, "y", D), ("qx", qx, "z", D),
-           ("qy", qy, "x", D), ("qy", qy, "y", C), ("qy", qy, "z", D),
-           ("qz", qz, "x", D), ("qz", qz, "y", D), ("qz", qz, "z", C)]
-    return fwd, inv
+def array_bytes(N):
+    """Bytes of each MAC array of an N^3 grid (FP32)."""
+    shapes = {"xv": (N + 1, N, N), "yv": (N, N + 1, N), "zv": (N, N, N + 1),
+              "qx": (N, N + 1, N + 1), "qy": (N + 1, N, N + 1), "qz": (N + 1, N + 1, N)}
+    return {k: 4 * v[0] * v[1] * v[2] for k, v in shapes.items()}
+
+
+def stage_bytes(stage, ab):
+    """Algorithmic bytes of one traced stage: a line pass reads and writes its
+    array once; the projection reads three velocity and writes three potential
+    arrays once."""
+    if stage == "project":
+        return sum(ab[k] for k in ("xv", "yv", "zv", "qx", "qy", "qz"))
+    return 2 * ab[stage.split()[1]]
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the calibration
+    kernel's reading), or None when the summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "traffic_recover3d_256.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)
+    except (OSError, ValueError):
+        return None
+    row = rows.get("kernels", {}).get(kernel)
+    if not row:
+        return None
+    return {"bytes": int(row["fetch_bytes"] + row["write_bytes"]),
+            "source": f"profiles/traffic_recover3d_256.json ({rows.get('command', '')})"}
 
 
 def main():
@@ -61,7 +80,6 @@ def main():
     import torch
 
     import waveletpotentials_amd as W
-    from waveletpotentials_amd import _lib
     from waveletpotentials_amd import inputs as I
 
     N = args.n
@@ -96,68 +114,40 @@ def main():
                              / sum(np.sum(b ** 2) for b in (xv, yv, zv))))
     max_div = float(np.abs(I.div3d(*curl, dx)).max())
 
-    # ---- per-launch timing of every stage with HIP events on the library's stream ----
-    L = _lib.lib()
-    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    fwd, inv = stage_plan(N)
-    bufs = {"xv": d_xv.clone(), "yv": d_yv.clone(), "zv": d_zv.clone(), "qx": qx, "qy": qy,
-            "qz": qz}
-    lv = int(np.log2(N))
-    stages = []
-
-    def time_launch(fn, reps):
-        for _ in range(2):
-            fn()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
-        for _ in range(reps):
-            fn()
-        e.record()
-        e.synchronize()
-        return s.elapsed_time(e) / reps  # ms
-
-    scratch = {k: torch.empty_like(v) for k, v in bufs.items()}
-    for name, dims, axis, (bc, wt) in fwd + inv:
-        src = ctypes.c_void_p(bufs[name].data_ptr())
-        dst = ctypes.c_void_p(scratch[name].data_ptr())
-        is_fwd = (name[0] != "q")
-
-        def fn(src=src, dst=dst, dims=dims, axis=axis, bc=bc, wt=wt, is_fwd=is_fwd):
-            _lib.check(L.whhd_transform1d(dst, src, *dims, lv, axis.encode(), bc.encode(), wt,
-                                          int(is_fwd), stream), "transform1d")
-        ms = time_launch(fn, args.stage_reps)
-        nbytes = 2 * 4 * int(np.prod(dims))
-        stages.append({"stage": f"{'fwd' if is_fwd else 'inv'} {name} {axis}",
-                       "kernel": f"lp_kernel<{'CDF4.6 d' if wt == 7 else 'CDF3.7 n'},L={lv},"
-                                 f"{'fwd' if is_fwd else 'inv'},"
-                                 f"{'contiguous' if axis == 'y' else 'strided'}>",
-                       "us": ms * 1e3, "bytes": nbytes})
-    proj = W.CuDivLocalProject3D((N + 1,) * 3, (lv,) * 3, dx, T.WT_CDF4_6, "d")
-    ms = time_launch(lambda: proj.ProjectLocal_q_ccc(qx, qy, qz, bufs["xv"], bufs["yv"],
-                                                     bufs["zv"], (lv,) * 3), args.stage_reps)
-    pbytes = 4 * (sum(int(np.prod(s)) for s in ((N + 1, N, N),) * 3)
-                  + sum(int(np.prod(s)) for s in ((N + 1, N + 1, N),) * 3))
-    stages.append({"stage": "project", "kernel": "project3d_kernel", "us": ms * 1e3,
-                   "bytes": pbytes})
-    # calibration copy of a known size (for scaling rocprof byte counters)
+    # ---- per-launch durations in context: HIP events recorded by the library
+    # after every launch of K further solves, on the stream it launches on ----
+    rec.TraceBegin(args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rec.Solve(qx, qy, qz, d_xv, d_yv, d_zv)
+    torch.cuda.synchronize()
+    traced_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    stages, solves = rec.TraceEnd()
+    ab = array_bytes(N)
+    for st in stages:
+        st["bytes"] = stage_bytes(st["stage"], ab)
+    # calibration kernel of known size (16 Mi floats read, 16 Mi written) for
+    # scaling rocprof byte counters
     cal_src = torch.empty(16 * 1024 * 1024, device="cuda")
     cal_dst = torch.empty_like(cal_src)
     for _ in range(3):
-        torch.mul(cal_src, 1.0, out=cal_dst)  # an elementwise kernel distinct from the copies
+        torch.mul(cal_src, 1.0, out=cal_dst)
     torch.cuda.synchronize()
-    stage_total_us = sum(s["us"] for s in stages)
-    stage_bytes = sum(s["bytes"] for s in stages)
+    stage_total_us = sum(st["us"] for st in stages)
+    bytes_per_solve = sum(st["bytes"] for st in stages)
     # dominant kernel = the instantiation with the largest summed time
     by_kernel = {}
-    for s in stages:
-        k = by_kernel.setdefault(s["kernel"], {"us": 0.0, "bytes": 0, "launches": 0})
-        k["us"] += s["us"]
-        k["bytes"] += s["bytes"]
+    for st in stages:
+        k = by_kernel.setdefault(st["kernel"], {"us": 0.0, "bytes": 0, "launches": 0})
+        k["us"] += st["us"]
+        k["bytes"] += st["bytes"]
         k["launches"] += 1
     dom_name, dom = max(by_kernel.items(), key=lambda kv: kv[1]["us"])
     dom_us = dom["us"] / dom["launches"]
     dom_bytes = dom["bytes"] / dom["launches"]
     achieved = dom_bytes / (dom_us * 1e-6) / 1e9
+    traffic = measured_traffic(dom_name)
 
     out = {
         "metric": METRIC,
@@ -174,17 +164,21 @@ def main():
         "data": "synthetic (curl of a seeded Fourier potential, |k|<=16, |k|^-11/6)",
         "config": {"workload": f"3D vector-potential recovery {N}^3 MAC grid, curl CDF3.7 'n', "
                                f"div CDF4.6 'd', 19 stages", "cells": cells},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes_per_launch": int(dom_bytes),
-                     "avg_launch_us": round(dom_us, 2)},
-        "stage_model": {"bytes_per_solve": int(stage_bytes),
-                        "solve_GBs": round(stage_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-                        "solve_frac_of_peak": round(stage_bytes / (ms_per_step * 1e-3) / 1e9
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "kernel": dom_name, "launches_per_solve": dom["launches"],
+                     "algorithmic_bytes_per_launch": int(dom_bytes),
+                     "avg_launch_us": round(dom_us, 2),
+                     "traffic_source": traffic["source"] if traffic else None},
+        "stage_model": {"bytes_per_solve": int(bytes_per_solve),
+                        "solve_GBs": round(bytes_per_solve / (ms_per_step * 1e-3) / 1e9, 1),
+                        "solve_frac_of_peak": round(bytes_per_solve / (ms_per_step * 1e-3) / 1e9
                                                     / HBM_PEAK_GBS, 4),
-                        "sum_of_isolated_stage_us": round(stage_total_us, 1),
+                        "traced_ms_per_step": round(traced_ms, 4), "traced_solves": solves,
+                        "sum_of_stage_us": round(stage_total_us, 1),
                         "stages": [{k: (round(v, 2) if isinstance(v, float) else v)
-                                    for k, v in s.items()} for s in stages]},
+                                    for k, v in st.items()} for st in stages]},
         "accuracy": {"curl_q_vs_u_rel_l2": curl_err, "max_abs_div_curl_q": max_div},
     }
 

@@ -138,6 +138,20 @@ WHHD_API int whhd_recover3d_solve_proj(whhd_recover3d_t h, float* qx, float* qy,
 WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out,
                                 float* qz_out, const float* qx, const float* qy,
                                 const float* qz, int level, void* stream);
+/* Per-stage timing (no reference counterpart; the reference times whole
+ * solves on the host, WaveletPotentials/TestLocalProjection.cpp:757-775).
+ * After trace_begin, the next max_solves calls of solve / solve_proj / cut
+ * record a HIP event on their stream after every launch (no synchronization).
+ * trace_end waits for the last recorded event, writes the average duration of
+ * each stage in microseconds to stage_us[0 .. min(n_stages, max_stages)),
+ * the stage and solve counts to *n_stages / *n_solves (either may be NULL),
+ * and keeps the labels ("<fwd|inv> <array> <axis> | <kernel>") readable by
+ * trace_label until the next trace_begin or destroy.  Only solves with the
+ * stage list of the first recorded one are averaged. */
+WHHD_API int whhd_recover3d_trace_begin(whhd_recover3d_t h, int max_solves);
+WHHD_API int whhd_recover3d_trace_end(whhd_recover3d_t h, float* stage_us, int max_stages,
+                                      int* n_stages, int* n_solves);
+WHHD_API const char* whhd_recover3d_trace_label(whhd_recover3d_t h, int stage);
 WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h);
 
 typedef struct whhd_recover2d_s* whhd_recover2d_t;
@@ -149,6 +163,56 @@ WHHD_API int whhd_recover2d_create(int nx, int ny, double dx, double dy, char bc
 WHHD_API int whhd_recover2d_solve(whhd_recover2d_t h, float* qz, const float* xv,
                                   const float* yv, void* stream);
 WHHD_API void whhd_recover2d_destroy(whhd_recover2d_t h);
+
+/* ---- solid geometry ---------------------------------------------------- */
+/* CuSolidLevelSet3D::InitLsSphere (WaveletPotentials/cuSolidLevelSet3D.cu:11-26,
+ * 376-383) for a union of `count` (1..64) spheres: ls (nx, ny, nz) receives
+ * min_i |p - c_i| - r_i at the cell centres p = ((x+.5)dx, (y+.5)dy, (z+.5)dz);
+ * negative inside a solid.  centers (3*count) and radii (count) are host arrays. */
+WHHD_API int whhd_levelset_spheres(float* ls, int nx, int ny, int nz, double dx, double dy,
+                                   double dz, const double* centers, const double* radii,
+                                   int count, void* stream);
+/* CuSolidLevelSet3D::GetFrac (WaveletPotentials/cuSolidLevelSet3D.cu:348-353 with
+ * cuComputeFrac :28-34 and cuGetAx/Ay/Az :36-90): wet face fractions
+ * ax (nx+1, ny, nz), ay (nx, ny+1, nz), az (nx, ny, nz+1) of the cell level set ls. */
+WHHD_API int whhd_frac_from_ls(float* ax, float* ay, float* az, const float* ls, int nx, int ny,
+                               int nz, double dx, double dy, double dz, void* stream);
+/* CuFastSweeping::Solve(ls, dim, dx, n_iters) (WaveletPotentials/cuFastSweeping.cu:
+ * 215-257): n_iters relaxations of the eikonal update on cells with ls > -1e-12
+ * (solid cells stay fixed).  Each relaxation reads the previous iterate
+ * (the reference updates in place, so its result depends on thread order);
+ * `scratch` is a caller-owned array of nx*ny*nz floats, distinct from ls. */
+WHHD_API int whhd_eikonal_sweep(float* ls, float* scratch, int nx, int ny, int nz, double h,
+                                int n_iters, void* stream);
+
+/* ---- fractional-area multigrid ----------------------------------------- */
+typedef struct whhd_mgfrac3d_s* whhd_mgfrac3d_t;
+/* CuMultigridFrac3D(Ax, Ay, Az, int3 dim, DTYPE3 dx, char bc)
+ * (WaveletPotentials/cuMultigridFrac3D.cuh:15, .cu:9-134): copies the face
+ * fractions ax (nx+1, ny, nz), ay (nx, ny+1, nz), az (nx, ny, nz+1) and builds
+ * the level hierarchy (extents halved rounding up until a side reaches 1;
+ * fractions restricted, spacings summed).  bc 'n' closes the walls; 'd' puts
+ * a zero Dirichlet value on the wall faces (the reference's 'z' with zero
+ * exterior spacing); 'z' uses the cell spacing as exterior spacing.
+ * Synchronizes the null stream. */
+WHHD_API int whhd_mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx,
+                                  int ny, int nz, double dx, double dy, double dz, char bc,
+                                  whhd_mgfrac3d_t* out);
+/* Extents of the levels (3 ints each, finest first) into dims[0 .. 3*max_levels);
+ * returns the number of levels (-1 for a null handle). */
+WHHD_API int whhd_mgfrac3d_levels(whhd_mgfrac3d_t h, int* dims, int max_levels);
+/* Solve(dst, pre_v, f, n_iters, nu1, nu2, weight) (WaveletPotentials/cuMultigridFrac3D.cu:
+ * 249-327): n_iters V-cycles with nu1 / nu2 weighted-Jacobi sweeps before /
+ * after the coarse correction, 50 ('n') or 20 sweeps on the coarsest level,
+ * starting from pre_v (NULL: zero).  dst, pre_v, f are (nx, ny, nz) cell arrays;
+ * dst may equal pre_v. */
+WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre_v, const float* f,
+                                 int n_iters, int nu1, int nu2, float weight, void* stream);
+/* ||f - A v||_2 / ||f||_2 with the sums in FP64 (CuWeightedJacobi3D::GetFracRhs,
+ * WaveletPotentials/cuWeightedJacobi3D.cu:1384-1407).  Synchronizes `stream`. */
+WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f, double* rel,
+                                    void* stream);
+WHHD_API void whhd_mgfrac3d_destroy(whhd_mgfrac3d_t h);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,16 @@ def _setup(L):
     L.orc_curl3d.argtypes = [_D] * 6 + [i, i, i, d, d, d]
     L.orc_curl2d.argtypes = [_D] * 3 + [i, i, d, d]
     L.orc_div3d.argtypes = [_D] * 4 + [i, i, i, d, d, d]
+    L.orc_levelset_spheres.argtypes = [_D, i, i, i, d, d, d, _D, _D, i]
+    L.orc_frac_from_ls.argtypes = [_D] * 4 + [i, i, i, d, d, d]
+    L.orc_eikonal_update.argtypes = [d] * 6
+    L.orc_eikonal_update.restype = d
+    L.orc_eikonal_sweep.argtypes = [_D, i, i, i, d, i]
+    L.orc_mgfrac3d_levels.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_int)]
+    L.orc_mgfrac3d_solve.argtypes = [_D] * 6 + [i, i, i, d, d, d, c, i, i, i, d]
+    L.orc_wj_frac3d.argtypes = [_D] * 6 + [i, i, i, d, d, d, c, d, i]
+    L.orc_mgfrac3d_rel_residual.argtypes = [_D] * 5 + [i, i, i, d, d, d, c]
+    L.orc_mgfrac3d_rel_residual.restype = d
     L.orc_curl3d.restype = None
     L.orc_curl2d.restype = None
     L.orc_div3d.restype = None
@@ -200,3 +210,82 @@ def rel_l2(a, b):
     a = np.concatenate([np.ravel(x) for x in (a if isinstance(a, (list, tuple)) else [a])])
     b = np.concatenate([np.ravel(x) for x in (b if isinstance(b, (list, tuple)) else [b])])
     return float(np.sqrt(np.sum((a - b) ** 2) / (np.sum(a ** 2) + 1e-300)))
+
+
+# ---- solids ----------------------------------------------------------------
+def levelset_spheres(n, dx, centers, radii):
+    """Union-of-spheres signed distance at cell centres; returns (nz, nx, ny)."""
+    nx, ny, nz = n
+    c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+    ls = field(nx, ny, nz)
+    _check(lib().orc_levelset_spheres(_p(ls), nx, ny, nz, *map(float, dx), _p(c), _p(r), len(r)),
+           "levelset_spheres")
+    return ls
+
+
+def frac_from_ls(ls, dx):
+    """Face fractions (ax, ay, az) from a cell level set of shape (nz, nx, ny)."""
+    ls = np.ascontiguousarray(ls, dtype=np.float64)
+    nz, nx, ny = ls.shape
+    ax, ay, az = field(nx + 1, ny, nz), field(nx, ny + 1, nz), field(nx, ny, nz + 1)
+    _check(lib().orc_frac_from_ls(_p(ax), _p(ay), _p(az), _p(ls), nx, ny, nz, *map(float, dx)),
+           "frac_from_ls")
+    return ax, ay, az
+
+
+def eikonal_update(mx, my, mz, dx, dy, dz):
+    return lib().orc_eikonal_update(mx, my, mz, dx, dy, dz)
+
+
+def eikonal_sweep(ls, h, n_iters):
+    """n_iters Jacobi relaxations of the positive region; returns a new array."""
+    out = np.array(ls, dtype=np.float64, order="C", copy=True)
+    nz, nx, ny = out.shape
+    _check(lib().orc_eikonal_sweep(_p(out), nx, ny, nz, float(h), int(n_iters)), "eikonal_sweep")
+    return out
+
+
+# ---- fractional multigrid ---------------------------------------------------
+def mg_levels(n):
+    nx, ny, nz = n
+    out = (ctypes.c_int * 96)()
+    k = lib().orc_mgfrac3d_levels(nx, ny, nz, out)
+    return [tuple(out[3 * i:3 * i + 3]) for i in range(k)]
+
+
+def _fr(ax, ay, az):
+    return [np.ascontiguousarray(a, dtype=np.float64) for a in (ax, ay, az)]
+
+
+def mg_solve(f, ax, ay, az, dx, bc="n", n_iters=10, nu1=3, nu2=3, weight=0.8, pre_v=None):
+    """CuMultigridFrac3D(ax, ay, az, dim, dx, bc).Solve(dst, pre_v, f, ...); f is (nz, nx, ny)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, nx, ny = f.shape
+    ax, ay, az = _fr(ax, ay, az)
+    dst = np.empty_like(f)
+    pv = None if pre_v is None else np.ascontiguousarray(pre_v, dtype=np.float64)
+    _check(lib().orc_mgfrac3d_solve(_p(dst), _p(pv) if pv is not None else None, _p(f), _p(ax),
+                                    _p(ay), _p(az), nx, ny, nz, *map(float, dx), _b(bc),
+                                    int(n_iters), int(nu1), int(nu2), float(weight)), "mg_solve")
+    return dst
+
+
+def wj_sweep(v, f, ax, ay, az, dx, bc="n", weight=0.8, residual=False):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, nx, ny = v.shape
+    ax, ay, az = _fr(ax, ay, az)
+    out = np.empty_like(v)
+    _check(lib().orc_wj_frac3d(_p(out), _p(v), _p(f), _p(ax), _p(ay), _p(az), nx, ny, nz,
+                               *map(float, dx), _b(bc), float(weight), int(residual)), "wj_sweep")
+    return out
+
+
+def mg_rel_residual(v, f, ax, ay, az, dx, bc="n"):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, nx, ny = v.shape
+    ax, ay, az = _fr(ax, ay, az)
+    return lib().orc_mgfrac3d_rel_residual(_p(v), _p(f), _p(ax), _p(ay), _p(az), nx, ny, nz,
+                                           *map(float, dx), _b(bc))

@@ -88,6 +88,33 @@ void orc_curl2d(double* xv, double* yv, const double* qz, int nx, int ny, double
 void orc_div3d(double* div, const double* xv, const double* yv, const double* zv, int nx,
                int ny, int nz, double dx, double dy, double dz);
 
+/* ---- solids (solids_oracle.c) ---- */
+/* cuInitLsSphere for a union of `count` spheres: min_i |p - c_i| - r_i at cell centres. */
+int orc_levelset_spheres(double* ls, int nx, int ny, int nz, double dx, double dy, double dz,
+                         const double* centers, const double* radii, int count);
+/* cuGetAx/Ay/Az with cuComputeFrac: face fractions ax (nx+1,ny,nz), ay (nx,ny+1,nz), az (nx,ny,nz+1). */
+int orc_frac_from_ls(double* ax, double* ay, double* az, const double* ls, int nx, int ny, int nz,
+                     double dx, double dy, double dz);
+/* solve_eikonal_nomin for neighbour minima along x, y, z and spacings dx, dy, dz. */
+double orc_eikonal_update(double mx, double my, double mz, double dx, double dy, double dz);
+/* n_iters Jacobi relaxations of fast_sweep_kernel_pos (in place on ls). */
+int orc_eikonal_sweep(double* ls, int nx, int ny, int nz, double h, int n_iters);
+/* ---- fractional multigrid (mg_oracle.c) ---- */
+/* Level extents of CuMultigridFrac3D's hierarchy (3 ints per level); returns the count. */
+int orc_mgfrac3d_levels(int nx, int ny, int nz, int* dims_out);
+/* CuMultigridFrac3D(ax, ay, az, dim, dx, bc).Solve(dst, pre_v, f, n_iters, nu1, nu2, weight). */
+int orc_mgfrac3d_solve(double* dst, const double* pre_v, const double* f, const double* ax,
+                       const double* ay, const double* az, int nx, int ny, int nz, double dx,
+                       double dy, double dz, char bc, int n_iters, int nu1, int nu2,
+                       double weight);
+/* One weighted Jacobi sweep dst <- src, or (residual != 0) dst = f - A src, uniform spacing. */
+int orc_wj_frac3d(double* dst, const double* src, const double* f, const double* ax,
+                  const double* ay, const double* az, int nx, int ny, int nz, double dx, double dy,
+                  double dz, char bc, double weight, int residual);
+/* ||f - A v||_2 / ||f||_2 (CuWeightedJacobi3D::GetFracRhs). */
+double orc_mgfrac3d_rel_residual(const double* v, const double* f, const double* ax,
+                                 const double* ay, const double* az, int nx, int ny, int nz,
+                                 double dx, double dy, double dz, char bc);
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,72 @@
+// Host build of the line-parallel transform algorithm (line_parallel.h with
+// WHHD_HOST_EMULATION): one "lane" at a time, through an I/O object that
+// reads and writes a line directly.  Exercises the level/block structure,
+// reflections, streamed look-ahead loads and run stores of the GPU kernels on
+// the CPU; the device-only parts (buffer addressing, LDS transposition) are
+// not covered here.
+//
+//   lp_emulate(dst, src, lines, n, stride, levels, wavelet, forward, look)
+//     wavelet 0: CDF3.7 'n' (n = 2^levels), 1: CDF4.6 'd' (n = 2^levels + 1);
+//     line j is src[j*stride + i], i < n; look = look-ahead (0, 16 or 32).
+#define WHHD_HOST_EMULATION 1
+#include "../../waveletpotentials_amd/csrc/line_parallel.h"
+
+#include <cstdint>
+
+namespace {
+
+template <int LOOK>
+struct HostIO {
+    static constexpr int kLook = LOOK;
+    const float* src;
+    float* dst;
+    int n;
+    float load(int i) const { return src[i]; }
+    template <int START, int CNT>
+    void store_run(const float* vals) const {
+        for (int v = 0; v < CNT; ++v) dst[START + v] = vals[v];
+    }
+};
+
+template <class W, int LEV, int EXT, int LOOK>
+void run(float* dst, const float* src, int64_t lines, int64_t stride, bool fwd) {
+    constexpr int n = (1 << LEV) + EXT;
+    for (int64_t j = 0; j < lines; ++j) {
+        HostIO<LOOK> io{src + j * stride, dst + j * stride, n};
+        if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
+        else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
+    }
+}
+
+template <class W, int EXT, int LOOK>
+int by_level(float* dst, const float* src, int64_t lines, int64_t stride, int levels, bool fwd) {
+    switch (levels) {
+    case 5: run<W, 5, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
+    case 6: run<W, 6, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
+    case 7: run<W, 7, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
+    case 8: run<W, 8, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
+    default: return -1;
+    }
+}
+
+template <int LOOK>
+int by_wavelet(float* dst, const float* src, int64_t lines, int64_t stride, int levels, int wt,
+               bool fwd) {
+    if (wt == 0) return by_level<whhd::W_Cdf37N, 0, LOOK>(dst, src, lines, stride, levels, fwd);
+    if (wt == 1) return by_level<whhd::W_Cdf46D, 1, LOOK>(dst, src, lines, stride, levels, fwd);
+    return -1;
+}
+
+}  // namespace
+
+extern "C" int lp_emulate(float* dst, const float* src, long long lines, int n, long long stride,
+                          int levels, int wt, int fwd, int look) {
+    if (!dst || !src || dst == src || lines < 0 || stride < n) return -1;
+    if (n != (1 << levels) + (wt == 1 ? 1 : 0)) return -1;
+    switch (look) {
+    case 0: return by_wavelet<0>(dst, src, lines, stride, levels, wt, fwd != 0);
+    case 16: return by_wavelet<16>(dst, src, lines, stride, levels, wt, fwd != 0);
+    case 32: return by_wavelet<32>(dst, src, lines, stride, levels, wt, fwd != 0);
+    default: return -1;
+    }
+}

@@ -70,3 +70,9 @@ def test_argument_errors_without_gpu(L):
     assert st == 3  # CDF3.9 has no kernel
     st = L.whhd_transform1d(None, None, 4, 4, 4, 2, b"y", b"n", 11, 1, None)
     assert st == 1
+
+
+def test_trace_calls_reject_null_handle(L):
+    assert L.whhd_recover3d_trace_begin(None, 4) == 1
+    assert L.whhd_recover3d_trace_end(None, None, 0, None, None) == 1
+    assert L.whhd_recover3d_trace_label(None, 0) is None

@@ -144,3 +144,40 @@ def test_recover3d_full_size_properties(whhd, orc, kernel_mode):
     assert np.abs(d).max() < 1e-6 * max(np.abs(x).max() for x in c) / dx[0]
     ref = orc.recover3d(xv, yv, zv, n, dx)
     assert rel(got, ref) < TOL_Q
+
+
+def test_stage_trace(whhd, kernel_mode):
+    """The stage trace of a solve lists its 19 launches (9 forward passes, the
+    projection, 9 inverse passes) and accounts for the solve's time."""
+    import time
+    N = 64
+    n = (N, N, N)
+    dx = (1.0 / N,) * 3
+    q = I.spectral_potential3d(n, 8, I.seed_for(1, 7))
+    v = [cuda(a) for a in I.curl3d(*q, dx)]
+    T = whhd.WaveletType
+    rec = whhd.CuWaveletPotentialRecover3D(n, dx, "n", T.WT_CDF3_7, T.WT_CDF4_6)
+    qs = alloc_q(n)
+    rec.Solve(*qs, *v)
+    rec.TraceBegin(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):  # only the first 3 are recorded
+        rec.Solve(*qs, *v)
+    torch.cuda.synchronize()
+    wall_us = (time.perf_counter() - t0) * 1e6 / 5
+    stages, solves = rec.TraceEnd()
+    assert solves == 3
+    assert [s["stage"] for s in stages] == (
+        [f"fwd {a} {x}" for a in ("xv", "yv", "zv") for x in "xyz"] + ["project"]
+        + [f"inv {a} {x}" for a in ("qx", "qy", "qz") for x in "xyz"])
+    assert stages[9]["kernel"] == "project3d_kernel"
+    want = "lp_kernel" if kernel_mode == 0 else "line_tile_kernel"
+    assert all(s["kernel"].startswith(want) for i, s in enumerate(stages) if i != 9)
+    assert all(s["us"] > 0 for s in stages)
+    assert sum(s["us"] for s in stages) < 2 * wall_us + 100
+    # a second trace starts from scratch
+    rec.TraceBegin(1)
+    rec.Solve(*qs, *v)
+    again, solves = rec.TraceEnd()
+    assert solves == 1 and len(again) == 19

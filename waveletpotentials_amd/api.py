@@ -26,7 +26,7 @@ import torch
 from ._lib import WhhdError, check, lib
 
 __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "string_from_type",
-           "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
+           "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -299,6 +299,24 @@ class CuWaveletPotentialRecover3D(_Handle):
                                        *self._q(qx, qy, qz), int(level), _stream()),
               "CutCoefficients")
 
+    def TraceBegin(self, max_solves):
+        """Record per-stage HIP events during the next `max_solves` solves."""
+        check(lib().whhd_recover3d_trace_begin(self._h, int(max_solves)), "TraceBegin")
+
+    def TraceEnd(self):
+        """Wait for the traced solves; returns ([{stage, kernel, us}], solves)."""
+        cap = 64
+        us = (ctypes.c_float * cap)()
+        ns, nsol = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().whhd_recover3d_trace_end(self._h, us, cap, ctypes.byref(ns), ctypes.byref(nsol)),
+              "TraceEnd")
+        out = []
+        for i in range(min(ns.value, cap)):
+            label = (lib().whhd_recover3d_trace_label(self._h, i) or b"").decode()
+            stage, _, kernel = label.partition(" | ")
+            out.append({"stage": stage, "kernel": kernel, "us": float(us[i])})
+        return out, nsol.value
+
 
 class CuWaveletPotentialRecover2D(_Handle):
     """CuWaveletPotentialRecover2D(int2 dim, DTYPE2 dx, char bc, WaveletType curl,
@@ -318,3 +336,91 @@ class CuWaveletPotentialRecover2D(_Handle):
                                          _ptr(xv, numel=(nx + 1) * ny, name="xv"),
                                          _ptr(yv, numel=nx * (ny + 1), name="yv"), _stream()),
               "Solve")
+
+
+class CuSolidLevelSet3D:
+    """CuSolidLevelSet3D(int3 dim, DTYPE3 dx) (WaveletPotentials/cuSolidLevelSet3D.cuh:8-31):
+    a cell-centred solid level set `ls` (shape (nz, nx, ny)), negative inside solids."""
+
+    def __init__(self, dim, dx):
+        self.dim = tuple(int(v) for v in dim)
+        self.dx = tuple(float(v) for v in dx)
+        nx, ny, nz = self.dim
+        self.ls = torch.empty((nz, nx, ny), dtype=torch.float32, device="cuda")
+
+    def InitLsSphere(self, center, radius):
+        """cuInitLsSphere (WaveletPotentials/cuSolidLevelSet3D.cu:11-26, 370-374)."""
+        self.InitLsSpheres([center], [radius])
+
+    def InitLsSpheres(self, centers, radii):
+        """Union of spheres: ls = min_i |p - c_i| - r_i (1..64 spheres)."""
+        c = (ctypes.c_double * (3 * len(radii)))(*[float(v) for cc in centers for v in cc])
+        r = (ctypes.c_double * len(radii))(*[float(v) for v in radii])
+        check(lib().whhd_levelset_spheres(_ptr(self.ls, name="ls"), *self.dim, *self.dx, c, r,
+                                          len(radii), _stream()), "InitLsSpheres")
+
+    def GetFrac(self, ax, ay, az):
+        """GetFrac(ax, ay, az) (WaveletPotentials/cuSolidLevelSet3D.cu:348-353)."""
+        nx, ny, nz = self.dim
+        check(lib().whhd_frac_from_ls(_ptr(ax, numel=(nx + 1) * ny * nz, name="ax"),
+                                      _ptr(ay, numel=nx * (ny + 1) * nz, name="ay"),
+                                      _ptr(az, numel=nx * ny * (nz + 1), name="az"),
+                                      _ptr(self.ls, name="ls"), *self.dim, *self.dx, _stream()),
+              "GetFrac")
+
+
+class CuFastSweeping:
+    """CuFastSweeping (WaveletPotentials/cuFastSweeping.cuh): eikonal relaxation."""
+    _instance = None
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def Solve(self, ls, dim, dx, n_iters):
+        """Solve(ls, dim, dx, n_iters) (WaveletPotentials/cuFastSweeping.cu:250-257);
+        the scratch iterate comes from torch's allocator."""
+        nx, ny, nz = (int(v) for v in dim)
+        p = _ptr(ls, numel=nx * ny * nz, name="ls")
+        scratch = torch.empty_like(ls)
+        check(lib().whhd_eikonal_sweep(p, _ptr(scratch, name="scratch"), nx, ny, nz, float(dx),
+                                       int(n_iters), _stream()), "CuFastSweeping.Solve")
+
+
+class CuMultigridFrac3D(_Handle):
+    """CuMultigridFrac3D(DTYPE* Ax, DTYPE* Ay, DTYPE* Az, int3 dim, DTYPE3 dx, char bc='n')
+    (WaveletPotentials/cuMultigridFrac3D.cuh:15-18): fractional-area multigrid."""
+    _destroy = "whhd_mgfrac3d_destroy"
+
+    def __init__(self, Ax, Ay, Az, dim, dx, bc="n"):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        nx, ny, nz = self.dim
+        self._n = nx * ny * nz
+        check(lib().whhd_mgfrac3d_create(_ptr(Ax, numel=(nx + 1) * ny * nz, name="Ax"),
+                                         _ptr(Ay, numel=nx * (ny + 1) * nz, name="Ay"),
+                                         _ptr(Az, numel=nx * ny * (nz + 1), name="Az"),
+                                         nx, ny, nz, *(float(v) for v in dx), _b(bc),
+                                         ctypes.byref(self._h)), "CuMultigridFrac3D")
+
+    def levels(self):
+        dims = (ctypes.c_int * 96)()
+        k = lib().whhd_mgfrac3d_levels(self._h, dims, 32)
+        return [tuple(dims[3 * i:3 * i + 3]) for i in range(k)]
+
+    def Solve(self, dst, pre_v, f, n_iters=10, nu1=2, nu2=2, weight=0.8):
+        """Solve(dst, pre_v, f, n_iters, nu1, nu2, weight) (cuMultigridFrac3D.cu:249-265)."""
+        pv = None if pre_v is None else _ptr(pre_v, numel=self._n, name="pre_v")
+        check(lib().whhd_mgfrac3d_solve(self._h, _ptr(dst, numel=self._n, name="dst"), pv,
+                                        _ptr(f, numel=self._n, name="f"), int(n_iters), int(nu1),
+                                        int(nu2), float(weight), _stream()), "Solve")
+
+    def GetResdualError(self, dst, f):
+        """||f - A dst|| / ||f|| in FP64 (the reference's GetResdualError / GetFracRhs)."""
+        rel = ctypes.c_double(0.0)
+        check(lib().whhd_mgfrac3d_residual(self._h, _ptr(dst, numel=self._n, name="v"),
+                                           _ptr(f, numel=self._n, name="f"), ctypes.byref(rel),
+                                           _stream()), "GetResdualError")
+        return rel.value

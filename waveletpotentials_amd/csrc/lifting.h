@@ -9,14 +9,21 @@
 // the dispatch table WaveletPotentials/cuWHHDForward.cu:744-1101.
 #pragma once
 
+// WHHD_HOST_EMULATION compiles the lane-independent parts (index functions,
+// lifting statements on pair arrays) as plain host C++, for CPU checks of the
+// line-parallel algorithm; the cross-lane moves exist only on the device.
+#ifdef WHHD_HOST_EMULATION
+#define WHHD_DEV inline
+#else
 #include <hip/hip_runtime.h>
+#define WHHD_DEV __device__ __forceinline__
+#endif
 
 #include "wt_tables.h"
 
 namespace whhd {
 
-#define WHHD_DEV __device__ __forceinline__
-
+#ifndef WHHD_HOST_EMULATION
 // Whole-wavefront shifts by one lane on the VALU (DPP wave_shr:1 / wave_shl:1);
 // a lane without a source keeps its own value.  Shifts by m lanes chain m of
 // them, so edge lanes may hold values that differ from a direct m-lane
@@ -58,6 +65,7 @@ WHHD_DEV double lane_get(double v, int src) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+#endif  // WHHD_HOST_EMULATION
 
 // ---------------------------------------------------------------------------
 // Index / sign reflections.  Forward forms receive a = 2k, d = 2k+1; inverse

@@ -16,13 +16,25 @@
 // as the 64-lane windows).
 #pragma once
 
+#ifndef WHHD_HOST_EMULATION
 #include <hip/hip_runtime.h>
+#define WHHD_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
+#else
+#define WHHD_SCHED_BARRIER() ((void)0)
+#endif
 
 #include <cstdint>
 #include <type_traits>
 #include <utility>
 
 #include "lifting.h"
+
+#ifndef WHHD_LOOK_S
+#define WHHD_LOOK_S 32
+#endif
+#ifndef WHHD_LOOK_C
+#define WHHD_LOOK_C 32
+#endif
 
 namespace whhd {
 namespace lp {
@@ -89,11 +101,8 @@ WHHD_DEV void static_for(F&& f) {
 }
 
 constexpr int kBlock = 16;  // pairs written per block
-// Lines per wavefront.  Fewer than 64 leaves lanes idle but doubles the
-// number of wavefronts, which the memory system needs to overlap latency:
-// a 256^3 pass has only ~65k lines, i.e. one 64-line wavefront per SIMD.
-constexpr int kLinesPerWave = 64;
 
+#ifndef WHHD_HOST_EMULATION
 // ---- I/O of one lane's line ----------------------------------------------
 // Strided lines (x and z axes): sample i of the lane's line is at
 // base + i * step; a load of sample i across the wavefront reads 64
@@ -105,6 +114,7 @@ constexpr uint32_t kBufWord3 = 0x00020000;  // raw dword buffer, gfx9 family
 constexpr uint32_t kOutOfRange = 0xF0000000u;
 
 struct StridedIO {
+    static constexpr int kLook = WHHD_LOOK_S;  // samples loaded ahead of use
     __amdgpu_buffer_rsrc_t rs;
     __amdgpu_buffer_rsrc_t rd;
     uint32_t voff_ld;  // byte offset of the lane's line start (loads)
@@ -127,9 +137,11 @@ struct StridedIO {
 };
 
 // Contiguous lines (y axis): the lane's line is n consecutive floats.  Loads
-// fetch 16-byte runs of four samples (dword alignment suffices on gfx950) and
-// the compiler folds the four element reads of a run into one load; the last
-// run of a line is read sample by sample so nothing past the array is touched.
+// fetch 16-byte runs of four samples (dword alignment suffices on gfx950:
+// odd-length lines start at any dword) and the four element reads of a run
+// fold into one load; the last run of a line is read sample by sample so
+// nothing past the line is requested.  Stores go through a buffer resource.
+using U4 = __attribute__((ext_vector_type(4))) uint32_t;
 struct __attribute__((packed, aligned(4))) Quad {
     float v[4];
 };
@@ -137,81 +149,111 @@ constexpr int kStageRun = 32;       // longest run written at once
 constexpr int kStageRow = 64 + 1;   // LDS row: one value of each of the 64 lines
 
 struct ContigIO {
-    const float* src;
-    float* dst;
-    int64_t base;
-    int64_t step;
-    bool valid;
+    static constexpr int kLook = WHHD_LOOK_C;  // samples loaded ahead of use
+    const float* line;    // the lane's own line (loads)
+    __amdgpu_buffer_rsrc_t rd;
+    uint32_t voff_st;     // byte offset of the lane's line, or out of range for lanes that do not store
+    uint32_t first_off;   // byte offset of the line of lane 0
+    int lines_here;       // lines of this wavefront that exist (<= 64)
     int n;
-    float* stage;          // this wavefront's LDS staging area, kStageRun rows
-    int64_t first_line;    // line of lane 0
-    int64_t total_lines;
+    float* stage;         // this wavefront's LDS staging area, kStageRun rows
     int lane;
+    // Loads go through a plain pointer: LLVM narrows a raw buffer load of 16
+    // bytes whose leading elements are unused without moving its offset, so
+    // element extraction from __builtin_amdgcn_raw_buffer_load_b128 returns
+    // the wrong samples; ordinary vector loads are narrowed correctly.
     WHHD_DEV float load(int i) const {
         const int q = i & ~3;
         if (q + 3 < n) {
-            const Quad r = *reinterpret_cast<const Quad*>(src + base + q);
+            const Quad r = *reinterpret_cast<const Quad*>(line + q);
             return r.v[i & 3];
         }
-        return src[base + i];
+        return line[i];
     }
-    // Samples [START, START+CNT) of every lane's line.  The run is transposed
-    // through LDS so that consecutive lanes store consecutive 16-byte pieces
-    // of one line (full 64-128 byte segments per line) instead of 64 lines
-    // touched 16 bytes at a time.
+    // Samples [START, START+CNT) of every lane's line.  Whole 16-byte pieces
+    // are transposed through LDS so that consecutive lanes store consecutive
+    // pieces of one line (64-128 byte segments per line) instead of 64 lines
+    // touched 16 bytes at a time; a remainder of CNT % 4 samples is stored by
+    // each lane for its own line.  Lines past the end get an out-of-range
+    // offset, which the hardware discards, so there is no branch.
     template <int START, int CNT>
     WHHD_DEV void store_run(const float* vals) const {
-        static_assert(CNT <= kStageRun, "run too long");
-        constexpr int NQ = (CNT + 3) / 4;          // 16-byte pieces per line
-        constexpr int LPI = 64 / NQ;               // lines per pass (NQ divides 64 here)
-        static_assert(64 % NQ == 0, "pieces per line must divide 64");
+        constexpr int CQ = CNT & ~3;
+        if constexpr (CQ > 0) {
+            static_assert(CQ <= kStageRun, "run too long");
+            constexpr int NQ = CQ / 4;                 // 16-byte pieces per line
+            constexpr int LPI = 64 / NQ;               // lines per pass
+            static_assert(64 % NQ == 0, "pieces per line must divide 64");
 #pragma unroll
-        for (int v = 0; v < CNT; ++v) stage[v * kStageRow + lane] = vals[v];
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
-        const int q = lane % NQ;
+            for (int v = 0; v < CQ; ++v) stage[v * kStageRow + lane] = vals[v];
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+            const int q = lane % NQ;
 #pragma unroll
-        for (int r = 0; r < NQ; ++r) {
-            const int l = lane / NQ + r * LPI;
-            const int64_t line = first_line + l;
-            const bool mine = l < kLinesPerWave;
-            float w[4];
+            for (int r = 0; r < NQ; ++r) {
+                const int l = lane / NQ + r * LPI;
+                U4 w;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int idx = 4 * q + e;
-                w[e] = idx < CNT ? stage[idx * kStageRow + l] : 0.f;
+                for (int e = 0; e < 4; ++e) w[e] = __builtin_bit_cast(uint32_t, stage[(4 * q + e) * kStageRow + l]);
+                const uint32_t off = l < lines_here ? first_off + (uint32_t)((l * n + START + 4 * q) * 4)
+                                                    : kOutOfRange;
+                __builtin_amdgcn_raw_buffer_store_b128(w, rd, off, 0, 0);
             }
-            if (mine && line < total_lines) {
-                float* out = dst + line * n + START + 4 * q;
-                if (4 * q + 3 < CNT) {
-                    Quad qv;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) qv.v[e] = w[e];
-                    *reinterpret_cast<Quad*>(out) = qv;
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (4 * q + e < CNT) out[e] = w[e];
-                }
-            }
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
         }
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int v = CQ; v < CNT; ++v)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, vals[v]), rd, voff_st,
+                                                  (START + v) * 4, 0);
     }
 };
 
+#endif  // WHHD_HOST_EMULATION
+
+// ---- streamed loads ------------------------------------------------------
+// Both directions consume a line's samples in increasing index order (the
+// forward's top level block by block; the inverse's coarse block, then the
+// details of level 2, 3, ... block by block).  The loads are issued ahead of
+// their use: before a block is computed, the samples up to its need plus a
+// look-ahead (IO::kLook samples) are requested, and a scheduling barrier keeps the compiler
+// from sinking them back to their uses.  Without it the loads of a block are
+// issued right before the block and each block pays the full memory latency.
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int round4(int v) { return (v + 3) & ~3; }
+
+// Issue loads of samples [FROM, TO) into x (FROM >= TO: nothing).
+template <int FROM, int TO, class IO>
+WHHD_DEV void issue(const IO& io, float* x) {
+    if constexpr (FROM < TO) {
+        static_for<FROM, TO>([&](auto i) { x[decltype(i)::value] = io.load(decltype(i)::value); });
+        WHHD_SCHED_BARRIER();
+    }
+}
+
 // ---- forward --------------------------------------------------------------
-// One forward level: the 2^(l+1)+EXT samples come from `in` (or the line when
-// TOP), approximations go to `out`, details to the line at [W, 2W-EXT).
+// Samples of the line a forward top-level block b (pairs [b*kBlock - H,
+// (b+1)*kBlock + H)) can read; reflections at the line end stay below N.
+template <int H, int N>
+constexpr int fwd_need(int b) { return cmin(N, 2 * (b * kBlock + kBlock + H) + 2); }
+template <int H, int N, int LA>
+constexpr int fwd_front(int b) { return b < 0 ? 0 : cmin(N, round4(fwd_need<H, N>(b) + LA)); }
+
+// One forward level: the 2^(l+1)+EXT samples come from `in`, approximations
+// go to `out`, details to the line at [W, 2W-EXT).  For the top level (TOP)
+// `in` is filled here from the line by streamed loads.
 template <class W, int l, int EXT, bool TOP, class IO>
-WHHD_DEV void fwd_level(const IO& io, const float* in, float* out) {
+WHHD_DEV void fwd_level(const IO& io, float* in, float* out) {
     constexpr int H = W::halo;
     constexpr int L = 1 << (l + 1);
+    constexpr int N = L + EXT;
     constexpr int Wd = L / 2 + EXT;
     constexpr int NP = kBlock + 2 * H;
     constexpr int NB = (Wd + kBlock - 1) / kBlock;
     static_for<0, NB>([&](auto bi) {
-        constexpr int b0 = decltype(bi)::value * kBlock;
+        constexpr int b = decltype(bi)::value;
+        constexpr int b0 = b * kBlock;
+        if constexpr (TOP) issue<fwd_front<H, N, IO::kLook>(b - 1), fwd_front<H, N, IO::kLook>(b)>(io, in);
         Arr<float, NP> A, D, SA, SD;
         static_for<0, NP>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
@@ -225,13 +267,8 @@ WHHD_DEV void fwd_level(const IO& io, const float* in, float* out) {
             SA[j] = sa;
             SD[j] = sd;
             if constexpr (k < Wd + H) {
-                if constexpr (TOP) {
-                    A[j] = io.load(ai);
-                    D[j] = io.load(di);
-                } else {
-                    A[j] = in[ai];
-                    D[j] = in[di];
-                }
+                A[j] = in[ai];
+                D[j] = in[di];
             } else {
                 A[j] = 0.f;
                 D[j] = 0.f;
@@ -253,7 +290,7 @@ WHHD_DEV void fwd_level(const IO& io, const float* in, float* out) {
 }
 
 template <class W, int l, int EXT, class IO>
-WHHD_DEV void fwd_levels_below(const IO& io, const float* in) {
+WHHD_DEV void fwd_levels_below(const IO& io, float* in) {
     // `in` holds the 2^(l+1)+EXT approximations produced by level l+1
     if constexpr (l >= 2) {
         float out[(1 << l) + EXT];
@@ -277,30 +314,51 @@ WHHD_DEV void fwd_levels_below(const IO& io, const float* in) {
 
 template <class W, int LEV, int EXT, class IO>
 WHHD_DEV void forward_line_lp(const IO& io) {
+    float x[(1 << LEV) + EXT];
     if constexpr (LEV > 2) {
         float out[(1 << (LEV - 1)) + EXT];
-        fwd_level<W, LEV - 1, EXT, true>(io, nullptr, out);
+        fwd_level<W, LEV - 1, EXT, true>(io, x, out);
         fwd_levels_below<W, LEV - 2, EXT>(io, out);
     } else {
-        float in[(1 << LEV) + EXT];
-        static_for<0, (1 << LEV) + EXT>([&](auto i) { in[decltype(i)::value] = io.load(decltype(i)::value); });
-        fwd_levels_below<W, 1, EXT>(io, in);
+        issue<0, (1 << LEV) + EXT>(io, x);
+        fwd_levels_below<W, 1, EXT>(io, x);
     }
 }
 
 // ---- inverse --------------------------------------------------------------
-// One inverse level l: approximations [0, 2^l+EXT) from `in`, details from the
-// line at [2^l+EXT, 2^(l+1)+EXT); reconstructed samples to `out`, or to the
-// line when LAST.
-template <class W, int l, int EXT, bool LAST, class IO>
-WHHD_DEV void inv_level(const IO& io, const float* in, float* out) {
+// Highest line index (exclusive) block b of inverse level l reads: its
+// details [L+EXT, 2L+EXT) around pairs [b*kBlock - H, (b+1)*kBlock + H).
+template <int H, int EXT, int N>
+constexpr int inv_need(int l, int b) {
+    return cmin(N, (1 << l) + EXT + cmin(1 << l, b * kBlock + kBlock + H + 1));
+}
+template <int H, int EXT, int N, int M, int LA>
+constexpr int inv_front(int l, int b) {
+    return cmin(N, round4((l < 2 ? M : inv_need<H, EXT, N>(l, b)) + LA));
+}
+template <int H, int EXT, int N, int M, int LA>
+constexpr int inv_front_before(int l, int b) {
+    // front of the block consumed before (l, b)
+    return b > 0 ? inv_front<H, EXT, N, M, LA>(l, b - 1)
+                 : (l == 2 ? inv_front<H, EXT, N, M, LA>(1, 0)
+                           : inv_front<H, EXT, N, M, LA>(l - 1, (((1 << (l - 1)) + EXT + kBlock - 1) / kBlock) - 1));
+}
+
+// One inverse level l: approximations [0, 2^l+EXT) from `in`, details from
+// the streamed line x at [2^l+EXT, 2^(l+1)+EXT); reconstructed samples to
+// `out`, or to the line when LAST.
+template <class W, int l, int EXT, int N, bool LAST, class IO>
+WHHD_DEV void inv_level(const IO& io, float* x, const float* in, float* out) {
     constexpr int H = W::halo;
+    constexpr int M = W::m10;
     constexpr int L = 1 << l;
     constexpr int Wd = L + EXT;
     constexpr int NP = kBlock + 2 * H;
     constexpr int NB = (Wd + kBlock - 1) / kBlock;
     static_for<0, NB>([&](auto bi) {
-        constexpr int b0 = decltype(bi)::value * kBlock;
+        constexpr int b = decltype(bi)::value;
+        constexpr int b0 = b * kBlock;
+        issue<inv_front_before<H, EXT, N, M, IO::kLook>(l, b), inv_front<H, EXT, N, M, IO::kLook>(l, b)>(io, x);
         Arr<float, NP> A, D, SA, SD;
         static_for<0, NP>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
@@ -314,7 +372,7 @@ WHHD_DEV void inv_level(const IO& io, const float* in, float* out) {
             SD[j] = sd;
             if constexpr (k < L + H) {
                 A[j] = in[ai];
-                D[j] = io.load(L + EXT + di);
+                D[j] = x[L + EXT + di];
             } else {
                 A[j] = 0.f;
                 D[j] = 0.f;
@@ -341,38 +399,42 @@ WHHD_DEV void inv_level(const IO& io, const float* in, float* out) {
 }
 
 template <class W, int l, int LEV, int EXT, class IO>
-WHHD_DEV void inv_levels_from(const IO& io, const float* in) {
+WHHD_DEV void inv_levels_from(const IO& io, float* x, const float* in) {
     // `in` holds the 2^l+EXT approximations of level l
+    constexpr int N = (1 << LEV) + EXT;
     if constexpr (l == LEV - 1) {
-        inv_level<W, l, EXT, true>(io, in, nullptr);
+        inv_level<W, l, EXT, N, true>(io, x, in, nullptr);
     } else {
         float out[(1 << (l + 1)) + EXT];
-        inv_level<W, l, EXT, false>(io, in, out);
-        inv_levels_from<W, l + 1, LEV, EXT>(io, out);
+        inv_level<W, l, EXT, N, false>(io, x, in, out);
+        inv_levels_from<W, l + 1, LEV, EXT>(io, x, out);
     }
 }
 
 template <class W, int LEV, int EXT, class IO>
 WHHD_DEV void inverse_line_lp(const IO& io) {
     constexpr int M = W::m10;
-    float c[M], a[M];
-    static_for<0, M>([&](auto i) { c[decltype(i)::value] = io.load(decltype(i)::value); });
+    constexpr int N = (1 << LEV) + EXT;
+    float x[N];
+    issue<0, inv_front<W::halo, EXT, N, M, IO::kLook>(1, 0)>(io, x);
+    float a[M];
     static_for<0, M>([&](auto ri) {
         constexpr int r = decltype(ri)::value;
         float acc = 0.f;
         static_for<0, M>([&](auto ci) {
             constexpr int cc = decltype(ci)::value;
-            acc += c[cc] * float(W::il10[r * M + cc]);
+            acc += x[cc] * float(W::il10[r * M + cc]);
         });
         a[r] = acc;
     });
     if constexpr (LEV > 2) {
-        inv_levels_from<W, 2, LEV, EXT>(io, a);
+        inv_levels_from<W, 2, LEV, EXT>(io, x, a);
     } else {
         io.template store_run<0, M>(a);
     }
 }
 
+#ifndef WHHD_HOST_EMULATION
 // ---- kernel -----------------------------------------------------------------
 // Lines are numbered li = outer * lines_inner + inner; line li starts at
 // inner * inner_stride + outer * outer_stride and its samples are `step` apart.
@@ -386,30 +448,33 @@ struct LpGeom {
 };
 
 template <class W, int LEV, int EXT, bool FWD, bool CONTIG>
-__global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const float* __restrict__ src,
-                                                 LpGeom g) {
+// Two wavefronts per SIMD: a pass over 256^3 has about one wavefront of lines
+// per SIMD, so the register budget goes to loads in flight, not occupancy.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
     const int lane_ = threadIdx.x & 63;
-    const int64_t wave_first = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kLinesPerWave;
-    const int64_t li = wave_first + (lane_ % kLinesPerWave);
-    const bool in_range = li < g.total_lines;
-    const bool valid = in_range && lane_ < kLinesPerWave;  // lanes past kLinesPerWave mirror, never store
-    const int64_t l = in_range ? li : 0;
+    const int64_t wave_first = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    const int64_t li = wave_first + lane_;
+    const bool valid = li < g.total_lines;  // lanes past the last line load line 0, never store
+    const int64_t l = valid ? li : 0;
     const int64_t outer = l / g.lines_inner;
     const int64_t inner = l - outer * g.lines_inner;
     constexpr int n = (1 << LEV) + EXT;
+    const uint32_t bytes = (uint32_t)(g.total_elems * 4);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)src, 0, bytes, kBufWord3);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void*)dst, 0, bytes, kBufWord3);
     if constexpr (CONTIG) {
+        // contiguous lines are numbered in memory order: line li starts at li * n
         __shared__ float stage[4][kStageRun * kStageRow];
-        const int lane = threadIdx.x & 63;
-        ContigIO io{src, dst, inner * g.inner_stride + outer * g.outer_stride, g.step, valid, n,
-                    stage[threadIdx.x >> 6], wave_first, g.total_lines, lane};
+        const int64_t here = g.total_lines - wave_first;
+        const uint32_t voff = (uint32_t)(l * n * 4);
+        ContigIO io{src + l * n, rd, valid ? voff : kOutOfRange, (uint32_t)(wave_first * n * 4),
+                    (int)(here < 64 ? here : 64), n, stage[threadIdx.x >> 6], lane_};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     } else {
-        const uint32_t bytes = (uint32_t)(g.total_elems * 4);
         const uint32_t voff = (uint32_t)((inner * g.inner_stride + outer * g.outer_stride) * 4);
-        StridedIO io{__builtin_amdgcn_make_buffer_rsrc((void*)src, 0, bytes, kBufWord3),
-                     __builtin_amdgcn_make_buffer_rsrc((void*)dst, 0, bytes, kBufWord3),
-                     voff, valid ? voff : kOutOfRange, (int)(g.step * 4), n};
+        StridedIO io{rs, rd, voff, valid ? voff : kOutOfRange, (int)(g.step * 4), n};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     }
@@ -419,7 +484,7 @@ __global__ void __launch_bounds__(256) lp_kernel(float* __restrict__ dst, const 
 template <class W, int EXT, bool FWD, bool CONTIG>
 bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hipStream_t s) {
     const dim3 block(256);
-    const int64_t lines_per_block = 4 * kLinesPerWave;
+    const int64_t lines_per_block = 4 * 64;
     const dim3 grid((unsigned)((g.total_lines + lines_per_block - 1) / lines_per_block));
     switch (levels) {
     case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
@@ -430,8 +495,11 @@ bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hi
     }
 }
 
+#endif  // WHHD_HOST_EMULATION
+
 }  // namespace lp
 
+#ifndef WHHD_HOST_EMULATION
 template <class W, int EXT>
 bool lp_launch(int levels, bool fwd, bool contig, float* dst, const float* src,
                const lp::LpGeom& g, hipStream_t s) {
@@ -441,5 +509,7 @@ bool lp_launch(int levels, bool fwd, bool contig, float* dst, const float* src,
     return fwd ? lp::lp_launch_one<W, EXT, true, false>(levels, dst, src, g, s)
                : lp::lp_launch_one<W, EXT, false, false>(levels, dst, src, g, s);
 }
+
+#endif  // WHHD_HOST_EMULATION
 
 }  // namespace whhd

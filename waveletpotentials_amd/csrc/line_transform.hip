@@ -14,6 +14,8 @@
 // sample-major staging loads are both free of bank conflicts.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include <cstdint>
 
 #include "lifting.h"
@@ -284,7 +286,12 @@ int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, ch
     if (g.tile <= 0) return WHHD_INVALID_ARG;
     if ((size_t)2 * g.tile * g.lds_row * sizeof(T) > kMaxLdsBytes) return WHHD_INVALID_ARG;
     hipError_t e = dispatch<T>(wt, bc, dst, src, g, fwd, strided, c.threads, stream);
-    return e == hipSuccess ? WHHD_OK : WHHD_HIP_ERROR;
+    char name[96];
+    std::snprintf(name, sizeof(name), "line_tile_kernel<%s %c,L=%d,%s,%s,%s>", wavelet_name(wt), bc,
+                  levels, fwd ? "fwd" : "inv", strided ? "strided" : "contiguous",
+                  sizeof(T) == sizeof(float) ? "f32" : "f64");
+    set_last_kernel(name);
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "line tile kernel");
 }
 
 template int transform_lines<float>(float*, const float*, int, int, int, int, char, char, int, bool,

@@ -4,6 +4,8 @@
 // lp_cdf37n.hip and lp_cdf46d.hip.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include "line_parallel.h"
 #include "whhd_internal.h"
 
@@ -44,6 +46,10 @@ int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int level
     else if (wt == WT_CDF4_6 && bc == 'd' && n == (1 << levels) + 1)
         ok = lp_launch<W_Cdf46D, 1>(levels, fwd, contig, dst, src, g, stream);
     if (!ok) return -1;
+    char name[96];
+    std::snprintf(name, sizeof(name), "lp_kernel<%s %c,L=%d,%s,%s>", wavelet_name(wt), bc, levels,
+                  fwd ? "fwd" : "inv", contig ? "contiguous" : "strided");
+    set_last_kernel(name);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "line-parallel kernel");
 }

@@ -1,0 +1,568 @@
+// Fractional-area multigrid for the pressure Poisson problem with solids:
+// CuMultigridFrac3D (WaveletPotentials/cuMultigridFrac3D.cu:9-327) with the
+// weighted Jacobi smoother, residual, restriction and prolongation of
+// WaveletPotentials/cuWeightedJacobi3D.cu:620-1722, cuRestrict3D.cu:524-1140 and
+// cuProlongate3D.cu:353-616.
+//
+// Every level is a y-fastest cell grid with face fractions ax (nx+1, ny, nz),
+// ay (nx, ny+1, nz), az (nx, ny, nz+1) and per-axis cell spacings.  All
+// kernels are streaming stencils (HBM-bound): one thread per output cell, a
+// (64, 4) block over (y, x) and one z-plane per grid row.  The handle owns
+// the hierarchy; solves enqueue on the caller's stream without host
+// synchronization (the residual queries read back two FP64 sums).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "whhd_internal.h"
+
+namespace whhd {
+namespace {
+
+struct Dim3 {
+    int x, y, z;
+};
+
+int half_up(int n) { return (n & 1) ? (n >> 1) + 1 : n >> 1; }
+
+dim3 grid_of(int nx, int ny, int nz) { return dim3((ny + 63) / 64, (nx + 3) / 4, nz); }
+
+// Coefficient of one face, cuWjFrac3D_n / _z: A * 2 / ((h_nbr + h) * h).
+__device__ __forceinline__ float face_cf(float a, float hn, float h) {
+    return a / (hn + h) / h * 2.f;
+}
+
+struct Level {
+    Dim3 d;
+    float *f = nullptr, *v = nullptr, *t = nullptr;  // rhs, iterate, Jacobi partner
+    float *ax = nullptr, *ay = nullptr, *az = nullptr;
+    float *hx = nullptr, *hy = nullptr, *hz = nullptr;
+};
+
+struct LevelArgs {
+    const float *ax, *ay, *az, *hx, *hy, *hz;
+    int nx, ny, nz;
+    float ex, ey, ez;  // exterior spacing at the walls ('z')
+};
+
+// Diagonal and off-diagonal sum of cell (x, y, z) applied to src.  Z: walls
+// carry a Dirichlet coefficient with the exterior spacing; otherwise closed.
+template <bool Z>
+__device__ __forceinline__ void stencil(const LevelArgs& a, const float* __restrict__ src, int x,
+                                        int y, int z, int64_t i, float& diag, float& off) {
+    const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
+    const float hx = a.hx[x], hy = a.hy[y], hz = a.hz[z];
+    diag = 0.f;
+    off = 0.f;
+    // y faces: ay index (y, x, z) with extents (nx, ny+1, nz)
+    const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
+    if (y > 0 || Z) {
+        const float cf = face_cf(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, hy);
+        diag += cf;
+        if (y > 0) off -= cf * src[i - 1];
+    }
+    if (y < a.ny - 1 || Z) {
+        const float cf = face_cf(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, hy);
+        diag += cf;
+        if (y < a.ny - 1) off -= cf * src[i + 1];
+    }
+    const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
+    if (x > 0 || Z) {
+        const float cf = face_cf(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, hx);
+        diag += cf;
+        if (x > 0) off -= cf * src[i - sx];
+    }
+    if (x < a.nx - 1 || Z) {
+        const float cf = face_cf(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, hx);
+        diag += cf;
+        if (x < a.nx - 1) off -= cf * src[i + sx];
+    }
+    if (z > 0 || Z) {
+        const float cf = face_cf(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, hz);
+        diag += cf;
+        if (z > 0) off -= cf * src[i - sz];
+    }
+    if (z < a.nz - 1 || Z) {
+        const float cf = face_cf(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, hz);
+        diag += cf;
+        if (z < a.nz - 1) off -= cf * src[i + sz];
+    }
+}
+
+// One weighted Jacobi sweep dst <- src (cuWjFrac3D_n / _z); cells whose
+// faces are all closed get 0.
+template <bool Z>
+__global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                              const float* __restrict__ f, LevelArgs a, float w) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= a.ny || x >= a.nx) return;
+    const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+    float diag, off;
+    stencil<Z>(a, src, x, y, z, i, diag, off);
+    dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
+}
+
+// Residual f - A v (cuWjGetFracRes_3d_n / _z).
+template <bool Z>
+__global__ void residual_kernel(float* __restrict__ r, const float* __restrict__ v,
+                                const float* __restrict__ f, LevelArgs a) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= a.ny || x >= a.nx) return;
+    const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+    float diag, off;
+    stencil<Z>(a, v, x, y, z, i, diag, off);
+    r[i] = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
+}
+
+// SolveP1RN with spacings (cuRestrict3D.cu:939-970): the coarse value is the
+// spacing-weighted average of the up-to-2x2x2 children, axis by axis.
+__global__ void restrict_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                                const float* __restrict__ hx, const float* __restrict__ hy,
+                                const float* __restrict__ hz, int nx, int ny, int nz, int cx,
+                                int cy, int cz) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= cy || x >= cx) return;
+    const int x0 = 2 * x, y0 = 2 * y, z0 = 2 * z;
+    const int x1 = x0 + 1 < nx ? x0 + 1 : -1, y1 = y0 + 1 < ny ? y0 + 1 : -1,
+              z1 = z0 + 1 < nz ? z0 + 1 : -1;
+    const int64_t sx = ny, sz = (int64_t)nx * ny;
+    float accz = 0.f, wz = 0.f;
+    for (int kz = 0; kz < 2; ++kz) {
+        const int zz = kz ? z1 : z0;
+        if (zz < 0) continue;
+        float accx = 0.f, wx = 0.f;
+        for (int kx = 0; kx < 2; ++kx) {
+            const int xx = kx ? x1 : x0;
+            if (xx < 0) continue;
+            const int64_t b = xx * sx + zz * sz;
+            float accy = hy[y0] * src[b + y0], wy = hy[y0];
+            if (y1 >= 0) {
+                accy += hy[y1] * src[b + y1];
+                wy += hy[y1];
+            }
+            accx += hx[xx] * (accy / wy);
+            wx += hx[xx];
+        }
+        accz += hz[zz] * (accx / wx);
+        wz += hz[zz];
+    }
+    dst[y + (int64_t)x * cy + (int64_t)z * cx * cy] = accz / wz;
+}
+
+// SolveAddP1RN (cuProlongate3D.cu:594-616): every fine cell adds its parent.
+__global__ void prolong_add_kernel(float* __restrict__ v, const float* __restrict__ vc, int nx,
+                                   int ny, int nz, int cx, int cy) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= ny || x >= nx) return;
+    v[y + (int64_t)x * ny + (int64_t)z * nx * ny] +=
+        vc[(y >> 1) + (int64_t)(x >> 1) * cy + (int64_t)(z >> 1) * cx * cy];
+}
+
+// SolveP1Ax/Ay/Az (cuRestrict3D.cu:972-1140) for one face array: along the face
+// normal the coarse face X takes the fine face min(2X, F-1); along the other
+// axes it averages the one or two children.  (fx, fy, fz) are the fine face
+// array's extents, (cx, cy, cz) the coarse ones; `normal` is 0, 1 or 2.
+__global__ void restrict_faces_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                                      int fx, int fy, int fz, int cx, int cy, int cz, int normal) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= cy || x >= cx) return;
+    int lo[3], hi[3];
+    const int c[3] = {x, y, z}, F[3] = {fx, fy, fz};
+    for (int k = 0; k < 3; ++k) {
+        if (k == normal) {
+            lo[k] = hi[k] = 2 * c[k] < F[k] - 1 ? 2 * c[k] : F[k] - 1;
+        } else {
+            lo[k] = 2 * c[k];
+            hi[k] = 2 * c[k] + 1 < F[k] ? 2 * c[k] + 1 : 2 * c[k];
+        }
+    }
+    // separable average: y innermost, then x, then z (the reference's order)
+    float accz = 0.f;
+    for (int zz = lo[2]; zz <= hi[2]; ++zz) {
+        float accx = 0.f;
+        for (int xx = lo[0]; xx <= hi[0]; ++xx) {
+            float accy = 0.f;
+            for (int yy = lo[1]; yy <= hi[1]; ++yy)
+                accy += src[yy + (int64_t)xx * fy + (int64_t)zz * fx * fy];
+            accx += accy / (hi[1] - lo[1] + 1);
+        }
+        accz += accx / (hi[0] - lo[0] + 1);
+    }
+    dst[y + (int64_t)x * cy + (int64_t)z * cx * cy] = accz / (hi[2] - lo[2] + 1);
+}
+
+// cuRestrictDx (WaveletPotentials/cuMultigridFrac2D.cu:4-17).
+__global__ void restrict_h_kernel(float* __restrict__ dst, const float* __restrict__ src, int nc,
+                                  int nf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    dst[i] = src[2 * i] + (2 * i + 1 < nf ? src[2 * i + 1] : 0.f);
+}
+
+__global__ void fill_kernel(float* __restrict__ p, float v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Sums of squares of the residual and of f in FP64: per-block partial sums
+// (wavefront shuffles, then LDS), added into out[0], out[1] by one block each
+// in a second pass.
+constexpr int kNormBlocks = 1024;
+
+__global__ void norm_partial_kernel(double* __restrict__ part, const float* __restrict__ r,
+                                    const float* __restrict__ f, int64_t n) {
+    double sr = 0.0, sf = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = r[i], b = f[i];
+        sr += a * a;
+        sf += b * b;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        sf += __shfl_xor(sf, o);
+    }
+    __shared__ double lr[4], lf[4];
+    if ((threadIdx.x & 63) == 0) {
+        lr[threadIdx.x >> 6] = sr;
+        lf[threadIdx.x >> 6] = sf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = lr[0] + lr[1] + lr[2] + lr[3];
+        part[2 * blockIdx.x + 1] = lf[0] + lf[1] + lf[2] + lf[3];
+    }
+}
+
+__global__ void norm_final_kernel(double* __restrict__ out, const double* __restrict__ part,
+                                  int nparts) {
+    double sr = 0.0, sf = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        sr += part[2 * i];
+        sf += part[2 * i + 1];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        sf += __shfl_xor(sf, o);
+    }
+    __shared__ double lr[4], lf[4];
+    if ((threadIdx.x & 63) == 0) {
+        lr[threadIdx.x >> 6] = sr;
+        lf[threadIdx.x >> 6] = sf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = lr[0] + lr[1] + lr[2] + lr[3];
+        out[1] = lf[0] + lf[1] + lf[2] + lf[3];
+    }
+}
+
+}  // namespace
+}  // namespace whhd
+
+struct whhd_mgfrac3d_s {
+    whhd::Dim3 dim;
+    char bc;            // 'n' or 'z' (the reference maps 'd' to 'z')
+    float he[3];        // exterior spacing: dx for 'n'/'z', 0 for 'd'
+    std::vector<whhd::Level> lv;
+    float* r = nullptr;       // fine-level residual
+    double* part = nullptr;   // norm partial sums (2 * kNormBlocks) + the two totals
+    double* host = nullptr;   // pinned host copy of the totals
+    std::vector<void*> owned;
+};
+
+namespace whhd {
+namespace {
+
+LevelArgs args_of(const whhd_mgfrac3d_s* h, const Level& L) {
+    return LevelArgs{L.ax, L.ay, L.az, L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z,
+                     h->he[0], h->he[1], h->he[2]};
+}
+
+hipError_t launch_status(const char* name) {
+    set_last_kernel(name);
+    return hipGetLastError();
+}
+
+int jacobi(const whhd_mgfrac3d_s* h, const Level& L, float* dst, const float* src, const float* f,
+           float w, hipStream_t s) {
+    const LevelArgs a = args_of(h, L);
+    if (h->bc == 'z')
+        hipLaunchKernelGGL(jacobi_kernel<true>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, dst,
+                           src, f, a, w);
+    else
+        hipLaunchKernelGGL(jacobi_kernel<false>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
+                           dst, src, f, a, w);
+    hipError_t e = launch_status("mg_jacobi_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi kernel");
+}
+
+int residual(const whhd_mgfrac3d_s* h, const Level& L, float* r, const float* v, const float* f,
+             hipStream_t s) {
+    const LevelArgs a = args_of(h, L);
+    if (h->bc == 'z')
+        hipLaunchKernelGGL(residual_kernel<true>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
+                           r, v, f, a);
+    else
+        hipLaunchKernelGGL(residual_kernel<false>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
+                           r, v, f, a);
+    hipError_t e = launch_status("mg_residual_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg residual kernel");
+}
+
+int copy(float* dst, const float* src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+    hipError_t e = launch_status("mg_copy_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg copy kernel");
+}
+
+int fill(float* dst, float v, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, v, n);
+    hipError_t e = launch_status("mg_fill_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg fill kernel");
+}
+
+int64_t cells(const Dim3& d) { return (int64_t)d.x * d.y * d.z; }
+
+#define TRY(e) do { int s_ = (e); if (s_ != WHHD_OK) return s_; } while (0)
+
+// n Jacobi sweeps on level L (SolveFrac, cuWeightedJacobi3D.cu:1075-1105); the
+// result stays in L.v.
+int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t s) {
+    float* a = L.v;
+    float* b = L.t;
+    for (int k = 0; k < n; ++k) {
+        TRY(jacobi(h, L, b, a, L.f, w, s));
+        std::swap(a, b);
+    }
+    if (a != L.v) TRY(copy(L.v, a, cells(L.d), s));
+    return WHHD_OK;
+}
+
+// The reference's coarsest level runs 50 ('n') or 20 ('z') sweeps of the
+// volume-scaled system in one block (cuWeightedJacobi3D.cu:1473-1722); scaling
+// rows by the cell volume leaves the Jacobi iteration unchanged, so the same
+// sweeps run here with the level's own kernel.
+int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
+    return smooth(h, L, h->bc == 'z' ? 20 : 50, w, s);
+}
+
+int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s) {
+    const Level& L = h->lv[l];
+    if (l + 1 >= h->lv.size()) return coarsest(h, L, w, s);
+    const Level& C = h->lv[l + 1];
+    TRY(smooth(h, L, nu1, w, s));
+    TRY(residual(h, L, h->r, L.v, L.f, s));
+    hipLaunchKernelGGL(restrict_kernel, grid_of(C.d.x, C.d.y, C.d.z), dim3(64, 4), 0, s, C.f, h->r,
+                       L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y, C.d.z);
+    hipError_t e = launch_status("mg_restrict_kernel");
+    if (e != hipSuccess) return hip_fail(e, "mg restrict kernel");
+    TRY(fill(C.v, 0.f, cells(C.d), s));
+    TRY(vcycle(h, l + 1, nu1, nu2, w, s));
+    hipLaunchKernelGGL(prolong_add_kernel, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, L.v,
+                       C.v, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y);
+    e = launch_status("mg_prolong_add_kernel");
+    if (e != hipSuccess) return hip_fail(e, "mg prolong kernel");
+    return smooth(h, L, nu2, w, s);
+}
+
+void release(whhd_mgfrac3d_s* h) {
+    if (!h) return;
+    for (void* p : h->owned) (void)hipFree(p);
+    if (h->host) (void)hipHostFree(h->host);
+    delete h;
+}
+
+}  // namespace
+
+int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, int ny, int nz,
+                    double dx, double dy, double dz, char bc, whhd_mgfrac3d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "mgfrac3d_create: null handle pointer");
+    *out = nullptr;
+    if (!ax || !ay || !az) return fail(WHHD_INVALID_ARG, "mgfrac3d_create: null fraction array");
+    if (nx < 2 || ny < 2 || nz < 2 || nz > 65535 || (int64_t)(nx + 1) * (ny + 1) * (nz + 1) > (1LL << 31))
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_create: grid extents out of range");
+    if (bc != 'n' && bc != 'd' && bc != 'z')
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_create: bc must be 'n', 'd' or 'z'");
+    if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "mgfrac3d_create: spacing <= 0");
+    auto* h = new whhd_mgfrac3d_s;
+    h->dim = {nx, ny, nz};
+    // cuMultigridFrac3D.cu:12-13: 'd' runs as 'z' with a wall on the face
+    h->bc = bc == 'n' ? 'n' : 'z';
+    h->he[0] = bc == 'd' ? 0.f : (float)dx;
+    h->he[1] = bc == 'd' ? 0.f : (float)dy;
+    h->he[2] = bc == 'd' ? 0.f : (float)dz;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        h->owned.push_back(p);
+        return p;
+    };
+    hipStream_t s = nullptr;
+    Dim3 d = {nx, ny, nz}, last = d;
+    int min_dim = std::min(nx, std::min(ny, nz));
+    int st = WHHD_OK;
+    while (min_dim > 1 && st == WHHD_OK) {
+        Level L;
+        L.d = d;
+        const size_t n = (size_t)cells(d);
+        L.f = (float*)alloc(n * 4);
+        L.v = (float*)alloc(n * 4);
+        L.t = (float*)alloc(n * 4);
+        L.ax = (float*)alloc((size_t)(d.x + 1) * d.y * d.z * 4);
+        L.ay = (float*)alloc((size_t)d.x * (d.y + 1) * d.z * 4);
+        L.az = (float*)alloc((size_t)d.x * d.y * (d.z + 1) * 4);
+        L.hx = (float*)alloc((size_t)d.x * 4);
+        L.hy = (float*)alloc((size_t)d.y * 4);
+        L.hz = (float*)alloc((size_t)d.z * 4);
+        if (!L.f || !L.v || !L.t || !L.ax || !L.ay || !L.az || !L.hx || !L.hy || !L.hz) {
+            st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+            break;
+        }
+        if (h->lv.empty()) {
+            hipError_t e = hipMemcpyAsync(L.ax, ax, (size_t)(d.x + 1) * d.y * d.z * 4,
+                                          hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(L.ay, ay, (size_t)d.x * (d.y + 1) * d.z * 4,
+                                   hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(L.az, az, (size_t)d.x * d.y * (d.z + 1) * 4,
+                                   hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "mgfrac3d_create: fraction copy");
+                break;
+            }
+            st = fill(L.hx, (float)dx, d.x, s);
+            if (st == WHHD_OK) st = fill(L.hy, (float)dy, d.y, s);
+            if (st == WHHD_OK) st = fill(L.hz, (float)dz, d.z, s);
+        } else {
+            const Level& P = h->lv.back();
+            hipLaunchKernelGGL(restrict_faces_kernel, grid_of(d.x + 1, d.y, d.z), dim3(64, 4), 0, s,
+                               L.ax, P.ax, last.x + 1, last.y, last.z, d.x + 1, d.y, d.z, 0);
+            hipLaunchKernelGGL(restrict_faces_kernel, grid_of(d.x, d.y + 1, d.z), dim3(64, 4), 0, s,
+                               L.ay, P.ay, last.x, last.y + 1, last.z, d.x, d.y + 1, d.z, 1);
+            hipLaunchKernelGGL(restrict_faces_kernel, grid_of(d.x, d.y, d.z + 1), dim3(64, 4), 0, s,
+                               L.az, P.az, last.x, last.y, last.z + 1, d.x, d.y, d.z + 1, 2);
+            hipLaunchKernelGGL(restrict_h_kernel, dim3((d.x + 255) / 256), dim3(256), 0, s, L.hx,
+                               P.hx, d.x, last.x);
+            hipLaunchKernelGGL(restrict_h_kernel, dim3((d.y + 255) / 256), dim3(256), 0, s, L.hy,
+                               P.hy, d.y, last.y);
+            hipLaunchKernelGGL(restrict_h_kernel, dim3((d.z + 255) / 256), dim3(256), 0, s, L.hz,
+                               P.hz, d.z, last.z);
+            hipError_t e = launch_status("mg_setup_kernels");
+            if (e != hipSuccess) st = hip_fail(e, "mgfrac3d_create: hierarchy kernels");
+        }
+        h->lv.push_back(L);
+        last = d;
+        d = {half_up(d.x), half_up(d.y), half_up(d.z)};
+        min_dim = std::min(d.x, std::min(d.y, d.z));
+    }
+    if (st == WHHD_OK) {
+        h->r = (float*)alloc((size_t)cells(h->dim) * 4);
+        h->part = (double*)alloc((2 * kNormBlocks + 2) * sizeof(double));
+        if (!h->r || !h->part) st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+    }
+    if (st == WHHD_OK && hipHostMalloc((void**)&h->host, 2 * sizeof(double)) != hipSuccess) {
+        h->host = nullptr;
+        st = fail(WHHD_OOM, "mgfrac3d_create: pinned allocation failed");
+    }
+    if (st == WHHD_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(e, "mgfrac3d_create: setup");
+    }
+    if (st != WHHD_OK) {
+        release(h);
+        return st;
+    }
+    *out = h;
+    return WHHD_OK;
+}
+
+int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const float* f,
+                   int n_iters, int nu1, int nu2, float w, hipStream_t s) {
+    if (!h || !dst || !f) return fail(WHHD_INVALID_ARG, "mgfrac3d_solve: null argument");
+    if (n_iters < 0 || nu1 < 0 || nu2 < 0)
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_solve: negative iteration count");
+    Level& L0 = h->lv[0];
+    const int64_t n = cells(h->dim);
+    // Solve (cuMultigridFrac3D.cu:249-265): the iterate starts from pre_v or zero
+    if (pre_v) TRY(copy(L0.v, pre_v, n, s));
+    else TRY(fill(L0.v, 0.f, n, s));
+    TRY(copy(L0.f, f, n, s));
+    for (int i = 0; i < n_iters; ++i) TRY(vcycle(h, 0, nu1, nu2, w, s));
+    return copy(dst, L0.v, n, s);
+}
+
+int mgfrac3d_residual(whhd_mgfrac3d_s* h, const float* v, const float* f, double* rel,
+                      double* sums, hipStream_t s) {
+    if (!h || !v || !f) return fail(WHHD_INVALID_ARG, "mgfrac3d_residual: null argument");
+    const Level& L0 = h->lv[0];
+    const int64_t n = cells(h->dim);
+    TRY(residual(h, L0, h->r, v, f, s));
+    hipLaunchKernelGGL(norm_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, h->r, f, n);
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, h->part + 2 * kNormBlocks,
+                       h->part, kNormBlocks);
+    hipError_t e = launch_status("mg_norm_kernels");
+    if (e != hipSuccess) return hip_fail(e, "mg norm kernels");
+    e = hipMemcpyAsync(h->host, h->part + 2 * kNormBlocks, 2 * sizeof(double),
+                       hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "mgfrac3d_residual: read-back");
+    if (sums) {
+        sums[0] = h->host[0];
+        sums[1] = h->host[1];
+    }
+    if (rel) *rel = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
+    return WHHD_OK;
+}
+
+}  // namespace whhd
+
+extern "C" {
+
+WHHD_API int whhd_mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx,
+                                  int ny, int nz, double dx, double dy, double dz, char bc,
+                                  whhd_mgfrac3d_t* out) {
+    return whhd::mgfrac3d_create(ax, ay, az, nx, ny, nz, dx, dy, dz, bc, out);
+}
+
+WHHD_API int whhd_mgfrac3d_levels(whhd_mgfrac3d_t h, int* dims, int max_levels) {
+    if (!h) return -1;
+    const int n = (int)h->lv.size();
+    for (int l = 0; l < n && l < max_levels && dims; ++l) {
+        dims[3 * l] = h->lv[l].d.x;
+        dims[3 * l + 1] = h->lv[l].d.y;
+        dims[3 * l + 2] = h->lv[l].d.z;
+    }
+    return n;
+}
+
+WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre_v, const float* f,
+                                 int n_iters, int nu1, int nu2, float weight, void* stream) {
+    return whhd::mgfrac3d_solve(h, dst, pre_v, f, n_iters, nu1, nu2, weight, (hipStream_t)stream);
+}
+
+WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f,
+                                    double* rel, void* stream) {
+    return whhd::mgfrac3d_residual(h, v, f, rel, nullptr, (hipStream_t)stream);
+}
+
+WHHD_API void whhd_mgfrac3d_destroy(whhd_mgfrac3d_t h) { whhd::release(h); }
+
+}  // extern "C"

@@ -291,6 +291,7 @@ int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, fl
         hipLaunchKernelGGL((project3d_kernel<int, ZB>), grid, block, 0, st, a);
     else
         hipLaunchKernelGGL((project3d_kernel<int64_t, ZB>), grid, block, 0, st, a);
+    set_last_kernel("project3d_kernel");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "project3d kernel");
 }
@@ -340,6 +341,7 @@ int project2d_run(const whhd_project2d_s* h, float* qz, const float* xv, const f
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, 1);
     hipLaunchKernelGGL(project2d_kernel, grid, block, 0, st, a);
+    set_last_kernel("project2d_kernel");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "project2d kernel");
 }

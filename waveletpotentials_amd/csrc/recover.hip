@@ -11,6 +11,8 @@
 // synchronizing, so a solve can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include "whhd_internal.h"
 
 struct whhd_project3d_s;
@@ -35,6 +37,7 @@ struct whhd_recover3d_s {
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
     float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
+    whhd::StageTrace trace;
 };
 
 struct whhd_recover2d_s {
@@ -85,39 +88,52 @@ struct Pass {
     char bc;
     int wt;
 };
+// Stage labels for the trace: "<fwd|inv> <array> <axis>".
+static void mark(StageTrace* tr, hipStream_t s, bool fwd, const char* arr, char axis) {
+    if (!tr || tr->cur < 0) return;
+    char lab[32];
+    std::snprintf(lab, sizeof(lab), "%s %s %c", fwd ? "fwd" : "inv", arr, axis);
+    tr->mark(s, lab);
+}
 static int three_pass(const float* in, float* scratch, float* out, int nx, int ny, int nz,
-                      const int lv[3], Pass px, Pass py, Pass pz, bool fwd, hipStream_t s) {
+                      const int lv[3], Pass px, Pass py, Pass pz, bool fwd, hipStream_t s,
+                      StageTrace* tr, const char* arr) {
     TRY(transform_lines<float>(out, in, nx, ny, nz, lv[0], 'x', px.bc, px.wt, fwd, s));
+    mark(tr, s, fwd, arr, 'x');
     TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s));
+    mark(tr, s, fwd, arr, 'y');
     TRY(transform_lines<float>(out, scratch, nx, ny, nz, lv[2], 'z', pz.bc, pz.wt, fwd, s));
+    mark(tr, s, fwd, arr, 'z');
     return WHHD_OK;
 }
 
 // Velocity component c (0: xv, 1: yv, 2: zv): its own axis uses the divergence
 // wavelet, the others the curl wavelet.
-static int vel_pass(const whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
+static int vel_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
                     bool fwd, hipStream_t s) {
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
+    StageTrace* tr = &h->trace;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s);
-    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s);
-    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s);
+    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s, tr, "xv");
+    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s, tr, "yv");
+    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s, tr, "zv");
     }
 }
 
 // Potential component c (0: qx, 1: qy, 2: qz): its own axis uses the curl
 // wavelet, the others the divergence wavelet.
-static int pot_pass(const whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
+static int pot_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
                     bool fwd, hipStream_t s) {
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
+    StageTrace* tr = &h->trace;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s);
-    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s);
-    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s);
+    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s, tr, "qx");
+    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s, tr, "qy");
+    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s, tr, "qz");
     }
 }
 
@@ -174,14 +190,17 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
     const float* v[3] = {xv, yv, zv};
     float* q[3] = {qx, qy, qz};
     float* vo[3] = {xo, yo, zo};
+    h->trace.solve_start(s);
     for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, v[c], S[c], t[c], true, s));
     // q coefficients land in the scratch; with v_out, s x q overwrites the
     // transformed velocity in place at the projected coefficients
     if (vout) TRY(project3d_run(h->proj, S[0], S[1], S[2], t[0], t[1], t[2], t[0], t[1], t[2], s));
     else TRY(project3d_run(h->proj, S[0], S[1], S[2], nullptr, nullptr, nullptr, t[0], t[1], t[2], s));
+    h->trace.mark(s, "project");
     for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], q[c], false, s));
     if (vout)
         for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, t[c], S[c], vo[c], false, s));
+    h->trace.solve_end();
     return WHHD_OK;
 }
 
@@ -192,6 +211,7 @@ int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const fl
     float* const* S = h->sq;
     const float* q[3] = {qx, qy, qz};
     float* o[3] = {ox, oy, oz};
+    h->trace.solve_start(s);
     for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, q[c], o[c], S[c], true, s));
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, h->nz + 1);
@@ -199,7 +219,10 @@ int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const fl
                        h->ly, h->lz, level);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "cut kernel");
+    set_last_kernel("cut_kernel");
+    h->trace.mark(s, "cut");
     for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], o[c], false, s));
+    h->trace.solve_end();
     return WHHD_OK;
 }
 
@@ -279,6 +302,22 @@ WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out
                                 const float* qx, const float* qy, const float* qz, int level,
                                 void* stream) {
     return whhd::recover3d_cut(h, qx_out, qy_out, qz_out, qx, qy, qz, level, (hipStream_t)stream);
+}
+
+WHHD_API int whhd_recover3d_trace_begin(whhd_recover3d_t h, int max_solves) {
+    if (!h) return whhd::fail(WHHD_INVALID_ARG, "trace_begin: null handle");
+    return h->trace.begin(max_solves);
+}
+
+WHHD_API int whhd_recover3d_trace_end(whhd_recover3d_t h, float* stage_us, int max_stages,
+                                      int* n_stages, int* n_solves) {
+    if (!h) return whhd::fail(WHHD_INVALID_ARG, "trace_end: null handle");
+    return h->trace.end(stage_us, max_stages, n_stages, n_solves);
+}
+
+WHHD_API const char* whhd_recover3d_trace_label(whhd_recover3d_t h, int stage) {
+    if (!h || stage < 0 || stage >= (int)h->trace.labels.size()) return nullptr;
+    return h->trace.labels[stage].c_str();
 }
 
 WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h) {

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/whhd.h"
 
@@ -66,5 +67,31 @@ int gram_table_device(double* out, int wt, char bc, int n, hipStream_t stream);
 
 char proj_bc(char bc);
 int proj_wavelet(int wt);
+const char* wavelet_name(int wt);
+
+// Name of the kernel the calling thread launched last (set by the dispatchers
+// of the line transforms and the projection).
+const char* last_kernel();
+void set_last_kernel(const char* name);
+
+// Per-stage timing of a driver's solves: HIP events recorded on the solve's
+// stream after each stage, read back by trace_end.  Recording costs one event
+// record per launch and no synchronization.
+struct StageTrace {
+    static constexpr int kMaxStages = 64;
+    int cap = 0;      // solves that can be recorded (0: tracing off)
+    int solves = 0;   // solves recorded so far
+    int stages = 0;   // stages per solve, fixed by the first recorded solve
+    int cur = -1;     // stage index within the solve being recorded, -1: none
+    std::vector<hipEvent_t> ev;
+    std::vector<std::string> labels;
+    int begin(int max_solves);
+    void solve_start(hipStream_t s);
+    void mark(hipStream_t s, const char* stage);
+    void solve_end();
+    int end(float* stage_us, int max_stages, int* n_stages, int* n_solves);
+    void release();
+    ~StageTrace() { release(); }
+};
 
 }  // namespace whhd
