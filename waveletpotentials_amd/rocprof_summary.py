@@ -1,0 +1,119 @@
+"""Summarize rocprofv3 output of bench.py into the text files under profiles/.
+
+Inputs are the CSV directories of three rocprofv3 runs of the same bench
+command (``--kernel-trace --stats``, ``--pmc FETCH_SIZE``, ``--pmc WRITE_SIZE``).
+Byte counters are scaled by what they read for the calibration kernel that
+bench.py runs on a known size (an elementwise multiply reading and writing
+16 Mi floats, 67.1 MB each way), so the figures are HBM bytes per launch.
+
+    python -m waveletpotentials_amd.rocprof_summary TRACE_DIR FETCH_DIR WRITE_DIR OUT_PREFIX \\
+        --command "..."
+
+writes OUT_PREFIX.md (per-kernel durations and bytes) and OUT_PREFIX.json
+(per-kernel fetch/write bytes, read by bench.py for ``roofline.traffic``).
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+import statistics
+
+CAL_BYTES = 16 * 1024 * 1024 * 4  # calibration kernel: bytes read and bytes written
+CAL_NAME = "vectorized_elementwise_kernel"
+
+_WT = {"Cdf37": "cdf3.7 n", "Cdf46": "cdf4.6 d"}
+
+
+def kernel_label(name):
+    """The label the library's stage trace uses for a kernel symbol."""
+    m = re.search(r"lp_kernel.*Up(Cdf\d\d)E.*ELi(\d)ELi(\d)ELb([01])ELb([01])E", name)
+    if m:
+        return (f"lp_kernel<{_WT.get(m.group(1), m.group(1))},L={m.group(2)},"
+                f"{'fwd' if m.group(4) == '1' else 'inv'},"
+                f"{'contiguous' if m.group(5) == '1' else 'strided'}>")
+    m = re.match(r"(?:void )?whhd::(\w+)", name)
+    if m:
+        return m.group(1)
+    return name.split("(")[0][:80]
+
+
+def _one(pattern):
+    hits = glob.glob(pattern)
+    if not hits:
+        raise FileNotFoundError(pattern)
+    return hits[0]
+
+
+def counters(d):
+    out = collections.defaultdict(list)
+    with open(_one(os.path.join(d, "*counter_collection.csv"))) as f:
+        for r in csv.DictReader(f):
+            out[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)  # KiB -> bytes
+    return out
+
+
+def summarize(trace_dir, fetch_dir, write_dir):
+    stats = {}
+    with open(_one(os.path.join(trace_dir, "*kernel_stats.csv"))) as f:
+        for r in csv.DictReader(f):
+            stats[r["Name"]] = r
+    fetch, write = counters(fetch_dir), counters(write_dir)
+    cal = [k for k in fetch if CAL_NAME in k]
+    if not cal:
+        raise RuntimeError("calibration kernel not found in the counter runs")
+    f_scale = CAL_BYTES / statistics.median(fetch[cal[0]])
+    w_scale = CAL_BYTES / statistics.median(write[cal[0]])
+    kernels = {}
+    for name, r in stats.items():
+        label = kernel_label(name)
+        row = kernels.setdefault(label, {"calls": 0, "total_ns": 0.0})
+        row["calls"] += int(r["Calls"])
+        row["total_ns"] += float(r["TotalDurationNs"])
+        if name in fetch:
+            row["fetch_bytes"] = statistics.median(fetch[name]) * f_scale
+        if name in write:
+            row["write_bytes"] = statistics.median(write[name]) * w_scale
+    for row in kernels.values():
+        row["avg_us"] = row["total_ns"] / row["calls"] / 1000.0
+    return kernels, f_scale, w_scale
+
+
+def write_outputs(kernels, f_scale, w_scale, prefix, command):
+    lib = {k: v for k, v in kernels.items()
+           if k.startswith(("lp_kernel", "project3d", "line_tile", "mg_", "frac", "levelset",
+                            "eikonal", "cut_kernel"))}
+    with open(prefix + ".json", "w") as f:
+        json.dump({"command": command, "fetch_scale": f_scale, "write_scale": w_scale,
+                   "kernels": {k: {"avg_us": round(v["avg_us"], 3), "calls": v["calls"],
+                                   "fetch_bytes": round(v.get("fetch_bytes", 0.0)),
+                                   "write_bytes": round(v.get("write_bytes", 0.0))}
+                               for k, v in sorted(lib.items())}}, f, indent=1)
+    lines = [f"# rocprofv3 summary: {os.path.basename(prefix)}", "",
+             "Command (three runs; byte counters scaled by the calibration kernel, "
+             f"FETCH_SIZE x{f_scale:.3f}, WRITE_SIZE x{w_scale:.3f}):", "", "```", command, "```",
+             "", "| kernel | calls | avg us | HBM fetch MB/launch | HBM write MB/launch |",
+             "|---|---:|---:|---:|---:|"]
+    for k, v in sorted(lib.items(), key=lambda kv: -kv[1]["total_ns"]):
+        lines.append(f"| `{k}` | {v['calls']} | {v['avg_us']:.2f} | "
+                     f"{v.get('fetch_bytes', 0) / 1e6:.1f} | {v.get('write_bytes', 0) / 1e6:.1f} |")
+    with open(prefix + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("trace_dir")
+    p.add_argument("fetch_dir")
+    p.add_argument("write_dir")
+    p.add_argument("out_prefix")
+    p.add_argument("--command", required=True)
+    a = p.parse_args()
+    kernels, fs, ws = summarize(a.trace_dir, a.fetch_dir, a.write_dir)
+    write_outputs(kernels, fs, ws, a.out_prefix, a.command)
+
+
+if __name__ == "__main__":
+    main()
