@@ -214,6 +214,44 @@ WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const flo
                                     void* stream);
 WHHD_API void whhd_mgfrac3d_destroy(whhd_mgfrac3d_t h);
 
+/* ---- field operators and reductions ------------------------------------ */
+/* CuGradient::Gradient3D_Frac_P(xv, yv, zv, p, ax, ay, az, dim, dx, dy, dz, dx_e, bc)
+ * (WaveletPotentials/cuGradient.cu:1841-1951, 2514-2545) for uniform spacing:
+ * face gradients of the cell field p, zero on faces with fraction <= 1e-12.
+ * bc 'n': wall faces 0; 'd': zero wall value on the face (distance h/2); 'z':
+ * zero value one cell outside (distance h).  The far-wall face tests its own
+ * fraction (the reference's spacing-array 'z' kernel tests the neighbouring
+ * face's, cuGradient.cu:1939-1950). */
+WHHD_API int whhd_gradient3d_frac(float* xv, float* yv, float* zv, const float* p, const float* ax,
+                                  const float* ay, const float* az, int nx, int ny, int nz,
+                                  double dx, double dy, double dz, char bc, void* stream);
+/* u <- a * (u - grad p) on every face in one pass: the gradient above followed
+ * by CudaMatAdd(u, grad, -1) and CudaMatMul(u, u, a)
+ * (WaveletPotentials/TestLocalProjection.cpp:690-701).  No reference counterpart
+ * as a single call. */
+WHHD_API int whhd_subtract_gradient_frac(float* xv, float* yv, float* zv, const float* p,
+                                         const float* ax, const float* ay, const float* az, int nx,
+                                         int ny, int nz, double dx, double dy, double dz, char bc,
+                                         void* stream);
+/* CuConvergence::GetB_3d_frac_vel(b, xv, yv, zv, ax, ay, az, dim, dx, bc)
+ * (WaveletPotentials/cuConvergence.cu:40-124, 435-447): b = -div(a u) per cell,
+ * over all six faces (both of the reference's kernels compute this value). */
+WHHD_API int whhd_divergence3d_frac(float* b, const float* xv, const float* yv, const float* zv,
+                                    const float* ax, const float* ay, const float* az, int nx,
+                                    int ny, int nz, double dx, double dy, double dz, void* stream);
+/* CudaMatAdd(dst, src, n, scale): dst += scale * src; CudaMatMul(dst, l, r, n):
+ * dst = l * r (WaveletPotentials/cudaMath.cu:110-168). */
+WHHD_API int whhd_axpy(float* dst, const float* src, long long n, float scale, void* stream);
+WHHD_API int whhd_mul(float* dst, const float* l, const float* r, long long n, void* stream);
+/* Reductions of CudaArraySum / CudaArrayNormSum / CudaArrayAbsSum / CudaFindMaxValue
+ * (WaveletPotentials/cudaMath.cu:35-108, 1230-1280), accumulated in FP64 on the
+ * device: `scratch` is caller-owned device memory of whhd_reduce_scratch_doubles()
+ * doubles; the result lands in the host double *out.  Synchronizes `stream`. */
+enum { WHHD_REDUCE_SUM = 0, WHHD_REDUCE_SUM_SQ = 1, WHHD_REDUCE_SUM_ABS = 2, WHHD_REDUCE_MAX_ABS = 3 };
+WHHD_API int whhd_reduce(const float* x, long long n, int op, double* scratch, double* out,
+                         void* stream);
+WHHD_API int whhd_reduce_scratch_doubles(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,8 @@ def _setup(L):
     L.orc_wj_frac3d.argtypes = [_D] * 6 + [i, i, i, d, d, d, c, d, i]
     L.orc_mgfrac3d_rel_residual.argtypes = [_D] * 5 + [i, i, i, d, d, d, c]
     L.orc_mgfrac3d_rel_residual.restype = d
+    L.orc_gradient3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d, c]
+    L.orc_divergence3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d]
     L.orc_curl3d.restype = None
     L.orc_curl2d.restype = None
     L.orc_div3d.restype = None
@@ -289,3 +291,25 @@ def mg_rel_residual(v, f, ax, ay, az, dx, bc="n"):
     ax, ay, az = _fr(ax, ay, az)
     return lib().orc_mgfrac3d_rel_residual(_p(v), _p(f), _p(ax), _p(ay), _p(az), nx, ny, nz,
                                            *map(float, dx), _b(bc))
+
+
+# ---- field operators --------------------------------------------------------
+def gradient3d_frac(p, ax, ay, az, dx, bc="n"):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    nz, nx, ny = p.shape
+    ax, ay, az = _fr(ax, ay, az)
+    xv, yv, zv = field(nx + 1, ny, nz), field(nx, ny + 1, nz), field(nx, ny, nz + 1)
+    _check(lib().orc_gradient3d_frac(_p(xv), _p(yv), _p(zv), _p(p), _p(ax), _p(ay), _p(az), nx, ny,
+                                     nz, *map(float, dx), _b(bc)), "gradient3d_frac")
+    return xv, yv, zv
+
+
+def divergence3d_frac(xv, yv, zv, ax, ay, az, dx):
+    xv, yv, zv = _fr(xv, yv, zv)
+    ax, ay, az = _fr(ax, ay, az)
+    nz, nx1, ny = xv.shape
+    nx = nx1 - 1
+    b = field(nx, ny, nz)
+    _check(lib().orc_divergence3d_frac(_p(b), _p(xv), _p(yv), _p(zv), _p(ax), _p(ay), _p(az), nx,
+                                       ny, nz, *map(float, dx)), "divergence3d_frac")
+    return b

@@ -115,6 +115,15 @@ int orc_wj_frac3d(double* dst, const double* src, const double* f, const double*
 double orc_mgfrac3d_rel_residual(const double* v, const double* f, const double* ax,
                                  const double* ay, const double* az, int nx, int ny, int nz,
                                  double dx, double dy, double dz, char bc);
+/* ---- field operators (fields_oracle.c) ---- */
+/* Gradient3D_Frac_P for uniform spacing: face gradients xv (nx+1,ny,nz), yv, zv of p. */
+int orc_gradient3d_frac(double* xv, double* yv, double* zv, const double* p, const double* ax,
+                        const double* ay, const double* az, int nx, int ny, int nz, double dx,
+                        double dy, double dz, char bc);
+/* GetB_3d_frac_vel: b = -div(a u) per cell. */
+int orc_divergence3d_frac(double* b, const double* xv, const double* yv, const double* zv,
+                          const double* ax, const double* ay, const double* az, int nx, int ny,
+                          int nz, double dx, double dy, double dz);
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,13 @@ SIGNATURES = {
     "whhd_mgfrac3d_solve": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.c_float, _P]),
     "whhd_mgfrac3d_residual": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "whhd_mgfrac3d_destroy": (None, [_P]),
+    "whhd_gradient3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
+    "whhd_subtract_gradient_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
+    "whhd_divergence3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _P]),
+    "whhd_axpy": (_I, [_P, _P, ctypes.c_longlong, ctypes.c_float, _P]),
+    "whhd_mul": (_I, [_P, _P, _P, ctypes.c_longlong, _P]),
+    "whhd_reduce": (_I, [_P, ctypes.c_longlong, _I, _P, ctypes.POINTER(_D), _P]),
+    "whhd_reduce_scratch_doubles": (_I, []),
 }
 
 _lib = None

@@ -26,7 +26,9 @@ import torch
 from ._lib import WhhdError, check, lib
 
 __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "string_from_type",
-           "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
+           "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuGradient",
+           "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
+           "CudaArrayAbsSum", "CudaFindMaxValue", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -424,3 +426,104 @@ class CuMultigridFrac3D(_Handle):
                                            _ptr(f, numel=self._n, name="f"), ctypes.byref(rel),
                                            _stream()), "GetResdualError")
         return rel.value
+
+
+def _face_ptrs(dim, xv, yv, zv, names=("xv", "yv", "zv")):
+    nx, ny, nz = dim
+    return (_ptr(xv, numel=(nx + 1) * ny * nz, name=names[0]),
+            _ptr(yv, numel=nx * (ny + 1) * nz, name=names[1]),
+            _ptr(zv, numel=nx * ny * (nz + 1), name=names[2]))
+
+
+class CuGradient:
+    """CuGradient (WaveletPotentials/cuGradient.cuh): face gradients with fractions."""
+    _instance = None
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def Gradient3D_Frac_P(self, xv, yv, zv, p, ax, ay, az, dim, dx, bc="n"):
+        """Gradient3D_Frac_P (WaveletPotentials/cuGradient.cu:2514-2545), uniform spacing."""
+        dim = tuple(int(v) for v in dim)
+        nx, ny, nz = dim
+        check(lib().whhd_gradient3d_frac(*_face_ptrs(dim, xv, yv, zv),
+                                         _ptr(p, numel=nx * ny * nz, name="p"),
+                                         *_face_ptrs(dim, ax, ay, az, ("ax", "ay", "az")), *dim,
+                                         *(float(v) for v in dx), _b(bc), _stream()),
+              "Gradient3D_Frac_P")
+
+    def SubtractGradient3D_Frac_P(self, xv, yv, zv, p, ax, ay, az, dim, dx, bc="n"):
+        """u <- a * (u - grad p) in one pass."""
+        dim = tuple(int(v) for v in dim)
+        nx, ny, nz = dim
+        check(lib().whhd_subtract_gradient_frac(*_face_ptrs(dim, xv, yv, zv),
+                                                _ptr(p, numel=nx * ny * nz, name="p"),
+                                                *_face_ptrs(dim, ax, ay, az, ("ax", "ay", "az")),
+                                                *dim, *(float(v) for v in dx), _b(bc), _stream()),
+              "SubtractGradient3D_Frac_P")
+
+
+class CuConvergence:
+    """CuConvergence (WaveletPotentials/cuConvergence.cuh): divergence of a*u."""
+    _instance = None
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._instance is None:
+            cls._instance = cls()
+        return cls._instance
+
+    def GetB_3d_frac_vel(self, b, xv, yv, zv, ax, ay, az, dim, dx, bc="n"):
+        """GetB_3d_frac_vel (WaveletPotentials/cuConvergence.cu:435-447): b = -div(a u)."""
+        dim = tuple(int(v) for v in dim)
+        nx, ny, nz = dim
+        check(lib().whhd_divergence3d_frac(_ptr(b, numel=nx * ny * nz, name="b"),
+                                           *_face_ptrs(dim, xv, yv, zv),
+                                           *_face_ptrs(dim, ax, ay, az, ("ax", "ay", "az")), *dim,
+                                           *(float(v) for v in dx), _stream()),
+              "GetB_3d_frac_vel")
+
+
+def CudaMatAdd(dst, src, size, scale):
+    """dst += src * scale (WaveletPotentials/cudaMath.cu:110-117)."""
+    check(lib().whhd_axpy(_ptr(dst, name="dst"), _ptr(src, name="src"), int(size), float(scale),
+                          _stream()), "CudaMatAdd")
+
+
+def CudaMatMul(dst, lsrc, rsrc, size):
+    """dst = lsrc * rsrc (WaveletPotentials/cudaMath.cu:161-168)."""
+    check(lib().whhd_mul(_ptr(dst, name="dst"), _ptr(lsrc, name="lsrc"), _ptr(rsrc, name="rsrc"),
+                         int(size), _stream()), "CudaMatMul")
+
+
+def _reduce(src, size, op):
+    scratch = torch.empty(lib().whhd_reduce_scratch_doubles(), dtype=torch.float64,
+                          device=src.device)
+    out = ctypes.c_double(0.0)
+    check(lib().whhd_reduce(_ptr(src, name="src"), int(size), op,
+                            ctypes.c_void_p(scratch.data_ptr()), ctypes.byref(out), _stream()),
+          "reduce")
+    return out.value
+
+
+def CudaArraySum(src, size):
+    """Sum of the first `size` elements, FP64 accumulation (cudaMath.cu:1230-1237)."""
+    return _reduce(src, size, 0)
+
+
+def CudaArrayNormSum(src, size):
+    """Sum of squares (cudaMath.cu:1249-1257)."""
+    return _reduce(src, size, 1)
+
+
+def CudaArrayAbsSum(src, size):
+    """Sum of absolute values (cudaMath.cu:1239-1247)."""
+    return _reduce(src, size, 2)
+
+
+def CudaFindMaxValue(src, size):
+    """max |x| over the array (cudaMath.cu:1269-1280; reduced on the device)."""
+    return _reduce(src, size, 3)

@@ -100,7 +100,10 @@ WHHD_DEV void static_for(F&& f) {
     }
 }
 
-constexpr int kBlock = 16;  // pairs written per block
+#ifndef WHHD_KBLOCK
+#define WHHD_KBLOCK 16
+#endif
+constexpr int kBlock = WHHD_KBLOCK;  // pairs written per block
 
 #ifndef WHHD_HOST_EMULATION
 // ---- I/O of one lane's line ----------------------------------------------
