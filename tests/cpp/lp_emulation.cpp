@@ -7,7 +7,9 @@
 //
 //   lp_emulate(dst, src, lines, n, stride, levels, wavelet, forward, look)
 //     wavelet 0: CDF3.7 'n' (n = 2^levels), 1: CDF4.6 'd' (n = 2^levels + 1);
-//     line j is src[j*stride + i], i < n; look = look-ahead (0, 16 or 32).
+//     line j is src[j*stride + i], i < n; look = look-ahead (0, 16 or 32),
+//     or -1 for the two-phase loads of the contiguous-line kernels (returns -2
+//     if their ranges are out of order, leave gaps or miss part of the line).
 #define WHHD_HOST_EMULATION 1
 #include "../../waveletpotentials_amd/csrc/line_parallel.h"
 
@@ -18,6 +20,8 @@ namespace {
 template <int LOOK>
 struct HostIO {
     static constexpr int kLook = LOOK;
+    static constexpr bool kTwoPhase = false;
+    struct Pend {};
     const float* src;
     float* dst;
     int n;
@@ -28,23 +32,73 @@ struct HostIO {
     }
 };
 
+// Two-phase loads as the contiguous-line kernels do them: fetch<A, B> holds
+// samples [A, B) until unpack<A, B>; ranges must come in order, without gaps,
+// start on multiples of 4 and end on multiples of 4 or at the line end.
+struct HostIO2 {
+    static constexpr int kLook = 0;
+    static constexpr bool kTwoPhase = true;
+    struct Pend {
+        float v[65];
+        int a = -1, b = -1;
+    };
+    const float* src;
+    float* dst;
+    int n;
+    int* fetched_to;   // end of the last fetched range
+    int* unpacked_to;  // end of the last unpacked range
+    int* bad;
+    template <int A, int B>
+    void fetch(Pend& p) const {
+        if constexpr (A < B) {
+            static_assert((A & 3) == 0 && B - A <= 64, "range");
+            if (A != *fetched_to || ((B & 3) && B != n)) ++*bad;
+            for (int i = A; i < B; ++i) p.v[i - A] = src[i];
+            p.a = A;
+            p.b = B;
+            *fetched_to = B;
+        }
+    }
+    template <int A, int B>
+    void unpack(const Pend& p, float* x) const {
+        if constexpr (A < B) {
+            if (p.a != A || p.b != B || A != *unpacked_to) ++*bad;
+            for (int i = A; i < B; ++i) x[i] = p.v[i - A];
+            *unpacked_to = B;
+        }
+    }
+    template <int START, int CNT>
+    void store_run(const float* vals) const {
+        for (int v = 0; v < CNT; ++v) dst[START + v] = vals[v];
+    }
+};
+
 template <class W, int LEV, int EXT, int LOOK>
-void run(float* dst, const float* src, int64_t lines, int64_t stride, bool fwd) {
+int run(float* dst, const float* src, int64_t lines, int64_t stride, bool fwd) {
     constexpr int n = (1 << LEV) + EXT;
     for (int64_t j = 0; j < lines; ++j) {
-        HostIO<LOOK> io{src + j * stride, dst + j * stride, n};
-        if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
-        else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
+        if constexpr (LOOK >= 0) {
+            HostIO<LOOK> io{src + j * stride, dst + j * stride, n};
+            if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
+            else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
+        } else {
+            int f = 0, u = 0, bad = 0;
+            HostIO2 io{src + j * stride, dst + j * stride, n, &f, &u, &bad};
+            if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
+            else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
+            if (bad || f != n || u != n) return -2;  // out of order, gaps, or not the whole line
+        }
     }
+    return 0;
 }
 
 template <class W, int EXT, int LOOK>
 int by_level(float* dst, const float* src, int64_t lines, int64_t stride, int levels, bool fwd) {
     switch (levels) {
-    case 5: run<W, 5, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
-    case 6: run<W, 6, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
-    case 7: run<W, 7, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
-    case 8: run<W, 8, EXT, LOOK>(dst, src, lines, stride, fwd); return 0;
+    case 5: return run<W, 5, EXT, LOOK>(dst, src, lines, stride, fwd);
+    case 6: return run<W, 6, EXT, LOOK>(dst, src, lines, stride, fwd);
+    case 7: return run<W, 7, EXT, LOOK>(dst, src, lines, stride, fwd);
+    case 8: return run<W, 8, EXT, LOOK>(dst, src, lines, stride, fwd);
     default: return -1;
     }
 }
@@ -67,6 +121,7 @@ extern "C" int lp_emulate(float* dst, const float* src, long long lines, int n, 
     case 0: return by_wavelet<0>(dst, src, lines, stride, levels, wt, fwd != 0);
     case 16: return by_wavelet<16>(dst, src, lines, stride, levels, wt, fwd != 0);
     case 32: return by_wavelet<32>(dst, src, lines, stride, levels, wt, fwd != 0);
+    case -1: return by_wavelet<-1>(dst, src, lines, stride, levels, wt, fwd != 0);
     default: return -1;
     }
 }

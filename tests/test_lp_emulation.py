@@ -1,7 +1,9 @@
 """CPU check of the line-parallel transform algorithm (the register-resident
 kernels of csrc/line_parallel.h, compiled for the host by
 tests/cpp/lp_emulation.cpp) against the FP64 oracle, for both wavelets of the
-recovery path, levels 5-8, both directions and every look-ahead depth.
+recovery path, levels 5-8, both directions, every look-ahead depth of the
+single-phase loads and the two-phase (fetch, then unpack one block later)
+loads of the contiguous-line kernels.
 
 The host build runs the same templates as the GPU kernels (level and block
 structure, reflections, streamed loads, run stores) one line at a time; the
@@ -44,7 +46,7 @@ def test_lp_algorithm_matches_oracle(emu, orc, wt, name, bc, ext, lev, fwd):
     rng = np.random.default_rng(1000 * lev + 10 * wt + fwd)
     src = rng.standard_normal((1, lines, n))
     ref, _ = orc.transform(src, lev, "y", bc, name, bool(fwd))
-    for look in (0, 16, 32):
+    for look in (0, 16, 32, -1):
         s32 = src.astype(np.float32)
         d32 = np.full_like(s32, np.nan)
         assert emu.lp_emulate(d32.ctypes.data, s32.ctypes.data, lines, n, n, lev, wt, fwd, look) == 0

@@ -118,6 +118,8 @@ constexpr uint32_t kOutOfRange = 0xF0000000u;
 
 struct StridedIO {
     static constexpr int kLook = WHHD_LOOK_S;  // samples loaded ahead of use
+    static constexpr bool kTwoPhase = false;   // loads land in the line registers directly
+    struct Pend {};
     __amdgpu_buffer_rsrc_t rs;
     __amdgpu_buffer_rsrc_t rd;
     uint32_t voff_ld;  // byte offset of the lane's line start (loads)
@@ -152,8 +154,10 @@ constexpr int kStageRun = 32;       // longest run written at once
 constexpr int kStageRow = 64 + 1;   // LDS row: one value of each of the 64 lines
 
 struct ContigIO {
-    static constexpr int kLook = WHHD_LOOK_C;  // samples loaded ahead of use
-    const float* line;    // the lane's own line (loads)
+    static constexpr int kLook = 0;            // (two-phase: one block ahead)
+    static constexpr bool kTwoPhase = true;
+    const float* line;    // the lane's own line (tail loads)
+    __amdgpu_buffer_rsrc_t rs;
     __amdgpu_buffer_rsrc_t rd;
     uint32_t voff_st;     // byte offset of the lane's line, or out of range for lanes that do not store
     uint32_t first_off;   // byte offset of the line of lane 0
@@ -161,17 +165,83 @@ struct ContigIO {
     int n;
     float* stage;         // this wavefront's LDS staging area, kStageRun rows
     int lane;
-    // Loads go through a plain pointer: LLVM narrows a raw buffer load of 16
-    // bytes whose leading elements are unused without moving its offset, so
-    // element extraction from __builtin_amdgcn_raw_buffer_load_b128 returns
-    // the wrong samples; ordinary vector loads are narrowed correctly.
-    WHHD_DEV float load(int i) const {
-        const int q = i & ~3;
-        if (q + 3 < n) {
-            const Quad r = *reinterpret_cast<const Quad*>(line + q);
-            return r.v[i & 3];
+    // Two-phase loads.  fetch<A, B> issues the loads of samples [A, B) of the
+    // wavefront's 64 lines as coalesced 16-byte pieces: in each instruction
+    // consecutive lanes read consecutive pieces of one line (8 lanes cover 128
+    // bytes of a line), instead of every lane reading 16 bytes of its own line,
+    // which touches 64 lines per instruction and streams poorly from HBM.
+    // unpack<A, B> later passes the pieces through LDS so that each lane ends
+    // up with its own line's samples; between the two the wavefront computes
+    // the previous block, which hides the load latency.  A is a multiple of 4;
+    // B is too, or the line length (a one-sample tail of an odd line, read by
+    // each lane for its own line).  All four elements of every 16-byte raw
+    // buffer load are used (LLVM narrows such a load whose leading elements
+    // are unused without moving its offset, so single samples are never
+    // extracted from one).
+    static constexpr int kMaxQ = 16;  // 16-byte pieces per lane in flight
+    struct Pend {
+        Quad q[kMaxQ];
+        float tail;
+    };
+    template <int NQ, int K>
+    static constexpr int chunk() { return NQ - K >= 8 ? 8 : NQ - K >= 4 ? 4 : NQ - K >= 2 ? 2 : 1; }
+    template <int Q0, int NQ, int K>
+    WHHD_DEV void fetch_chunks(Pend& p) const {
+        if constexpr (K < NQ) {
+            constexpr int c = chunk<NQ, K>();
+            // one per-lane offset per chunk shape; rounds differ by constants
+            // (scalar offsets).  Lines past the end read past the array and
+            // get zeros from the buffer range check.
+            const uint32_t voff = first_off + (uint32_t)(((lane / c) * n + 4 * (lane % c)) * 4);
+#pragma unroll
+            for (int r = 0; r < c; ++r) {
+                const U4 v = __builtin_amdgcn_raw_buffer_load_b128(
+                    rs, voff, (r * (64 / c) * n + 4 * (Q0 + K)) * 4, 0);
+                p.q[K + r] = __builtin_bit_cast(Quad, v);
+            }
+            fetch_chunks<Q0, NQ, K + c>(p);
         }
-        return line[i];
+    }
+    template <int A, int B>
+    WHHD_DEV void fetch(Pend& p) const {
+        if constexpr (A < B) {
+            static_assert((A & 3) == 0, "range start must be a multiple of 4");
+            constexpr int QA = A / 4, QB = (B & ~3) / 4, NQ = QB - QA;
+            static_assert(NQ <= kMaxQ, "range too long");
+            fetch_chunks<QA, NQ, 0>(p);
+            if constexpr ((B & 3) != 0) {
+                static_assert((B & 3) == 1, "only a one-sample tail");
+                p.tail = line[B - 1];
+            }
+            // keep the loads ahead of the block computed next
+            WHHD_SCHED_BARRIER();
+        }
+    }
+    template <int A, int NQ, int K>
+    WHHD_DEV void unpack_chunks(const Pend& p, float* x) const {
+        if constexpr (K < NQ) {
+            constexpr int c = chunk<NQ, K>();
+#pragma unroll
+            for (int r = 0; r < c; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    stage[(4 * (lane % c) + e) * kStageRow + lane / c + r * (64 / c)] = p.q[K + r].v[e];
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int s2 = 0; s2 < 4 * c; ++s2) x[A + 4 * K + s2] = stage[s2 * kStageRow + lane];
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+            unpack_chunks<A, NQ, K + c>(p, x);
+        }
+    }
+    template <int A, int B>
+    WHHD_DEV void unpack(const Pend& p, float* x) const {
+        if constexpr (A < B) {
+            constexpr int NQ = (B & ~3) / 4 - A / 4;
+            unpack_chunks<A, NQ, 0>(p, x);
+            if constexpr ((B & 3) != 0) x[B - 1] = p.tail;
+        }
     }
     // Samples [START, START+CNT) of every lane's line.  Whole 16-byte pieces
     // are transposed through LDS so that consecutive lanes store consecutive
@@ -253,10 +323,18 @@ WHHD_DEV void fwd_level(const IO& io, float* in, float* out) {
     constexpr int Wd = L / 2 + EXT;
     constexpr int NP = kBlock + 2 * H;
     constexpr int NB = (Wd + kBlock - 1) / kBlock;
+    [[maybe_unused]] typename IO::Pend pend;
+    if constexpr (TOP && IO::kTwoPhase) io.template fetch<0, fwd_front<H, N, 0>(0)>(pend);
     static_for<0, NB>([&](auto bi) {
         constexpr int b = decltype(bi)::value;
         constexpr int b0 = b * kBlock;
-        if constexpr (TOP) issue<fwd_front<H, N, IO::kLook>(b - 1), fwd_front<H, N, IO::kLook>(b)>(io, in);
+        if constexpr (TOP && IO::kTwoPhase) {
+            // block b's samples were fetched one block earlier; fetch block b+1's now
+            io.template unpack<fwd_front<H, N, 0>(b - 1), fwd_front<H, N, 0>(b)>(pend, in);
+            io.template fetch<fwd_front<H, N, 0>(b), fwd_front<H, N, 0>(b + 1)>(pend);
+        } else if constexpr (TOP) {
+            issue<fwd_front<H, N, IO::kLook>(b - 1), fwd_front<H, N, IO::kLook>(b)>(io, in);
+        }
         Arr<float, NP> A, D, SA, SD;
         static_for<0, NP>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
@@ -323,7 +401,13 @@ WHHD_DEV void forward_line_lp(const IO& io) {
         fwd_level<W, LEV - 1, EXT, true>(io, x, out);
         fwd_levels_below<W, LEV - 2, EXT>(io, out);
     } else {
-        issue<0, (1 << LEV) + EXT>(io, x);
+        if constexpr (IO::kTwoPhase) {
+            typename IO::Pend pend;
+            io.template fetch<0, (1 << LEV) + EXT>(pend);
+            io.template unpack<0, (1 << LEV) + EXT>(pend, x);
+        } else {
+            issue<0, (1 << LEV) + EXT>(io, x);
+        }
         fwd_levels_below<W, 1, EXT>(io, x);
     }
 }
@@ -347,11 +431,28 @@ constexpr int inv_front_before(int l, int b) {
                            : inv_front<H, EXT, N, M, LA>(l - 1, (((1 << (l - 1)) + EXT + kBlock - 1) / kBlock) - 1));
 }
 
+// Two-phase fronts: the samples step (l, b) needs, rounded to whole 16-byte
+// pieces (l = 1 is the coarse block); the step after (l, b) is (l, b+1), then
+// (l+1, 0), and after the last block of level LEV-1 nothing.
+template <int H, int EXT, int N, int M>
+constexpr int inv_need2(int l, int b) {
+    return cmin(N, round4(l < 2 ? M : inv_need<H, EXT, N>(l, b)));
+}
+template <int H, int EXT, int N, int M, int LEV>
+constexpr int inv_need2_next(int l, int b) {
+    return l < 2 ? (LEV > 2 ? inv_need2<H, EXT, N, M>(2, 0) : N)
+                 : (b + 1 < (((1 << l) + EXT + kBlock - 1) / kBlock)
+                        ? inv_need2<H, EXT, N, M>(l, b + 1)
+                        : (l + 1 < LEV ? inv_need2<H, EXT, N, M>(l + 1, 0) : N));
+}
+
 // One inverse level l: approximations [0, 2^l+EXT) from `in`, details from
 // the streamed line x at [2^l+EXT, 2^(l+1)+EXT); reconstructed samples to
 // `out`, or to the line when LAST.
-template <class W, int l, int EXT, int N, bool LAST, class IO>
-WHHD_DEV void inv_level(const IO& io, float* x, const float* in, float* out) {
+template <class W, int l, int LEV, int EXT, bool LAST, class IO>
+WHHD_DEV void inv_level(const IO& io, typename IO::Pend& pend, float* x, const float* in,
+                        float* out) {
+    constexpr int N = (1 << LEV) + EXT;
     constexpr int H = W::halo;
     constexpr int M = W::m10;
     constexpr int L = 1 << l;
@@ -361,7 +462,15 @@ WHHD_DEV void inv_level(const IO& io, float* x, const float* in, float* out) {
     static_for<0, NB>([&](auto bi) {
         constexpr int b = decltype(bi)::value;
         constexpr int b0 = b * kBlock;
-        issue<inv_front_before<H, EXT, N, M, IO::kLook>(l, b), inv_front<H, EXT, N, M, IO::kLook>(l, b)>(io, x);
+        if constexpr (IO::kTwoPhase) {
+            constexpr int prev = b > 0 ? inv_need2<H, EXT, N, M>(l, b - 1)
+                                       : (l == 2 ? inv_need2<H, EXT, N, M>(1, 0)
+                                                 : inv_need2<H, EXT, N, M>(l - 1, (((1 << (l - 1)) + EXT + kBlock - 1) / kBlock) - 1));
+            io.template unpack<prev, inv_need2<H, EXT, N, M>(l, b)>(pend, x);
+            io.template fetch<inv_need2<H, EXT, N, M>(l, b), inv_need2_next<H, EXT, N, M, LEV>(l, b)>(pend);
+        } else {
+            issue<inv_front_before<H, EXT, N, M, IO::kLook>(l, b), inv_front<H, EXT, N, M, IO::kLook>(l, b)>(io, x);
+        }
         Arr<float, NP> A, D, SA, SD;
         static_for<0, NP>([&](auto ji) {
             constexpr int j = decltype(ji)::value;
@@ -402,15 +511,14 @@ WHHD_DEV void inv_level(const IO& io, float* x, const float* in, float* out) {
 }
 
 template <class W, int l, int LEV, int EXT, class IO>
-WHHD_DEV void inv_levels_from(const IO& io, float* x, const float* in) {
+WHHD_DEV void inv_levels_from(const IO& io, typename IO::Pend& pend, float* x, const float* in) {
     // `in` holds the 2^l+EXT approximations of level l
-    constexpr int N = (1 << LEV) + EXT;
     if constexpr (l == LEV - 1) {
-        inv_level<W, l, EXT, N, true>(io, x, in, nullptr);
+        inv_level<W, l, LEV, EXT, true>(io, pend, x, in, nullptr);
     } else {
         float out[(1 << (l + 1)) + EXT];
-        inv_level<W, l, EXT, N, false>(io, x, in, out);
-        inv_levels_from<W, l + 1, LEV, EXT>(io, x, out);
+        inv_level<W, l, LEV, EXT, false>(io, pend, x, in, out);
+        inv_levels_from<W, l + 1, LEV, EXT>(io, pend, x, out);
     }
 }
 
@@ -418,8 +526,16 @@ template <class W, int LEV, int EXT, class IO>
 WHHD_DEV void inverse_line_lp(const IO& io) {
     constexpr int M = W::m10;
     constexpr int N = (1 << LEV) + EXT;
+    constexpr int H = W::halo;
     float x[N];
-    issue<0, inv_front<W::halo, EXT, N, M, IO::kLook>(1, 0)>(io, x);
+    typename IO::Pend pend;
+    if constexpr (IO::kTwoPhase) {
+        io.template fetch<0, inv_need2<H, EXT, N, M>(1, 0)>(pend);
+        io.template unpack<0, inv_need2<H, EXT, N, M>(1, 0)>(pend, x);
+        io.template fetch<inv_need2<H, EXT, N, M>(1, 0), inv_need2_next<H, EXT, N, M, LEV>(1, 0)>(pend);
+    } else {
+        issue<0, inv_front<H, EXT, N, M, IO::kLook>(1, 0)>(io, x);
+    }
     float a[M];
     static_for<0, M>([&](auto ri) {
         constexpr int r = decltype(ri)::value;
@@ -431,7 +547,7 @@ WHHD_DEV void inverse_line_lp(const IO& io) {
         a[r] = acc;
     });
     if constexpr (LEV > 2) {
-        inv_levels_from<W, 2, LEV, EXT>(io, x, a);
+        inv_levels_from<W, 2, LEV, EXT>(io, pend, x, a);
     } else {
         io.template store_run<0, M>(a);
     }
@@ -471,8 +587,9 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
         __shared__ float stage[4][kStageRun * kStageRow];
         const int64_t here = g.total_lines - wave_first;
         const uint32_t voff = (uint32_t)(l * n * 4);
-        ContigIO io{src + l * n, rd, valid ? voff : kOutOfRange, (uint32_t)(wave_first * n * 4),
-                    (int)(here < 64 ? here : 64), n, stage[threadIdx.x >> 6], lane_};
+        ContigIO io{src + l * n, rs, rd, valid ? voff : kOutOfRange,
+                    (uint32_t)(wave_first * n * 4), (int)(here < 64 ? here : 64), n,
+                    stage[threadIdx.x >> 6], lane_};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     } else {

@@ -23,6 +23,13 @@
 
 namespace whhd {
 
+#ifndef WHHD_PROJ_ZB
+#define WHHD_PROJ_ZB 4
+#endif
+#ifndef WHHD_PROJ_NT
+#define WHHD_PROJ_NT 0
+#endif
+
 constexpr double kEps = 1e-12;  // eps<DTYPE>, WaveletPotentials/Utils.h:52-53
 
 static int bitlen(int v) {
@@ -70,7 +77,12 @@ __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
     const int zend = min(z0 + ZB, p.nz);
     for (int z = z0; z < zend; ++z) {
         const float sz = p.cz.s[z];
+#if WHHD_PROJ_NT
+        const float vx = __builtin_nontemporal_load(p.xv + ixv), vy = __builtin_nontemporal_load(p.yv + iyv),
+                    vz = __builtin_nontemporal_load(p.zv + izv);
+#else
         const float vx = p.xv[ixv], vy = p.yv[iyv], vz = p.zv[izv];
+#endif
         float lx = 0.f, ly = 0.f, lz = 0.f;
         if (x > 0 && y > 0 && z > 0) {
             const float inv = 1.f / (sxx_syy + sz * sz);
@@ -283,7 +295,7 @@ int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, fl
     if ((xo || yo || zo) && !(xo && yo && zo))
         return fail(WHHD_INVALID_ARG, "project3d_run: give all three or none of the v_out arrays");
     Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz};
-    constexpr int ZB = 4;
+    constexpr int ZB = WHHD_PROJ_ZB;
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 63) / 64, (h->nx + 3) / 4, (h->nz + ZB - 1) / ZB);
     const int64_t biggest = (int64_t)(h->nx + 1) * (h->ny + 1) * (h->nz + 1);
