@@ -66,7 +66,7 @@ def measured_traffic(kernel):
     if not row:
         return None
     return {"bytes": int(row["fetch_bytes"] + row["write_bytes"]),
-            "source": f"profiles/traffic_recover3d_256.json ({rows.get('command', '')})"}
+            "source": "profiles/traffic_recover3d_256.json"}
 
 
 def main():
