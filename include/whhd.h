@@ -252,6 +252,38 @@ WHHD_API int whhd_reduce(const float* x, long long n, int op, double* scratch, d
                          void* stream);
 WHHD_API int whhd_reduce_scratch_doubles(void);
 
+/* ---- sweep + DST comparator ---------------------------------------------- */
+typedef struct whhd_dst3d_s* whhd_dst3d_t;
+/* CuDst3D(int3 dim, DTYPE3 dx) (WaveletPotentials/cuDst3D.cu:183-252): a DST-I
+ * Poisson solver on an (n1, n2, n3) node grid.  Holds the sine matrices of the
+ * three axes and one scratch array. */
+WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz,
+                               whhd_dst3d_t* out);
+/* CuDst3D::Solve(dst, f) (WaveletPotentials/cuDst3D.cu:265-341): phi with
+ * (-L_h) phi = b for the 7-point Laplacian, zero one node outside the grid; the
+ * eigenvalues use each axis's own spacing (the reference scales every axis by
+ * dx.x*dx.y, the same for cubic cells).  phi may equal b. */
+WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream);
+WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h);
+
+typedef struct whhd_sweep3d_s* whhd_sweep3d_t;
+/* CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cu:
+ * 876-895) for nx*ny*nz cells; only velocity bc 'n' is implemented.  Owns a
+ * DST solver on the (nx-1, ny-1, nz-1) interior nodes. */
+WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                                 whhd_sweep3d_t* out);
+/* Solve(qx, qy, qz, xv, yv, zv) (WaveletPotentials/cuParallelSweep3D.cu:939-965): qx
+ * and qy integrate yv and -xv along z from zero at the bottom, qz = 0, then the
+ * top planes of qx, qy are cleared and qz's top plane takes the x-integral of
+ * the cleared qx.  zv is not read. */
+WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
+                                const float* xv, const float* yv, const float* zv, void* stream);
+/* Project(qx, qy, qz) for bc 'n' = ProjectZero (WaveletPotentials/cuParallelSweep3D.cu:
+ * 1054-1064, 1316-1326): q -= grad phi with (-L_h) phi = -div q at the interior
+ * nodes, phi = 0 on the boundary nodes. */
+WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz, void* stream);
+WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h);
+
 #ifdef __cplusplus
 }
 #endif

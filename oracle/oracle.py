@@ -67,6 +67,9 @@ def _setup(L):
     L.orc_mgfrac3d_rel_residual.restype = d
     L.orc_gradient3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d, c]
     L.orc_divergence3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d]
+    L.orc_sweep3d.argtypes = [_D] * 5 + [i, i, i, d, d, d]
+    L.orc_dst3d_solve.argtypes = [_D, _D, i, i, i, d, d, d]
+    L.orc_project_zero.argtypes = [_D] * 3 + [i, i, i, d, d, d]
     L.orc_curl3d.restype = None
     L.orc_curl2d.restype = None
     L.orc_div3d.restype = None
@@ -313,3 +316,33 @@ def divergence3d_frac(xv, yv, zv, ax, ay, az, dx):
     _check(lib().orc_divergence3d_frac(_p(b), _p(xv), _p(yv), _p(zv), _p(ax), _p(ay), _p(az), nx,
                                        ny, nz, *map(float, dx)), "divergence3d_frac")
     return b
+
+
+# ---- sweep + DST comparator ------------------------------------------------
+def sweep3d(xv, yv, n, dx):
+    """CuParallelSweep3D::Solve (bc 'n'): returns (qx, qy, qz)."""
+    nx, ny, nz = n
+    xv = np.ascontiguousarray(xv, dtype=np.float64)
+    yv = np.ascontiguousarray(yv, dtype=np.float64)
+    qx, qy, qz = field(nx, ny + 1, nz + 1), field(nx + 1, ny, nz + 1), field(nx + 1, ny + 1, nz)
+    _check(lib().orc_sweep3d(_p(qx), _p(qy), _p(qz), _p(xv), _p(yv), nx, ny, nz, *map(float, dx)),
+           "sweep3d")
+    return qx, qy, qz
+
+
+def dst3d_solve(b, dx):
+    """(-L_h) phi = b with zero values one node outside; b is (n3, n1, n2)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n3, n1, n2 = b.shape
+    phi = np.empty_like(b)
+    _check(lib().orc_dst3d_solve(_p(phi), _p(b), n1, n2, n3, *map(float, dx)), "dst3d_solve")
+    return phi
+
+
+def project_zero(qx, qy, qz, n, dx):
+    """CuParallelSweep3D::ProjectZero; returns new arrays."""
+    nx, ny, nz = n
+    q = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (qx, qy, qz)]
+    _check(lib().orc_project_zero(_p(q[0]), _p(q[1]), _p(q[2]), nx, ny, nz, *map(float, dx)),
+           "project_zero")
+    return tuple(q)

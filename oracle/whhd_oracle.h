@@ -124,6 +124,16 @@ int orc_gradient3d_frac(double* xv, double* yv, double* zv, const double* p, con
 int orc_divergence3d_frac(double* b, const double* xv, const double* yv, const double* zv,
                           const double* ax, const double* ay, const double* az, int nx, int ny,
                           int nz, double dx, double dy, double dz);
+/* ---- sweep + DST comparator (sweep_oracle.c) ---- */
+/* CuParallelSweep3D::Solve for velocity bc 'n': qx, qy by z sweeps, qz = 0, top gauge fix. */
+int orc_sweep3d(double* qx, double* qy, double* qz, const double* xv, const double* yv,
+                int nx, int ny, int nz, double dx, double dy, double dz);
+/* (-L_h) phi = b on an (n1, n2, n3) node grid, zero one node outside (CuDst3D::Solve). */
+int orc_dst3d_solve(double* phi, const double* b, int n1, int n2, int n3, double dx, double dy,
+                    double dz);
+/* CuParallelSweep3D::ProjectZero: q -= grad phi with (-L_h) phi = -div q at interior nodes. */
+int orc_project_zero(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
+                     double dy, double dz);
 #ifdef __cplusplus
 }
 #endif

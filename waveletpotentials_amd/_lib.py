@@ -59,6 +59,13 @@ SIGNATURES = {
     "whhd_mul": (_I, [_P, _P, _P, ctypes.c_longlong, _P]),
     "whhd_reduce": (_I, [_P, ctypes.c_longlong, _I, _P, ctypes.POINTER(_D), _P]),
     "whhd_reduce_scratch_doubles": (_I, []),
+    "whhd_dst3d_create": (_I, [_I, _I, _I, _D, _D, _D, ctypes.POINTER(_P)]),
+    "whhd_dst3d_solve": (_I, [_P, _P, _P, _P]),
+    "whhd_dst3d_destroy": (None, [_P]),
+    "whhd_sweep3d_create": (_I, [_I, _I, _I, _D, _D, _D, _C, ctypes.POINTER(_P)]),
+    "whhd_sweep3d_solve": (_I, [_P] * 7 + [_P]),
+    "whhd_sweep3d_project": (_I, [_P] * 4 + [_P]),
+    "whhd_sweep3d_destroy": (None, [_P]),
 }
 
 _lib = None

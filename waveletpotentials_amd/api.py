@@ -28,7 +28,7 @@ from ._lib import WhhdError, check, lib
 __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "string_from_type",
            "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuGradient",
            "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
-           "CudaArrayAbsSum", "CudaFindMaxValue", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
+           "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -527,3 +527,49 @@ def CudaArrayAbsSum(src, size):
 def CudaFindMaxValue(src, size):
     """max |x| over the array (cudaMath.cu:1269-1280; reduced on the device)."""
     return _reduce(src, size, 3)
+
+
+class CuDst3D(_Handle):
+    """CuDst3D(int3 dim, DTYPE3 dx) (WaveletPotentials/cuDst3D.cuh): DST-I Poisson solver."""
+    _destroy = "whhd_dst3d_destroy"
+
+    def __init__(self, dim, dx):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        check(lib().whhd_dst3d_create(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)),
+              "CuDst3D")
+
+    def Solve(self, dst, f):
+        """Solve(dst, f): (-L_h) dst = f (WaveletPotentials/cuDst3D.cu:265-341)."""
+        n = self.dim[0] * self.dim[1] * self.dim[2]
+        check(lib().whhd_dst3d_solve(self._h, _ptr(dst, numel=n, name="dst"),
+                                     _ptr(f, numel=n, name="f"), _stream()), "CuDst3D.Solve")
+
+
+class CuParallelSweep3D(_Handle):
+    """CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cuh):
+    the sweep + DST comparator (bc 'n')."""
+    _destroy = "whhd_sweep3d_destroy"
+
+    def __init__(self, dim, dx, bc="n"):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        check(lib().whhd_sweep3d_create(*self.dim, *(float(v) for v in dx), _b(bc),
+                                        ctypes.byref(self._h)), "CuParallelSweep3D")
+        s = mac_shapes3d(*self.dim)
+        self._size = {k: int(torch.Size(v).numel()) for k, v in s.items()}
+
+    def _q(self, qx, qy, qz):
+        s = self._size
+        return [_ptr(qx, numel=s["qx"], name="qx"), _ptr(qy, numel=s["qy"], name="qy"),
+                _ptr(qz, numel=s["qz"], name="qz")]
+
+    def Solve(self, qx, qy, qz, xv, yv, zv):
+        s = self._size
+        check(lib().whhd_sweep3d_solve(self._h, *self._q(qx, qy, qz),
+                                       _ptr(xv, numel=s["xv"], name="xv"),
+                                       _ptr(yv, numel=s["yv"], name="yv"),
+                                       _ptr(zv, numel=s["zv"], name="zv"), _stream()), "Solve")
+
+    def Project(self, qx, qy, qz):
+        check(lib().whhd_sweep3d_project(self._h, *self._q(qx, qy, qz), _stream()), "Project")

@@ -1,0 +1,389 @@
+// Sweep + DST comparator: a vector potential by integrating the velocity along
+// z, made divergence free by a DST-I Poisson solve.
+//   CuParallelSweep3D::Solve   WaveletPotentials/cuParallelSweep3D.cu:9-66, 939-965
+//   CuParallelSweep3D::Project WaveletPotentials/cuParallelSweep3D.cu:1054-1064, 1316-1326
+//   CuDst3D::Solve             WaveletPotentials/cuDst3D.cu:183-341
+// The reference sweeps one column per thread with cuFFT for the DST.  Here the
+// sweep is the same column walk (y-fastest, so each z step of a wavefront is a
+// coalesced row) and the DST-I is a dense product with the orthonormal sine
+// matrix per axis: an LDS-tiled kernel over a 64 x 64 output tile, the line
+// axis strided through memory as the array lays it out, so no transposes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "whhd_internal.h"
+
+namespace whhd {
+namespace {
+
+// One thread per node column (x <= nx, y <= ny): the qx column (x < nx) and qy
+// column (y < ny) are running sums along z (SweepingQx / SweepingQy), the qz
+// column is cleared.
+__global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
+                             float* __restrict__ qz, const float* __restrict__ xv,
+                             const float* __restrict__ yv, int nx, int ny, int nz, float dz) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    if (y > ny || x > nx) return;
+    if (x < nx) {  // qx (nx, ny+1, nz+1) from yv (nx, ny+1, nz)
+        const int64_t s = (int64_t)nx * (ny + 1);
+        const int64_t b = y + (int64_t)x * (ny + 1);
+        float acc = 0.f;
+        qx[b] = 0.f;
+#pragma unroll 8
+        for (int z = 1; z <= nz; ++z) {
+            acc += yv[b + (z - 1) * s] * dz;
+            qx[b + z * s] = acc;
+        }
+    }
+    if (y < ny) {  // qy (nx+1, ny, nz+1) from xv (nx+1, ny, nz)
+        const int64_t s = (int64_t)(nx + 1) * ny;
+        const int64_t b = y + (int64_t)x * ny;
+        float acc = 0.f;
+        qy[b] = 0.f;
+#pragma unroll 8
+        for (int z = 1; z <= nz; ++z) {
+            acc -= xv[b + (z - 1) * s] * dz;
+            qy[b + z * s] = acc;
+        }
+    }
+    {  // qz (nx+1, ny+1, nz) = 0
+        const int64_t s = (int64_t)(nx + 1) * (ny + 1);
+        const int64_t b = y + (int64_t)x * (ny + 1);
+        for (int z = 0; z < nz; ++z) qz[b + z * s] = 0.f;
+    }
+}
+
+// Top gauge for velocity bc 'n' (cuSweepingQ_x and the clears that follow it):
+// per y, qp(x) = sum_{x' < x} qx(x', y, top) * dx; the top planes of qx and qy
+// become zero and the top plane of qz takes -qp / dz.
+__global__ void top_gauge_kernel(float* __restrict__ qx, float* __restrict__ qy,
+                                 float* __restrict__ qz, int nx, int ny, int nz, float dx,
+                                 float dz) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y > ny) return;
+    const int64_t tqx = (int64_t)nz * nx * (ny + 1);        // top plane of qx
+    const int64_t tqy = (int64_t)nz * (nx + 1) * ny;        // top plane of qy
+    const int64_t tqz = (int64_t)(nz - 1) * (nx + 1) * (ny + 1);  // top plane of qz
+    float qp = 0.f;
+    for (int x = 1; x <= nx; ++x) {
+        float* e = qx + tqx + y + (int64_t)(x - 1) * (ny + 1);
+        qp += *e * dx;
+        *e = 0.f;
+        qz[tqz + y + (int64_t)x * (ny + 1)] -= qp / dz;
+    }
+    if (y < ny)
+        for (int x = 0; x <= nx; ++x) qy[tqy + y + (int64_t)x * ny] = 0.f;
+}
+
+// b = -div q at the interior nodes (GetB_3d_q_zeros); b is (nx-1, ny-1, nz-1).
+__global__ void node_div_kernel(float* __restrict__ b, const float* __restrict__ qx,
+                                const float* __restrict__ qy, const float* __restrict__ qz, int nx,
+                                int ny, int nz, float ihx, float ihy, float ihz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;  // y - 1
+    const int j = blockIdx.y * 4 + threadIdx.y;   // x - 1
+    const int k = blockIdx.z;                     // z - 1
+    const int py = ny - 1, px = nx - 1;
+    if (i >= py || j >= px) return;
+    const int64_t iqx = (i + 1) + (int64_t)j * (ny + 1) + (int64_t)(k + 1) * nx * (ny + 1);
+    const int64_t iqy = i + (int64_t)(j + 1) * ny + (int64_t)(k + 1) * (nx + 1) * ny;
+    const int64_t iqz = (i + 1) + (int64_t)(j + 1) * (ny + 1) + (int64_t)k * (nx + 1) * (ny + 1);
+    const float d = (qx[iqx + (ny + 1)] - qx[iqx]) * ihx + (qy[iqy + 1] - qy[iqy]) * ihy +
+                    (qz[iqz + (int64_t)(nx + 1) * (ny + 1)] - qz[iqz]) * ihz;
+    b[i + (int64_t)j * py + (int64_t)k * px * py] = -d;
+}
+
+// q -= grad phi with phi = 0 on the boundary nodes (GradientMinus3D_zero_Q):
+// one thread per node (x <= nx, y <= ny, z <= nz), owning the qx, qy, qz edges
+// that start at it.
+__device__ __forceinline__ float phi_at(const float* __restrict__ phi, int x, int y, int z, int nx,
+                                        int ny, int nz) {
+    if (x < 1 || y < 1 || z < 1 || x > nx - 1 || y > ny - 1 || z > nz - 1) return 0.f;
+    return phi[(y - 1) + (int64_t)(x - 1) * (ny - 1) + (int64_t)(z - 1) * (nx - 1) * (ny - 1)];
+}
+
+__global__ void grad_minus_kernel(float* __restrict__ qx, float* __restrict__ qy,
+                                  float* __restrict__ qz, const float* __restrict__ phi, int nx,
+                                  int ny, int nz, float ihx, float ihy, float ihz) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y > ny || x > nx) return;
+    const float p = phi_at(phi, x, y, z, nx, ny, nz);
+    if (x < nx)
+        qx[y + (int64_t)x * (ny + 1) + (int64_t)z * nx * (ny + 1)] -=
+            (phi_at(phi, x + 1, y, z, nx, ny, nz) - p) * ihx;
+    if (y < ny)
+        qy[y + (int64_t)x * ny + (int64_t)z * (nx + 1) * ny] -=
+            (phi_at(phi, x, y + 1, z, nx, ny, nz) - p) * ihy;
+    if (z < nz)
+        qz[y + (int64_t)x * (ny + 1) + (int64_t)z * (nx + 1) * (ny + 1)] -=
+            (phi_at(phi, x, y, z + 1, nx, ny, nz) - p) * ihz;
+}
+
+// out(i, k, b) = sum_j in(i, j, b) * S(k, j) for k, j < n, i < m, b < gridDim.z;
+// element (i, j, b) at i*si + j*sj + b*sb in both arrays.  A workgroup computes
+// a 64 (i) x 64 (k) tile, each thread 4 x 4, with 16-wide j slices staged in LDS.
+constexpr int kTI = 64, kTK = 64, kTJ = 16;
+
+__global__ void __launch_bounds__(256) dst_pass_kernel(float* __restrict__ out,
+                                                       const float* __restrict__ in,
+                                                       const float* __restrict__ S, int n, int m,
+                                                       int64_t si, int64_t sj, int64_t sb) {
+    __shared__ float As[kTJ][kTI + 1];
+    __shared__ float Bs[kTJ][kTK + 1];
+    const int i0 = blockIdx.x * kTI, k0 = blockIdx.y * kTK;
+    const int64_t boff = (int64_t)blockIdx.z * sb;
+    const int t = threadIdx.x;
+    const int ti = (t & 15) * 4, tk = (t >> 4) * 4;
+    float acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.f;
+    const bool jfast = (sj == 1);
+    for (int j0 = 0; j0 < n; j0 += kTJ) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = t + e * 256;
+            const int ii = jfast ? idx / kTJ : idx % kTI;
+            const int jj = jfast ? idx % kTJ : idx / kTI;
+            const int gi = i0 + ii, gj = j0 + jj;
+            As[jj][ii] = (gi < m && gj < n) ? in[boff + gi * si + gj * sj] : 0.f;
+            const int kk = idx / kTJ, jb = idx % kTJ;
+            const int gk = k0 + kk, gjb = j0 + jb;
+            Bs[jb][kk] = (gk < n && gjb < n) ? S[(int64_t)gk * n + gjb] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < kTJ; ++jj) {
+            float a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[jj][ti + r];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = Bs[jj][tk + c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int gi = i0 + ti + r, gk = k0 + tk + c;
+            if (gi < m && gk < n) out[boff + gi * si + gk * sj] = acc[r][c];
+        }
+}
+
+// phi_hat /= lambda_x + lambda_y + lambda_z (cuDstProj_dx)
+__global__ void eigen_scale_kernel(float* __restrict__ a, const float* __restrict__ lx,
+                                   const float* __restrict__ ly, const float* __restrict__ lz,
+                                   int n1, int n2, int n3) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= n2 || x >= n1) return;
+    a[y + (int64_t)x * n2 + (int64_t)z * n1 * n2] /= lx[x] + ly[y] + lz[z];
+}
+
+dim3 grid_xyz(int nx, int ny, int nz) { return dim3((ny + 63) / 64, (nx + 3) / 4, nz); }
+
+hipError_t done(const char* name) {
+    set_last_kernel(name);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace whhd
+
+struct whhd_dst3d_s {
+    int n1, n2, n3;
+    float* S[3] = {nullptr, nullptr, nullptr};
+    float* lam[3] = {nullptr, nullptr, nullptr};
+    float* tmp = nullptr;
+};
+
+struct whhd_sweep3d_s {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    whhd_dst3d_s* dst = nullptr;
+    float* b = nullptr;
+};
+
+namespace whhd {
+namespace {
+
+void dst_release(whhd_dst3d_s* h) {
+    if (!h) return;
+    for (int a = 0; a < 3; ++a) {
+        if (h->S[a]) (void)hipFree(h->S[a]);
+        if (h->lam[a]) (void)hipFree(h->lam[a]);
+    }
+    if (h->tmp) (void)hipFree(h->tmp);
+    delete h;
+}
+
+// one DST-I pass along axis a (0: x, 1: y, 2: z) of an (n1, n2, n3) y-fastest array
+int dst_pass(const whhd_dst3d_s* h, float* out, const float* in, int a, hipStream_t s) {
+    const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+    int n, m;
+    int64_t si, sj, sb;
+    unsigned nb;
+    if (a == 0) {
+        n = n1; m = n2; si = 1; sj = n2; sb = (int64_t)n1 * n2; nb = (unsigned)n3;
+    } else if (a == 1) {
+        n = n2; m = n1 * n3; si = n2; sj = 1; sb = 0; nb = 1;
+    } else {
+        n = n3; m = n1 * n2; si = 1; sj = (int64_t)n1 * n2; sb = 0; nb = 1;
+    }
+    dim3 grid((unsigned)((m + kTI - 1) / kTI), (unsigned)((n + kTK - 1) / kTK), nb);
+    hipLaunchKernelGGL(dst_pass_kernel, grid, dim3(256), 0, s, out, in, h->S[a], n, m, si, sj, sb);
+    hipError_t e = done("dst_pass_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "dst pass kernel");
+}
+
+#define TRY(e) do { int s_ = (e); if (s_ != WHHD_OK) return s_; } while (0)
+
+}  // namespace
+
+int dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz, whhd_dst3d_s** out) {
+    if (!out) return fail(WHHD_INVALID_ARG, "dst3d_create: null handle pointer");
+    *out = nullptr;
+    if (n1 < 1 || n2 < 1 || n3 < 1 || n3 > 65535 || (int64_t)n1 * n2 * n3 >= (1LL << 31))
+        return fail(WHHD_INVALID_ARG, "dst3d_create: extents out of range");
+    if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "dst3d_create: spacing <= 0");
+    auto* h = new whhd_dst3d_s;
+    h->n1 = n1; h->n2 = n2; h->n3 = n3;
+    const int ns[3] = {n1, n2, n3};
+    const double hs[3] = {dx, dy, dz};
+    for (int a = 0; a < 3; ++a) {
+        const int n = ns[a];
+        // orthonormal DST-I, S(k, j) = sqrt(2/(n+1)) sin(pi (k+1)(j+1) / (n+1)), and the
+        // eigenvalues (2 - 2 cos(pi (k+1)/(n+1))) / h^2 of the 3-point -d2/dx2
+        std::vector<float> S((size_t)n * n), lam(n);
+        const double sc = std::sqrt(2.0 / (n + 1));
+        for (int k = 0; k < n; ++k) {
+            for (int j = 0; j < n; ++j)
+                S[(size_t)k * n + j] = (float)(sc * std::sin(M_PI * (double)((int64_t)(k + 1) * (j + 1) % (2 * (n + 1))) / (n + 1)));
+            lam[k] = (float)((2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1))) / (hs[a] * hs[a]));
+        }
+        if (hipMalloc(&h->S[a], S.size() * 4) != hipSuccess ||
+            hipMalloc(&h->lam[a], lam.size() * 4) != hipSuccess ||
+            hipMemcpy(h->S[a], S.data(), S.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->lam[a], lam.data(), lam.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            dst_release(h);
+            return fail(WHHD_OOM, "dst3d_create: table allocation failed");
+        }
+    }
+    if (hipMalloc(&h->tmp, (size_t)n1 * n2 * n3 * 4) != hipSuccess) {
+        dst_release(h);
+        return fail(WHHD_OOM, "dst3d_create: scratch allocation failed");
+    }
+    *out = h;
+    return WHHD_OK;
+}
+
+int dst3d_solve(const whhd_dst3d_s* h, float* phi, const float* b, hipStream_t s) {
+    if (!h || !phi || !b) return fail(WHHD_INVALID_ARG, "dst3d_solve: null argument");
+    // b -> tmp (x) -> phi (y) -> tmp (z), scale, tmp -> phi (z) -> tmp (y) -> phi (x)
+    TRY(dst_pass(h, h->tmp, b, 0, s));
+    TRY(dst_pass(h, phi, h->tmp, 1, s));
+    TRY(dst_pass(h, h->tmp, phi, 2, s));
+    hipLaunchKernelGGL(eigen_scale_kernel, grid_xyz(h->n1, h->n2, h->n3), dim3(64, 4), 0, s, h->tmp,
+                       h->lam[0], h->lam[1], h->lam[2], h->n1, h->n2, h->n3);
+    hipError_t e = done("eigen_scale_kernel");
+    if (e != hipSuccess) return hip_fail(e, "eigen scale kernel");
+    TRY(dst_pass(h, phi, h->tmp, 2, s));
+    TRY(dst_pass(h, h->tmp, phi, 1, s));
+    return dst_pass(h, phi, h->tmp, 0, s);
+}
+
+}  // namespace whhd
+
+extern "C" {
+
+WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz,
+                               whhd_dst3d_t* out) {
+    return whhd::dst3d_create(n1, n2, n3, dx, dy, dz, out);
+}
+
+WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream) {
+    return whhd::dst3d_solve(h, phi, b, (hipStream_t)stream);
+}
+
+WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h) { whhd::dst_release(h); }
+
+WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                                 whhd_sweep3d_t* out) {
+    using namespace whhd;
+    if (!out) return fail(WHHD_INVALID_ARG, "sweep3d_create: null handle pointer");
+    *out = nullptr;
+    if (bc != 'n')
+        return fail(WHHD_INVALID_ARG, "sweep3d_create: only velocity bc 'n' is implemented");
+    if (nx < 2 || ny < 2 || nz < 2 || nz > 65534 ||
+        (int64_t)(nx + 1) * (ny + 1) * (nz + 1) >= (1LL << 31))
+        return fail(WHHD_INVALID_ARG, "sweep3d_create: extents out of range");
+    auto* h = new whhd_sweep3d_s;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    h->dx = dx; h->dy = dy; h->dz = dz;
+    int st = dst3d_create(nx - 1, ny - 1, nz - 1, dx, dy, dz, &h->dst);
+    if (st == WHHD_OK &&
+        hipMalloc(&h->b, (size_t)(nx - 1) * (ny - 1) * (nz - 1) * 4) != hipSuccess)
+        st = fail(WHHD_OOM, "sweep3d_create: scratch allocation failed");
+    if (st != WHHD_OK) {
+        dst_release(h->dst);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return WHHD_OK;
+}
+
+WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
+                                const float* xv, const float* yv, const float* zv, void* stream) {
+    using namespace whhd;
+    (void)zv;  // the z sweep integrates xv and yv only (SweepingQx/Qy)
+    if (!h || !qx || !qy || !qz || !xv || !yv)
+        return fail(WHHD_INVALID_ARG, "sweep3d_solve: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(sweep_kernel, grid_xyz(h->nx + 1, h->ny + 1, 1), dim3(64, 4), 0, s, qx, qy,
+                       qz, xv, yv, h->nx, h->ny, h->nz, (float)h->dz);
+    hipLaunchKernelGGL(top_gauge_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx, qy, qz,
+                       h->nx, h->ny, h->nz, (float)h->dx, (float)h->dz);
+    hipError_t e = done("sweep_kernels");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "sweep kernels");
+}
+
+WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
+                                  void* stream) {
+    using namespace whhd;
+    if (!h || !qx || !qy || !qz) return fail(WHHD_INVALID_ARG, "sweep3d_project: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    hipLaunchKernelGGL(node_div_kernel, grid_xyz(nx - 1, ny - 1, nz - 1), dim3(64, 4), 0, s, h->b,
+                       qx, qy, qz, nx, ny, nz, (float)(1.0 / h->dx), (float)(1.0 / h->dy),
+                       (float)(1.0 / h->dz));
+    hipError_t e = done("node_div_kernel");
+    if (e != hipSuccess) return hip_fail(e, "node divergence kernel");
+    int st = dst3d_solve(h->dst, h->b, h->b, s);
+    if (st != WHHD_OK) return st;
+    hipLaunchKernelGGL(grad_minus_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s, qx,
+                       qy, qz, h->b, nx, ny, nz, (float)(1.0 / h->dx), (float)(1.0 / h->dy),
+                       (float)(1.0 / h->dz));
+    e = done("grad_minus_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "gradient kernel");
+}
+
+WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h) {
+    if (!h) return;
+    whhd::dst_release(h->dst);
+    if (h->b) (void)hipFree(h->b);
+    delete h;
+}
+
+}  // extern "C"
