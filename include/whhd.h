@@ -154,6 +154,38 @@ WHHD_API int whhd_recover3d_trace_end(whhd_recover3d_t h, float* stage_us, int m
 WHHD_API const char* whhd_recover3d_trace_label(whhd_recover3d_t h, int stage);
 WHHD_API void whhd_recover3d_destroy(whhd_recover3d_t h);
 
+/* ---- batched recovery ---------------------------------------------------- */
+typedef struct whhd_batch3d_s* whhd_batch3d_t;
+/* Batched 3D recovery of n independent grids (no reference counterpart: the
+ * reference recovers one grid per CuWaveletPotentialRecover3D handle,
+ * WaveletPotentials/cuWaveletPotentialRecover3D.cu:6-61; each problem gets that
+ * solve).  dims holds (nx, ny, nz) and spacing (dx, dy, dz) per problem; bc and
+ * wavelets are shared.  Problems with equal extents and spacing form a size
+ * class recovered by one launch per stage; the classes run as parallel
+ * branches.  The handle owns its scratch, not the problems' arrays. */
+WHHD_API int whhd_batch3d_create(int n, const int* dims, const double* spacing, char bc,
+                                 int wavelet_curl, int wavelet_div, whhd_batch3d_t* out);
+/* Floats in the caller's arena and, when offsets is not NULL, offsets[6*i + k]
+ * (k = xv, yv, zv, qx, qy, qz) of problem i's arrays from the arena start. */
+WHHD_API int whhd_batch3d_layout(whhd_batch3d_t h, long long* arena_floats, long long* offsets);
+/* Recover every problem: q arrays from the velocity arrays of the arena. */
+WHHD_API int whhd_batch3d_solve(whhd_batch3d_t h, float* arena, void* stream);
+/* Capture the whole batch on `arena` as one HIP graph (replaces any earlier). */
+WHHD_API int whhd_batch3d_capture(whhd_batch3d_t h, float* arena);
+/* Launch the captured graph on `stream`. */
+WHHD_API int whhd_batch3d_replay(whhd_batch3d_t h, void* stream);
+/* Time each size class's launch shapes (1/2/4 wavefronts per line-pass
+ * workgroup, then 1/2/4/8 z cells per projection thread) over `reps` solves of
+ * the class on `arena` and keep the fastest; recaptures a captured graph.
+ * Synchronizes `stream`; overwrites the arena's q arrays. */
+WHHD_API int whhd_batch3d_autotune(whhd_batch3d_t h, float* arena, int reps, void* stream);
+WHHD_API int whhd_batch3d_num_classes(whhd_batch3d_t h);
+/* info[6] = nx, ny, nz, problems, line-pass wavefronts, projection z block. */
+WHHD_API int whhd_batch3d_class_info(whhd_batch3d_t h, int cls, int* info);
+/* Set a class's launch shapes by hand; drops a captured graph. */
+WHHD_API int whhd_batch3d_set_tiles(whhd_batch3d_t h, int cls, int lp_waves, int proj_zb);
+WHHD_API void whhd_batch3d_destroy(whhd_batch3d_t h);
+
 typedef struct whhd_recover2d_s* whhd_recover2d_t;
 /* CuWaveletPotentialRecover2D(int2 dim, DTYPE2 dx, char bc, WaveletType curl,
  * WaveletType div) (WaveletPotentials/cuWaveletPotentialRecover2D.cuh:12, .cu:6-12). */

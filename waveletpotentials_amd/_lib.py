@@ -41,6 +41,16 @@ SIGNATURES = {
     "whhd_recover3d_trace_end": (_I, [_P, _P, _I, _P, _P]),
     "whhd_recover3d_trace_label": (_S, [_P, _I]),
     "whhd_recover3d_destroy": (None, [_P]),
+    "whhd_batch3d_create": (_I, [_I, _P, _P, _C, _I, _I, ctypes.POINTER(_P)]),
+    "whhd_batch3d_layout": (_I, [_P, _P, _P]),
+    "whhd_batch3d_solve": (_I, [_P, _P, _P]),
+    "whhd_batch3d_capture": (_I, [_P, _P]),
+    "whhd_batch3d_replay": (_I, [_P, _P]),
+    "whhd_batch3d_autotune": (_I, [_P, _P, _I, _P]),
+    "whhd_batch3d_num_classes": (_I, [_P]),
+    "whhd_batch3d_class_info": (_I, [_P, _I, _P]),
+    "whhd_batch3d_set_tiles": (_I, [_P, _I, _I, _I]),
+    "whhd_batch3d_destroy": (None, [_P]),
     "whhd_recover2d_create": (_I, [_I, _I, _D, _D, _C, _I, _I, ctypes.POINTER(_P)]),
     "whhd_recover2d_solve": (_I, [_P, _P, _P, _P, _P]),
     "whhd_recover2d_destroy": (None, [_P]),

@@ -29,7 +29,8 @@ __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "st
            "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuGradient",
            "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
            "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
-           "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "WhhdError",
+           "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "CuBatchRecover3D",
+           "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
 
@@ -318,6 +319,76 @@ class CuWaveletPotentialRecover3D(_Handle):
             stage, _, kernel = label.partition(" | ")
             out.append({"stage": stage, "kernel": kernel, "us": float(us[i])})
         return out, nsol.value
+
+
+class CuBatchRecover3D(_Handle):
+    """Batched recovery of independent 3D grids of uneven sizes.  No reference
+    counterpart: each problem gets CuWaveletPotentialRecover3D::Solve
+    (WaveletPotentials/cuWaveletPotentialRecover3D.cu:16-61).  `dims` and `dxs`
+    list (nx, ny, nz) and (dx, dy, dz) per problem.  The arrays of every problem
+    live in one device arena owned by this object; :meth:`arrays` returns views
+    of problem i's xv, yv, zv, qx, qy, qz shaped as :func:`mac_shapes3d`."""
+    _destroy = "whhd_batch3d_destroy"
+    _KEYS = ("xv", "yv", "zv", "qx", "qy", "qz")
+
+    def __init__(self, dims, dxs, bc, type_w_curl, type_w_div):
+        super().__init__()
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        n = len(self.dims)
+        d = (ctypes.c_int * (3 * n))(*[v for t in self.dims for v in t])
+        x = (ctypes.c_double * (3 * n))(*[float(v) for t in dxs for v in t])
+        if len(x) != 3 * n:
+            raise ValueError("one (dx, dy, dz) per problem")
+        L = lib()
+        check(L.whhd_batch3d_create(n, d, x, _b(bc), int(type_w_curl), int(type_w_div),
+                                    ctypes.byref(self._h)), "CuBatchRecover3D")
+        total = ctypes.c_longlong()
+        offs = (ctypes.c_longlong * (6 * n))()
+        check(L.whhd_batch3d_layout(self._h, ctypes.byref(total), offs), "CuBatchRecover3D.layout")
+        self.arena = torch.zeros(int(total.value), dtype=torch.float32, device="cuda")
+        self._views = []
+        for i, (nx, ny, nz) in enumerate(self.dims):
+            shp = mac_shapes3d(nx, ny, nz)
+            v = {}
+            for k, key in enumerate(self._KEYS):
+                o, m = int(offs[6 * i + k]), int(torch.Size(shp[key]).numel())
+                v[key] = self.arena[o:o + m].view(shp[key])
+            self._views.append(v)
+
+    def arrays(self, i):
+        return self._views[i]
+
+    def _arena(self):
+        return ctypes.c_void_p(self.arena.data_ptr())
+
+    def Solve(self):
+        check(lib().whhd_batch3d_solve(self._h, self._arena(), _stream()), "CuBatchRecover3D.Solve")
+
+    def Capture(self):
+        """Capture the whole batch as one HIP graph (on this object's arena)."""
+        check(lib().whhd_batch3d_capture(self._h, self._arena()), "CuBatchRecover3D.Capture")
+
+    def Replay(self):
+        check(lib().whhd_batch3d_replay(self._h, _stream()), "CuBatchRecover3D.Replay")
+
+    def Autotune(self, reps=3):
+        """Pick each size class's launch shapes by timing; overwrites the q arrays."""
+        check(lib().whhd_batch3d_autotune(self._h, self._arena(), int(reps), _stream()),
+              "CuBatchRecover3D.Autotune")
+
+    def classes(self):
+        L = lib()
+        out = []
+        for c in range(L.whhd_batch3d_num_classes(self._h)):
+            info = (ctypes.c_int * 6)()
+            check(L.whhd_batch3d_class_info(self._h, c, info), "CuBatchRecover3D.classes")
+            out.append({"dims": tuple(info[:3]), "problems": info[3], "lp_waves": info[4],
+                        "proj_zb": info[5]})
+        return out
+
+    def SetTiles(self, cls, lp_waves, proj_zb):
+        check(lib().whhd_batch3d_set_tiles(self._h, int(cls), int(lp_waves), int(proj_zb)),
+              "CuBatchRecover3D.SetTiles")
 
 
 class CuWaveletPotentialRecover2D(_Handle):

@@ -557,13 +557,16 @@ WHHD_DEV void inverse_line_lp(const IO& io) {
 // ---- kernel -----------------------------------------------------------------
 // Lines are numbered li = outer * lines_inner + inner; line li starts at
 // inner * inner_stride + outer * outer_stride and its samples are `step` apart.
+// Problems of a batch are arrays of equal extents `batch_stride` elements
+// apart; grid row blockIdx.y transforms problem blockIdx.y.
 struct LpGeom {
     int64_t total_lines;
-    int64_t total_elems;  // elements in the array (buffer range)
+    int64_t total_elems;  // elements in one problem's array (buffer range)
     int lines_inner;
     int64_t inner_stride;
     int64_t outer_stride;
     int64_t step;
+    int64_t batch_stride;
 };
 
 template <class W, int LEV, int EXT, bool FWD, bool CONTIG>
@@ -571,6 +574,8 @@ template <class W, int LEV, int EXT, bool FWD, bool CONTIG>
 // per SIMD, so the register budget goes to loads in flight, not occupancy.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
+    src += (int64_t)blockIdx.y * g.batch_stride;
+    dst += (int64_t)blockIdx.y * g.batch_stride;
     const int lane_ = threadIdx.x & 63;
     const int64_t wave_first = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
     const int64_t li = wave_first + lane_;
@@ -584,12 +589,13 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void*)dst, 0, bytes, kBufWord3);
     if constexpr (CONTIG) {
         // contiguous lines are numbered in memory order: line li starts at li * n
-        __shared__ float stage[4][kStageRun * kStageRow];
+        // one kStageRun x kStageRow staging area per wavefront (dynamic LDS)
+        extern __shared__ float lp_stage[];
         const int64_t here = g.total_lines - wave_first;
         const uint32_t voff = (uint32_t)(l * n * 4);
         ContigIO io{src + l * n, rs, rd, valid ? voff : kOutOfRange,
                     (uint32_t)(wave_first * n * 4), (int)(here < 64 ? here : 64), n,
-                    stage[threadIdx.x >> 6], lane_};
+                    lp_stage + (threadIdx.x >> 6) * (kStageRun * kStageRow), lane_};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     } else {
@@ -600,17 +606,22 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
     }
 }
 
-// Launch the line-parallel kernel for `levels` (5..8); false if none exists.
+// Launch the line-parallel kernel for `levels` (5..8) over `nb` problems with
+// `waves` (1, 2 or 4) wavefronts per workgroup; false if none exists.
 template <class W, int EXT, bool FWD, bool CONTIG>
-bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hipStream_t s) {
-    const dim3 block(256);
-    const int64_t lines_per_block = 4 * 64;
-    const dim3 grid((unsigned)((g.total_lines + lines_per_block - 1) / lines_per_block));
+bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, int waves, int nb,
+                   hipStream_t s) {
+    if (waves != 1 && waves != 2 && waves != 4) return false;
+    if (nb < 1 || nb > 65535) return false;
+    const dim3 block(64 * waves);
+    const int64_t lines_per_block = 64 * waves;
+    const dim3 grid((unsigned)((g.total_lines + lines_per_block - 1) / lines_per_block), (unsigned)nb);
+    const size_t lds = CONTIG ? (size_t)waves * kStageRun * kStageRow * sizeof(float) : 0;
     switch (levels) {
-    case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
-    case 6: hipLaunchKernelGGL((lp_kernel<W, 6, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
-    case 7: hipLaunchKernelGGL((lp_kernel<W, 7, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
-    case 8: hipLaunchKernelGGL((lp_kernel<W, 8, EXT, FWD, CONTIG>), grid, block, 0, s, dst, src, g); return true;
+    case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
+    case 6: hipLaunchKernelGGL((lp_kernel<W, 6, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
+    case 7: hipLaunchKernelGGL((lp_kernel<W, 7, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
+    case 8: hipLaunchKernelGGL((lp_kernel<W, 8, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
     default: return false;
     }
 }
@@ -622,12 +633,12 @@ bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, hi
 #ifndef WHHD_HOST_EMULATION
 template <class W, int EXT>
 bool lp_launch(int levels, bool fwd, bool contig, float* dst, const float* src,
-               const lp::LpGeom& g, hipStream_t s) {
+               const lp::LpGeom& g, int waves, int nb, hipStream_t s) {
     if (contig)
-        return fwd ? lp::lp_launch_one<W, EXT, true, true>(levels, dst, src, g, s)
-                   : lp::lp_launch_one<W, EXT, false, true>(levels, dst, src, g, s);
-    return fwd ? lp::lp_launch_one<W, EXT, true, false>(levels, dst, src, g, s)
-               : lp::lp_launch_one<W, EXT, false, false>(levels, dst, src, g, s);
+        return fwd ? lp::lp_launch_one<W, EXT, true, true>(levels, dst, src, g, waves, nb, s)
+                   : lp::lp_launch_one<W, EXT, false, true>(levels, dst, src, g, waves, nb, s);
+    return fwd ? lp::lp_launch_one<W, EXT, true, false>(levels, dst, src, g, waves, nb, s)
+               : lp::lp_launch_one<W, EXT, false, false>(levels, dst, src, g, waves, nb, s);
 }
 
 #endif  // WHHD_HOST_EMULATION

@@ -251,7 +251,11 @@ static hipError_t dispatch(int wt, char bc, T* dst, const T* src, const LineGeom
 
 template <class T>
 int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, char axis, char bc,
-                    int wt, bool fwd, hipStream_t stream, const TileConfig* cfg) {
+                    int wt, bool fwd, hipStream_t stream, const TileConfig* cfg,
+                    const LineBatch* batch) {
+    const LineBatch one{};
+    const LineBatch& b = batch ? *batch : one;
+    if (b.nb < 1 || (b.nb > 1 && b.stride < (int64_t)nx * ny * nz)) return WHHD_INVALID_ARG;
     int st = check_wavelet(wt, bc);
     if (st != WHHD_OK) return st;
     if (nx <= 0 || ny <= 0 || nz <= 0 || !dst || !src) return WHHD_INVALID_ARG;
@@ -274,7 +278,7 @@ int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, ch
     if constexpr (sizeof(T) == sizeof(float)) {
         if ((!cfg || cfg->allow_line_parallel) && g_line_kernel_mode.load() == 0) {
             const int r = lp_transform((float*)dst, (const float*)src, nx, ny, nz, levels, axis, bc,
-                                       wt, fwd, stream);
+                                       wt, fwd, stream, b);
             if (r != -1) return r;
         }
     }
@@ -285,7 +289,10 @@ int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, ch
     g.tile = strided ? c.strided_tile : c.contig_tile;
     if (g.tile <= 0) return WHHD_INVALID_ARG;
     if ((size_t)2 * g.tile * g.lds_row * sizeof(T) > kMaxLdsBytes) return WHHD_INVALID_ARG;
-    hipError_t e = dispatch<T>(wt, bc, dst, src, g, fwd, strided, c.threads, stream);
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < b.nb && e == hipSuccess; ++p)  // the tile kernels take one problem a launch
+        e = dispatch<T>(wt, bc, dst + p * b.stride, src + p * b.stride, g, fwd, strided, c.threads,
+                        stream);
     char name[96];
     std::snprintf(name, sizeof(name), "line_tile_kernel<%s %c,L=%d,%s,%s,%s>", wavelet_name(wt), bc,
                   levels, fwd ? "fwd" : "inv", strided ? "strided" : "contiguous",
@@ -295,8 +302,8 @@ int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, ch
 }
 
 template int transform_lines<float>(float*, const float*, int, int, int, int, char, char, int, bool,
-                                    hipStream_t, const TileConfig*);
+                                    hipStream_t, const TileConfig*, const LineBatch*);
 template int transform_lines<double>(double*, const double*, int, int, int, int, char, char, int,
-                                     bool, hipStream_t, const TileConfig*);
+                                     bool, hipStream_t, const TileConfig*, const LineBatch*);
 
 }  // namespace whhd

@@ -3,5 +3,6 @@
 
 namespace whhd {
 template bool lp_launch<W_Cdf37N, 0>(int, bool, bool, float*, const float*, const lp::LpGeom&,
+                                   int, int,
                                    hipStream_t);
 }  // namespace whhd

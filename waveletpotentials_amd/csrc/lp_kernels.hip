@@ -12,17 +12,18 @@
 namespace whhd {
 
 extern template bool lp_launch<W_Cdf37N, 0>(int, bool, bool, float*, const float*,
-                                            const lp::LpGeom&, hipStream_t);
+                                            const lp::LpGeom&, int, int, hipStream_t);
 extern template bool lp_launch<W_Cdf46D, 1>(int, bool, bool, float*, const float*,
-                                            const lp::LpGeom&, hipStream_t);
+                                            const lp::LpGeom&, int, int, hipStream_t);
 
 // Returns WHHD_OK when a line-parallel kernel was launched, -1 when none fits
 // (the caller then uses the tile kernel).  Needs dst != src: lines are read
 // and written in one pass without staging.
 int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int levels, char axis,
-                 char bc, int wt, bool fwd, hipStream_t stream) {
+                 char bc, int wt, bool fwd, hipStream_t stream, const LineBatch& b) {
     if (dst == src) return -1;
     lp::LpGeom g{};
+    g.batch_stride = b.stride;
     int n;
     switch (axis) {
     case 'y': case 'Y':
@@ -42,9 +43,9 @@ int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int level
     bool ok = false;
     const bool contig = (axis == 'y' || axis == 'Y');
     if (wt == WT_CDF3_7 && bc == 'n' && n == (1 << levels))
-        ok = lp_launch<W_Cdf37N, 0>(levels, fwd, contig, dst, src, g, stream);
+        ok = lp_launch<W_Cdf37N, 0>(levels, fwd, contig, dst, src, g, b.waves, b.nb, stream);
     else if (wt == WT_CDF4_6 && bc == 'd' && n == (1 << levels) + 1)
-        ok = lp_launch<W_Cdf46D, 1>(levels, fwd, contig, dst, src, g, stream);
+        ok = lp_launch<W_Cdf46D, 1>(levels, fwd, contig, dst, src, g, b.waves, b.nb, stream);
     if (!ok) return -1;
     char name[96];
     std::snprintf(name, sizeof(name), "lp_kernel<%s %c,L=%d,%s,%s>", wavelet_name(wt), bc, levels,

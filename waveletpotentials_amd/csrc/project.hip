@@ -46,21 +46,32 @@ struct AxisCoef {
     const float* b1;
 };
 
+// Problems of a batch lie back to back: velocity component c of problem b at
+// b * bv[c] elements from the first one, potential component c at b * bq[c].
 struct Proj3Args {
     float *qx, *qy, *qz, *xo, *yo, *zo;
     const float *xv, *yv, *zv;
     AxisCoef cx, cy, cz;
     int nx, ny, nz;
+    int zblocks;  // grid z rows per problem
+    int64_t bv[3], bq[3];
 };
 
 // One thread per (x, y) column walking ZB consecutive z cells; offsets in
-// index type I (32-bit whenever every array fits).
+// index type I (32-bit whenever every array fits).  Grid row blockIdx.z
+// covers z block (blockIdx.z % zblocks) of problem (blockIdx.z / zblocks).
 template <class I, int ZB>
 __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
-    const int z0 = blockIdx.z * ZB;
+    const int pb = blockIdx.z / p.zblocks;
+    const int z0 = (blockIdx.z - pb * p.zblocks) * ZB;
     if (y >= p.ny || x >= p.nx) return;
+    if (pb) {
+        p.xv += pb * p.bv[0]; p.yv += pb * p.bv[1]; p.zv += pb * p.bv[2];
+        p.qx += pb * p.bq[0]; p.qy += pb * p.bq[1]; p.qz += pb * p.bq[2];
+        if (p.xo) { p.xo += pb * p.bv[0]; p.yo += pb * p.bv[1]; p.zo += pb * p.bv[2]; }
+    }
     const I NY = p.ny, NX = p.nx;
     const I sxv_z = (NX + 1) * NY, syv_z = NX * (NY + 1), szv_z = NX * NY;
     const I sqx_x = NY + 1, sqx_z = NX * (NY + 1);
@@ -288,21 +299,38 @@ int project3d_create(int nx, int ny, int nz, double dx, double dy, double dz, in
     return WHHD_OK;
 }
 
+template <int ZB>
+static void launch_project3d(const Proj3Args& a, int nb, bool wide, hipStream_t st) {
+    dim3 block(64, 4, 1);
+    dim3 grid((a.ny + 63) / 64, (a.nx + 3) / 4, (unsigned)(a.zblocks * nb));
+    if (!wide) hipLaunchKernelGGL((project3d_kernel<int, ZB>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((project3d_kernel<int64_t, ZB>), grid, block, 0, st, a);
+}
+
 int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, float* xo, float* yo,
-                  float* zo, const float* xv, const float* yv, const float* zv, hipStream_t st) {
+                  float* zo, const float* xv, const float* yv, const float* zv, hipStream_t st,
+                  int nb, int zb) {
     if (!h || !qx || !qy || !qz || !xv || !yv || !zv)
         return fail(WHHD_INVALID_ARG, "project3d_run: null argument");
     if ((xo || yo || zo) && !(xo && yo && zo))
         return fail(WHHD_INVALID_ARG, "project3d_run: give all three or none of the v_out arrays");
-    Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz};
-    constexpr int ZB = WHHD_PROJ_ZB;
-    dim3 block(64, 4, 1);
-    dim3 grid((h->ny + 63) / 64, (h->nx + 3) / 4, (h->nz + ZB - 1) / ZB);
-    const int64_t biggest = (int64_t)(h->nx + 1) * (h->ny + 1) * (h->nz + 1);
-    if (biggest < ((int64_t)1 << 31))
-        hipLaunchKernelGGL((project3d_kernel<int, ZB>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((project3d_kernel<int64_t, ZB>), grid, block, 0, st, a);
+    if (zb <= 0) zb = WHHD_PROJ_ZB;
+    if (zb != 1 && zb != 2 && zb != 4 && zb != 8)
+        return fail(WHHD_INVALID_ARG, "project3d_run: z block must be 1, 2, 4 or 8");
+    const int64_t X = h->nx, Y = h->ny, Z = h->nz;
+    Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz,
+                (h->nz + zb - 1) / zb,
+                {(X + 1) * Y * Z, X * (Y + 1) * Z, X * Y * (Z + 1)},
+                {X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z}};
+    if (nb < 1 || (int64_t)a.zblocks * nb > 65535)
+        return fail(WHHD_INVALID_ARG, "project3d_run: batch too large for one launch");
+    const bool wide = (X + 1) * (Y + 1) * (Z + 1) >= ((int64_t)1 << 31);
+    switch (zb) {
+    case 1: launch_project3d<1>(a, nb, wide, st); break;
+    case 2: launch_project3d<2>(a, nb, wide, st); break;
+    case 4: launch_project3d<4>(a, nb, wide, st); break;
+    default: launch_project3d<8>(a, nb, wide, st); break;
+    }
     set_last_kernel("project3d_kernel");
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "project3d kernel");
@@ -371,7 +399,7 @@ WHHD_API int whhd_project3d_run(whhd_project3d_t h, float* qx, float* qy, float*
                                 float* xv_out, float* yv_out, float* zv_out, const float* xv,
                                 const float* yv, const float* zv, void* stream) {
     return whhd::project3d_run(h, qx, qy, qz, xv_out, yv_out, zv_out, xv, yv, zv,
-                               (hipStream_t)stream);
+                               (hipStream_t)stream, 1, 0);
 }
 
 WHHD_API void whhd_project3d_destroy(whhd_project3d_t h) {

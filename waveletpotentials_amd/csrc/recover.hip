@@ -21,7 +21,7 @@ struct whhd_project2d_s;
 namespace whhd {
 int project3d_create(int, int, int, double, double, double, int, char, whhd_project3d_s**);
 int project3d_run(const whhd_project3d_s*, float*, float*, float*, float*, float*, float*,
-                  const float*, const float*, const float*, hipStream_t);
+                  const float*, const float*, const float*, hipStream_t, int nb, int zb);
 int project2d_create(int, int, double, double, int, char, whhd_project2d_s**);
 int project2d_run(const whhd_project2d_s*, float*, const float*, const float*, hipStream_t);
 }  // namespace whhd
@@ -34,6 +34,9 @@ struct whhd_recover3d_s {
     int lx, ly, lz;
     char bc, bd;
     int wc, wd;
+    int nb = 1;        // problems per solve, arrays back to back
+    int lp_waves = 4;  // wavefronts per workgroup of the line passes
+    int proj_zb = 0;   // z cells per projection thread (0: default)
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
     float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
@@ -97,12 +100,14 @@ static void mark(StageTrace* tr, hipStream_t s, bool fwd, const char* arr, char 
 }
 static int three_pass(const float* in, float* scratch, float* out, int nx, int ny, int nz,
                       const int lv[3], Pass px, Pass py, Pass pz, bool fwd, hipStream_t s,
-                      StageTrace* tr, const char* arr) {
-    TRY(transform_lines<float>(out, in, nx, ny, nz, lv[0], 'x', px.bc, px.wt, fwd, s));
+                      const whhd_recover3d_s* h, const char* arr) {
+    StageTrace* tr = const_cast<StageTrace*>(&h->trace);
+    const LineBatch b{h->nb, (int64_t)nx * ny * nz, h->lp_waves};
+    TRY(transform_lines<float>(out, in, nx, ny, nz, lv[0], 'x', px.bc, px.wt, fwd, s, nullptr, &b));
     mark(tr, s, fwd, arr, 'x');
-    TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s));
+    TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s, nullptr, &b));
     mark(tr, s, fwd, arr, 'y');
-    TRY(transform_lines<float>(out, scratch, nx, ny, nz, lv[2], 'z', pz.bc, pz.wt, fwd, s));
+    TRY(transform_lines<float>(out, scratch, nx, ny, nz, lv[2], 'z', pz.bc, pz.wt, fwd, s, nullptr, &b));
     mark(tr, s, fwd, arr, 'z');
     return WHHD_OK;
 }
@@ -114,11 +119,10 @@ static int vel_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch,
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
-    StageTrace* tr = &h->trace;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s, tr, "xv");
-    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s, tr, "yv");
-    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s, tr, "zv");
+    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s, h, "xv");
+    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s, h, "yv");
+    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s, h, "zv");
     }
 }
 
@@ -129,20 +133,20 @@ static int pot_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch,
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
-    StageTrace* tr = &h->trace;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s, tr, "qx");
-    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s, tr, "qy");
-    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s, tr, "qz");
+    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s, h, "qx");
+    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s, h, "qy");
+    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s, h, "qz");
     }
 }
 
 static bool pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 
 int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc, int wc,
-                     int wd, whhd_recover3d_s** out) {
+                     int wd, whhd_recover3d_s** out, int nb) {
     if (!out) return fail(WHHD_INVALID_ARG, "recover3d_create: null handle pointer");
     *out = nullptr;
+    if (nb < 1 || nb > 4096) return fail(WHHD_INVALID_ARG, "recover3d_create: batch count out of range");
     if (!pow2(nx) || !pow2(ny) || !pow2(nz))
         return fail(WHHD_NOT_POW2, "recover3d: cell counts must be powers of two (>= 4)");
     const char bd = proj_bc(bc);
@@ -158,6 +162,7 @@ int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, ch
     h->nx = nx; h->ny = ny; h->nz = nz;
     h->lx = ilog2(nx); h->ly = ilog2(ny); h->lz = ilog2(nz);
     h->bc = bc; h->bd = bd; h->wc = wc; h->wd = wd;
+    h->nb = nb;
     int st = project3d_create(nx, ny, nz, dx, dy, dz, wd, bd, &h->proj);
     if (st != WHHD_OK) {
         delete h;
@@ -168,7 +173,7 @@ int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, ch
                           (size_t)(nx + 1) * ny * (nz + 1), (size_t)(nx + 1) * (ny + 1) * nz};
     float** slot[6] = {&h->tmp[0], &h->tmp[1], &h->tmp[2], &h->sq[0], &h->sq[1], &h->sq[2]};
     for (int i = 0; i < 6; ++i)
-        if (hipMalloc(slot[i], sz[i] * sizeof(float)) != hipSuccess) {
+        if (hipMalloc(slot[i], sz[i] * nb * sizeof(float)) != hipSuccess) {
             for (int j = 0; j < i; ++j) hipFree(*slot[j]);
             whhd_project3d_destroy(h->proj);
             delete h;
@@ -194,8 +199,12 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
     for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, v[c], S[c], t[c], true, s));
     // q coefficients land in the scratch; with v_out, s x q overwrites the
     // transformed velocity in place at the projected coefficients
-    if (vout) TRY(project3d_run(h->proj, S[0], S[1], S[2], t[0], t[1], t[2], t[0], t[1], t[2], s));
-    else TRY(project3d_run(h->proj, S[0], S[1], S[2], nullptr, nullptr, nullptr, t[0], t[1], t[2], s));
+    if (vout)
+        TRY(project3d_run(h->proj, S[0], S[1], S[2], t[0], t[1], t[2], t[0], t[1], t[2], s, h->nb,
+                          h->proj_zb));
+    else
+        TRY(project3d_run(h->proj, S[0], S[1], S[2], nullptr, nullptr, nullptr, t[0], t[1], t[2], s,
+                          h->nb, h->proj_zb));
     h->trace.mark(s, "project");
     for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], q[c], false, s));
     if (vout)
@@ -204,10 +213,22 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
     return WHHD_OK;
 }
 
+int recover3d_set_tiles(whhd_recover3d_s* h, int lp_waves, int proj_zb) {
+    if (!h) return fail(WHHD_INVALID_ARG, "recover3d_set_tiles: null handle");
+    if (lp_waves != 1 && lp_waves != 2 && lp_waves != 4)
+        return fail(WHHD_INVALID_ARG, "line-pass wavefronts per workgroup must be 1, 2 or 4");
+    if (proj_zb != 1 && proj_zb != 2 && proj_zb != 4 && proj_zb != 8)
+        return fail(WHHD_INVALID_ARG, "projection z block must be 1, 2, 4 or 8");
+    h->lp_waves = lp_waves;
+    h->proj_zb = proj_zb;
+    return WHHD_OK;
+}
+
 int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const float* qx,
                   const float* qy, const float* qz, int level, hipStream_t s) {
     if (!h || !ox || !oy || !oz || !qx || !qy || !qz)
         return fail(WHHD_INVALID_ARG, "recover3d_cut: null argument");
+    if (h->nb != 1) return fail(WHHD_INVALID_ARG, "recover3d_cut: one problem per handle");
     float* const* S = h->sq;
     const float* q[3] = {qx, qy, qz};
     float* o[3] = {ox, oy, oz};
@@ -280,7 +301,7 @@ extern "C" {
 
 WHHD_API int whhd_recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
                                    int wavelet_curl, int wavelet_div, whhd_recover3d_t* out) {
-    return whhd::recover3d_create(nx, ny, nz, dx, dy, dz, bc, wavelet_curl, wavelet_div, out);
+    return whhd::recover3d_create(nx, ny, nz, dx, dy, dz, bc, wavelet_curl, wavelet_div, out, 1);
 }
 
 WHHD_API int whhd_recover3d_solve(whhd_recover3d_t h, float* qx, float* qy, float* qz,

@@ -53,14 +53,34 @@ int check_wavelet(int wt, char bc);
 int check_line(int n, int levels, int wt);
 int ilog2(int n);
 
+// A batch of `nb` problems of equal extents whose arrays lie `stride` elements
+// apart, transformed by one launch; `waves` wavefronts per workgroup of the
+// line-parallel kernels (1, 2 or 4).
+struct LineBatch {
+    int nb = 1;
+    int64_t stride = 0;
+    int waves = 4;
+};
+
 template <class T>
 int transform_lines(T* dst, const T* src, int nx, int ny, int nz, int levels, char axis, char bc,
-                    int wt, bool fwd, hipStream_t stream, const TileConfig* cfg = nullptr);
+                    int wt, bool fwd, hipStream_t stream, const TileConfig* cfg = nullptr,
+                    const LineBatch* batch = nullptr);
 
 // Line-parallel transform (dst != src, curl CDF3.7 'n' / divergence CDF4.6 'd',
 // 2^5..2^8 samples); returns -1 when no such kernel applies.
 int lp_transform(float* dst, const float* src, int nx, int ny, int nz, int levels, char axis,
-                 char bc, int wt, bool fwd, hipStream_t stream);
+                 char bc, int wt, bool fwd, hipStream_t stream, const LineBatch& b);
+
+// Recovery driver over `nb` problems of equal extents stored back to back (the
+// arrays of problem b start b * (array size) elements after the first);
+// recover3d_set_tiles picks the launch shapes (1/2/4 wavefronts per line-pass
+// workgroup, 1/2/4/8 z cells per projection thread).
+int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc, int wc,
+                     int wd, whhd_recover3d_s** out, int nb);
+int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float* xo, float* yo,
+                    float* zo, const float* xv, const float* yv, const float* zv, hipStream_t s);
+int recover3d_set_tiles(whhd_recover3d_s* h, int lp_waves, int proj_zb);
 
 // Gram table (8 rows of n) of the synthesis basis of (wt, bc) into device memory.
 int gram_table_device(double* out, int wt, char bc, int n, hipStream_t stream);

@@ -12,7 +12,7 @@ inputs are the same on every machine.  Arrays use the (nz, nx, ny) layout.
 import numpy as np
 
 __all__ = ["seed_for", "spectral_potential3d", "curl3d", "curl2d", "div3d", "velocity3d",
-           "gaussian_vortex_stream2d", "velocity2d"]
+           "gaussian_vortex_stream2d", "velocity2d", "spheres", "BATCH_SIZES", "batch_problems"]
 
 
 def seed_for(config_id, problem_id=0):
@@ -129,3 +129,35 @@ def velocity2d(n, seed):
     dx = tuple(1.0 / v for v in n)
     psi = gaussian_vortex_stream2d(n, seed)
     return curl2d(psi, dx), psi, dx
+
+
+def spheres(count, seed):
+    """`count` spheres of the solid-obstacle workloads: centres U[0.2, 0.8]^3,
+    radii U[0.05, 0.15] (unit cube).  Returns (centres (count, 3), radii)."""
+    rng = _rng(seed)
+    return rng.uniform(0.2, 0.8, size=(count, 3)), rng.uniform(0.05, 0.15, size=count)
+
+
+# extents (nx, ny, nz) of the batched workload's problems
+BATCH_SIZES = ((32, 32, 32), (64, 64, 64), (128, 128, 128), (128, 64, 64))
+
+
+def batch_problems(count=64, seed=None):
+    """The batched workload: `count` problems with extents drawn uniformly from
+    BATCH_SIZES, each with a spectral velocity (modes |k| <= min(extent) / 8 per
+    box, |k|^(-11/6) amplitudes) and 0-4 spheres.  Cells are cubes of side
+    1 / max(extent), so the 128x64x64 grids cover a 1 x 0.5 x 0.5 box (the
+    multigrid's rediscretized coarse operator with piecewise-constant
+    prolongation diverges at a 2:1 cell aspect); sphere centres are drawn in
+    box-relative coordinates.  Returns a list of dicts: dims, dx, extent, kmax,
+    seed, centers, radii."""
+    rng = _rng(seed_for(4) if seed is None else seed)
+    out = []
+    for i in range(count):
+        d = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        h = 1.0 / max(d)
+        ext = tuple(v * h for v in d)
+        c, r = spheres(int(rng.integers(0, 5)), seed_for(4, 1000 + i))
+        out.append({"dims": d, "dx": (h, h, h), "extent": ext, "kmax": max(2, min(d) // 8),
+                    "seed": seed_for(4, i), "centers": c * np.array(ext), "radii": r})
+    return out
