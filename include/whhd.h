@@ -235,11 +235,26 @@ WHHD_API int whhd_mgfrac3d_create(const float* ax, const float* ay, const float*
 WHHD_API int whhd_mgfrac3d_levels(whhd_mgfrac3d_t h, int* dims, int max_levels);
 /* Solve(dst, pre_v, f, n_iters, nu1, nu2, weight) (WaveletPotentials/cuMultigridFrac3D.cu:
  * 249-327): n_iters V-cycles with nu1 / nu2 weighted-Jacobi sweeps before /
- * after the coarse correction, 50 ('n') or 20 sweeps on the coarsest level,
- * starting from pre_v (NULL: zero).  dst, pre_v, f are (nx, ny, nz) cell arrays;
+ * after the coarse correction, 50 ('n') or 20 sweeps on the coarsest level
+ * (in FP64), starting from pre_v (NULL: zero).  dst, pre_v, f are (nx, ny, nz) cell arrays;
  * dst may equal pre_v. */
 WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre_v, const float* f,
                                  int n_iters, int nu1, int nu2, float weight, void* stream);
+/* Solve to a relative residual (no reference counterpart: the reference calls
+ * Solve for a fixed cycle count, WaveletPotentials/cuMultigridFrac3D.cu:249-265,
+ * in its build precision).  Mixed-precision refinement: the iterate is the
+ * unevaluated sum dst + dst_lo of two FP32 arrays (an FP32 iterate alone has a
+ * relative-residual floor near 1e-6 on fine grids).  Each cycle forms
+ * r = f - A (dst + dst_lo) in FP64 with its FP64 norm, stops when
+ * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs one V-cycle
+ * (as Solve) on A e = r from e = 0 and adds e to the pair.  pre_v (NULL: zero;
+ * may equal dst) starts dst; dst_lo starts at zero.  Writes the V-cycles run
+ * and the last relative residual; returns WHHD_NONCONVERGED if tol was not
+ * reached.  Synchronizes `stream` once per cycle (16-byte read-back). */
+WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
+                                     const float* pre_v, const float* f, int max_cycles,
+                                     double tol, int nu1, int nu2, float weight, int* cycles,
+                                     double* rel, void* stream);
 /* ||f - A v||_2 / ||f||_2 with the sums in FP64 (CuWeightedJacobi3D::GetFracRhs,
  * WaveletPotentials/cuWeightedJacobi3D.cu:1384-1407).  Synchronizes `stream`. */
 WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f, double* rel,

@@ -11,6 +11,8 @@ import numpy as np
 import pytest
 import torch
 
+from waveletpotentials_amd import inputs as I
+
 pytestmark = pytest.mark.gpu
 
 
@@ -77,3 +79,37 @@ def test_warm_start_and_convergence(whhd, orc):
         res.append(mg.GetResdualError(v, df))
     assert all(b < a for a, b in zip(res, res[1:]))
     assert res[-1] < 1e-3
+
+
+def test_solve_to_tolerance_passes_the_fp32_floor(whhd, orc):
+    """A 128x64x64 grid with one sphere: plain FP32 V-cycles stall above 1e-6
+    (rounding even the converged FP64 solution to FP32 leaves ~1.5e-6), the
+    mixed-precision refinement reaches 1e-8, and the oracle's FP64 residual of
+    the returned pair agrees with the one the solver reports."""
+    p = I.batch_problems(64)[1]
+    n, h = p["dims"], p["dx"]
+    ls = orc.levelset_spheres(n, h, p["centers"], p["radii"])
+    fr = orc.frac_from_ls(ls, h)
+    q = I.spectral_potential3d(n, p["kmax"], p["seed"], p["extent"])
+    u = I.curl3d(*q, h)
+    au = [a * b for a, b in zip(fr, u)]
+    f = -((au[0][:, 1:, :] - au[0][:, :-1, :]) / h[0] + (au[1][:, :, 1:] - au[1][:, :, :-1]) / h[1]
+          + (au[2][1:] - au[2][:-1]) / h[2])
+    mg = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    df = cuda(f)
+    v = torch.zeros_like(df)
+    mg.Solve(v, v, df, 40, 2, 2, 0.8)
+    plain = mg.GetResdualError(v, df)
+    hi, lo = torch.empty_like(df), torch.empty_like(df)
+    cycles, res, ok = mg.SolveToTolerance(hi, lo, None, df, 1e-8, 100)
+    assert ok and res <= 1e-8
+    assert plain > 1e-6
+    torch.cuda.synchronize()
+    pair = hi.double().cpu().numpy() + lo.double().cpu().numpy()
+    fr32 = [a.astype(np.float32).astype(np.float64) for a in fr]  # the operator the GPU holds
+    r_orc = orc.mg_rel_residual(pair, df.double().cpu().numpy(), *fr32, h, "n")
+    assert abs(r_orc - res) <= 0.05 * res + 1e-12
+    # a warm start from the answer's high part (at the FP32 floor) needs no
+    # cycle for a tolerance above that floor
+    c2, res2, ok2 = mg.SolveToTolerance(hi, lo, hi, df, 1e-5, 10)
+    assert ok2 and c2 == 0 and res2 < 1e-5

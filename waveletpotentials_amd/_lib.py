@@ -60,6 +60,8 @@ SIGNATURES = {
     "whhd_mgfrac3d_create": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, _D, _C, ctypes.POINTER(_P)]),
     "whhd_mgfrac3d_levels": (_I, [_P, _P, _I]),
     "whhd_mgfrac3d_solve": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.c_float, _P]),
+    "whhd_mgfrac3d_solve_tol": (_I, [_P, _P, _P, _P, _P, _I, _D, _I, _I, ctypes.c_float, _P, _P,
+                                     _P]),
     "whhd_mgfrac3d_residual": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "whhd_mgfrac3d_destroy": (None, [_P]),
     "whhd_gradient3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),

@@ -490,6 +490,22 @@ class CuMultigridFrac3D(_Handle):
                                         _ptr(f, numel=self._n, name="f"), int(n_iters), int(nu1),
                                         int(nu2), float(weight), _stream()), "Solve")
 
+    def SolveToTolerance(self, dst, dst_lo, pre_v, f, tol=1e-6, max_cycles=100, nu1=2, nu2=2,
+                         weight=0.8):
+        """V-cycles until ||f - A (dst + dst_lo)|| <= tol ||f|| (mixed-precision
+        refinement; no reference counterpart).  Returns (cycles, residual,
+        converged)."""
+        pv = None if pre_v is None else _ptr(pre_v, numel=self._n, name="pre_v")
+        cyc, rel = ctypes.c_int(0), ctypes.c_double(0.0)
+        st = lib().whhd_mgfrac3d_solve_tol(self._h, _ptr(dst, numel=self._n, name="dst"),
+                                           _ptr(dst_lo, numel=self._n, name="dst_lo"), pv,
+                                           _ptr(f, numel=self._n, name="f"), int(max_cycles),
+                                           float(tol), int(nu1), int(nu2), float(weight),
+                                           ctypes.byref(cyc), ctypes.byref(rel), _stream())
+        if st not in (0, 6):
+            check(st, "SolveToTolerance")
+        return cyc.value, rel.value, st == 0
+
     def GetResdualError(self, dst, f):
         """||f - A dst|| / ||f|| in FP64 (the reference's GetResdualError / GetFracRhs)."""
         rel = ctypes.c_double(0.0)

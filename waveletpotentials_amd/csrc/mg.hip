@@ -29,8 +29,9 @@ int half_up(int n) { return (n & 1) ? (n >> 1) + 1 : n >> 1; }
 dim3 grid_of(int nx, int ny, int nz) { return dim3((ny + 63) / 64, (nx + 3) / 4, nz); }
 
 // Coefficient of one face, cuWjFrac3D_n / _z: A * 2 / ((h_nbr + h) * h).
-__device__ __forceinline__ float face_cf(float a, float hn, float h) {
-    return a / (hn + h) / h * 2.f;
+template <class T>
+__device__ __forceinline__ T face_cf(T a, T hn, T h) {
+    return a / (hn + h) / h * T(2);
 }
 
 struct Level {
@@ -46,47 +47,48 @@ struct LevelArgs {
     float ex, ey, ez;  // exterior spacing at the walls ('z')
 };
 
-// Diagonal and off-diagonal sum of cell (x, y, z) applied to src.  Z: walls
-// carry a Dirichlet coefficient with the exterior spacing; otherwise closed.
-template <bool Z>
-__device__ __forceinline__ void stencil(const LevelArgs& a, const float* __restrict__ src, int x,
-                                        int y, int z, int64_t i, float& diag, float& off) {
+// Diagonal and off-diagonal sum of cell (x, y, z) applied to the iterate,
+// read through src(j), in arithmetic T.  Z: walls carry a Dirichlet
+// coefficient with the exterior spacing; otherwise closed.
+template <bool Z, class T, class Src>
+__device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int y, int z, int64_t i,
+                                        T& diag, T& off) {
     const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
-    const float hx = a.hx[x], hy = a.hy[y], hz = a.hz[z];
-    diag = 0.f;
-    off = 0.f;
+    const T hx = a.hx[x], hy = a.hy[y], hz = a.hz[z];
+    diag = T(0);
+    off = T(0);
     // y faces: ay index (y, x, z) with extents (nx, ny+1, nz)
     const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
     if (y > 0 || Z) {
-        const float cf = face_cf(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, hy);
+        const T cf = face_cf<T>(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, hy);
         diag += cf;
-        if (y > 0) off -= cf * src[i - 1];
+        if (y > 0) off -= cf * src(i - 1);
     }
     if (y < a.ny - 1 || Z) {
-        const float cf = face_cf(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, hy);
+        const T cf = face_cf<T>(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, hy);
         diag += cf;
-        if (y < a.ny - 1) off -= cf * src[i + 1];
+        if (y < a.ny - 1) off -= cf * src(i + 1);
     }
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
     if (x > 0 || Z) {
-        const float cf = face_cf(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, hx);
+        const T cf = face_cf<T>(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, hx);
         diag += cf;
-        if (x > 0) off -= cf * src[i - sx];
+        if (x > 0) off -= cf * src(i - sx);
     }
     if (x < a.nx - 1 || Z) {
-        const float cf = face_cf(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, hx);
+        const T cf = face_cf<T>(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, hx);
         diag += cf;
-        if (x < a.nx - 1) off -= cf * src[i + sx];
+        if (x < a.nx - 1) off -= cf * src(i + sx);
     }
     if (z > 0 || Z) {
-        const float cf = face_cf(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, hz);
+        const T cf = face_cf<T>(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, hz);
         diag += cf;
-        if (z > 0) off -= cf * src[i - sz];
+        if (z > 0) off -= cf * src(i - sz);
     }
     if (z < a.nz - 1 || Z) {
-        const float cf = face_cf(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, hz);
+        const T cf = face_cf<T>(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, hz);
         diag += cf;
-        if (z < a.nz - 1) off -= cf * src[i + sz];
+        if (z < a.nz - 1) off -= cf * src(i + sz);
     }
 }
 
@@ -101,8 +103,91 @@ __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z>(a, src, x, y, z, i, diag, off);
+    stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
     dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
+}
+
+// The coarsest level's sweeps in FP64 (the reference solves it with one block,
+// cuWeightedJacobi3D.cu:1473-1722): one workgroup, FP64 ping-pong in `buf`
+// (2 * cells doubles), starting from and ending in v.
+template <bool Z>
+__global__ void __launch_bounds__(256) coarsest_kernel(float* __restrict__ v,
+                                                       const float* __restrict__ f, LevelArgs a,
+                                                       double* __restrict__ buf, int sweeps,
+                                                       double w) {
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    double* A = buf;
+    double* B = buf + n;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) A[i] = v[i];
+    __syncthreads();
+    for (int it = 0; it < sweeps; ++it) {
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int y = (int)(i % a.ny);
+            const int x = (int)((i / a.ny) % a.nx);
+            const int z = (int)(i / ((int64_t)a.ny * a.nx));
+            double diag, off;
+            stencil<Z, double>(a, [&](int64_t j) { return A[j]; }, x, y, z, i, diag, off);
+            B[i] = diag > 1e-12 ? (1.0 - w) * A[i] + w * ((double)f[i] - off) / diag : 0.0;
+        }
+        __syncthreads();
+        double* t = A;
+        A = B;
+        B = t;
+    }
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = (float)A[i];
+}
+
+// Residual of the two-float iterate hi + lo in FP64, r = f - A (hi + lo),
+// stored as FP32, with per-workgroup FP64 sums of r^2 and f^2 in part.
+template <bool Z>
+__global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
+                                                         const float* __restrict__ hi,
+                                                         const float* __restrict__ lo,
+                                                         const float* __restrict__ f, LevelArgs a,
+                                                         double* __restrict__ part) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    double sr = 0.0, sf = 0.0;
+    if (y < a.ny && x < a.nx) {
+        const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+        auto val = [&](int64_t j) { return (double)hi[j] + (double)lo[j]; };
+        double diag, off;
+        stencil<Z, double>(a, val, x, y, z, i, diag, off);
+        const double fi = f[i];
+        const double ri = diag > 1e-12 ? fi - (diag * val(i) + off) : 0.0;
+        r[i] = (float)ri;
+        sr = ri * ri;
+        sf = fi * fi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        sf += __shfl_xor(sf, o);
+    }
+    __shared__ double lr[4], lf[4];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    if ((t & 63) == 0) {
+        lr[t >> 6] = sr;
+        lf[t >> 6] = sf;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        part[2 * b] = lr[0] + lr[1] + lr[2] + lr[3];
+        part[2 * b + 1] = lf[0] + lf[1] + lf[2] + lf[3];
+    }
+}
+
+// hi + lo <- hi + lo + e as an unevaluated pair (TwoSum of hi and lo + e).
+__global__ void pair_add_kernel(float* __restrict__ hi, float* __restrict__ lo,
+                                const float* __restrict__ e, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = hi[i], t = lo[i] + e[i];
+    const float s = a + t;
+    const float bb = s - a;
+    hi[i] = s;
+    lo[i] = (a - (s - bb)) + (t - bb);
 }
 
 // Residual f - A v (cuWjGetFracRes_3d_n / _z).
@@ -115,7 +200,7 @@ __global__ void residual_kernel(float* __restrict__ r, const float* __restrict__
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z>(a, v, x, y, z, i, diag, off);
+    stencil<Z, float>(a, [&](int64_t j) { return v[j]; }, x, y, z, i, diag, off);
     r[i] = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
 }
 
@@ -284,6 +369,9 @@ struct whhd_mgfrac3d_s {
     float* r = nullptr;       // fine-level residual
     double* part = nullptr;   // norm partial sums (2 * kNormBlocks) + the two totals
     double* host = nullptr;   // pinned host copy of the totals
+    double* cbuf = nullptr;   // FP64 ping-pong of the coarsest level (2 * its cells)
+    double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
+    int64_t rblocks = 0;
     std::vector<void*> owned;
 };
 
@@ -358,9 +446,18 @@ int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t
 // The reference's coarsest level runs 50 ('n') or 20 ('z') sweeps of the
 // volume-scaled system in one block (cuWeightedJacobi3D.cu:1473-1722); scaling
 // rows by the cell volume leaves the Jacobi iteration unchanged, so the same
-// sweeps run here with the level's own kernel.
+// sweeps run here, in FP64, by one workgroup.
 int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
-    return smooth(h, L, h->bc == 'z' ? 20 : 50, w, s);
+    const LevelArgs a = args_of(h, L);
+    const int sweeps = h->bc == 'z' ? 20 : 50;
+    if (h->bc == 'z')
+        hipLaunchKernelGGL(coarsest_kernel<true>, dim3(1), dim3(256), 0, s, L.v, L.f, a, h->cbuf,
+                           sweeps, (double)w);
+    else
+        hipLaunchKernelGGL(coarsest_kernel<false>, dim3(1), dim3(256), 0, s, L.v, L.f, a, h->cbuf,
+                           sweeps, (double)w);
+    hipError_t e = launch_status("mg_coarsest_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg coarsest kernel");
 }
 
 int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s) {
@@ -474,9 +571,14 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         min_dim = std::min(d.x, std::min(d.y, d.z));
     }
     if (st == WHHD_OK) {
+        const dim3 g = grid_of(nx, ny, nz);
+        h->rblocks = (int64_t)g.x * g.y * g.z;
         h->r = (float*)alloc((size_t)cells(h->dim) * 4);
         h->part = (double*)alloc((2 * kNormBlocks + 2) * sizeof(double));
-        if (!h->r || !h->part) st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+        h->cbuf = (double*)alloc((size_t)cells(h->lv.back().d) * 2 * sizeof(double));
+        h->rpart = (double*)alloc((size_t)(2 * h->rblocks + 2) * sizeof(double));
+        if (!h->r || !h->part || !h->cbuf || !h->rpart)
+            st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
     }
     if (st == WHHD_OK && hipHostMalloc((void**)&h->host, 2 * sizeof(double)) != hipSuccess) {
         h->host = nullptr;
@@ -507,6 +609,56 @@ int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const flo
     TRY(copy(L0.f, f, n, s));
     for (int i = 0; i < n_iters; ++i) TRY(vcycle(h, 0, nu1, nu2, w, s));
     return copy(dst, L0.v, n, s);
+}
+
+// Mixed-precision refinement: the iterate is the pair dst + lo of FP32 arrays.
+// Each cycle forms r = f - A (dst + lo) in FP64 (into the finest level's rhs)
+// with its FP64 norm, stops once ||r|| <= tol ||f||, else runs one V-cycle on
+// A e = r from e = 0 and adds e to the pair.
+int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* pre_v,
+                       const float* f, int max_cycles, double tol, int nu1, int nu2, float w,
+                       int* cycles, double* rel, hipStream_t s) {
+    if (!h || !dst || !lo || !f) return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: null argument");
+    if (dst == lo || lo == f || dst == f)
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: dst, dst_lo and f must be distinct");
+    if (max_cycles < 0 || nu1 < 0 || nu2 < 0 || !(tol >= 0))
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: bad iteration limits");
+    Level& L0 = h->lv[0];
+    const int64_t n = cells(h->dim);
+    if (pre_v && pre_v != dst) TRY(copy(dst, pre_v, n, s));
+    else if (!pre_v) TRY(fill(dst, 0.f, n, s));
+    TRY(fill(lo, 0.f, n, s));
+    const LevelArgs a = args_of(h, L0);
+    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    double res = 0.0;
+    int c = 0;
+    for (;; ++c) {
+        if (h->bc == 'z')
+            hipLaunchKernelGGL(residual64_kernel<true>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
+                               h->rpart);
+        else
+            hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
+                               h->rpart);
+        hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, h->rpart + 2 * h->rblocks,
+                           h->rpart, (int)h->rblocks);
+        hipError_t e = launch_status("mg_residual64_kernel");
+        if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
+        e = hipMemcpyAsync(h->host, h->rpart + 2 * h->rblocks, 2 * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "mgfrac3d_solve_tol: read-back");
+        res = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
+        if (res <= tol || c >= max_cycles) break;
+        TRY(fill(L0.v, 0.f, n, s));
+        TRY(vcycle(h, 0, nu1, nu2, w, s));
+        hipLaunchKernelGGL(pair_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst,
+                           lo, L0.v, n);
+        e = launch_status("mg_pair_add_kernel");
+        if (e != hipSuccess) return hip_fail(e, "mg pair update kernel");
+    }
+    if (cycles) *cycles = c;
+    if (rel) *rel = res;
+    return res <= tol ? WHHD_OK : WHHD_NONCONVERGED;
 }
 
 int mgfrac3d_residual(whhd_mgfrac3d_s* h, const float* v, const float* f, double* rel,
@@ -556,6 +708,14 @@ WHHD_API int whhd_mgfrac3d_levels(whhd_mgfrac3d_t h, int* dims, int max_levels) 
 WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre_v, const float* f,
                                  int n_iters, int nu1, int nu2, float weight, void* stream) {
     return whhd::mgfrac3d_solve(h, dst, pre_v, f, n_iters, nu1, nu2, weight, (hipStream_t)stream);
+}
+
+WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
+                                     const float* pre_v, const float* f, int max_cycles,
+                                     double tol, int nu1, int nu2, float weight, int* cycles,
+                                     double* rel, void* stream) {
+    return whhd::mgfrac3d_solve_tol(h, dst, dst_lo, pre_v, f, max_cycles, tol, nu1, nu2, weight,
+                                    cycles, rel, (hipStream_t)stream);
 }
 
 WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f,

@@ -26,14 +26,11 @@ def face_fractions(dims, dx, centers, radii):
     return (ax, ay, az), ls.ls
 
 
-def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-5, max_cycles=60):
+def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-6, max_cycles=80):
     """In place: u <- a (u - grad p) with p from the fractional multigrid solve of
     -div(a grad p) = -div(a u), V-cycles (2 pre / 2 post weighted-Jacobi sweeps,
-    w = 0.8) until the FP64 relative residual is <= tol.  Returns
-    (cycles, residual).  An FP32 iterate has a residual floor: rounding the
-    converged FP64 solution of a 128x64x64 problem with one sphere to FP32
-    already leaves a relative residual of 1.5e-6, growing about as the square of
-    the cell count per axis."""
+    w = 0.8) until the FP64 relative residual of the two-float iterate is
+    <= tol.  Returns (cycles, residual)."""
     if len(radii) == 0:
         return 0, 0.0
     (ax, ay, az), _ = face_fractions(dims, dx, centers, radii)
@@ -41,13 +38,14 @@ def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-5, max_cycles=60
     b = torch.empty((nz, nx, ny), device="cuda")
     api.CuConvergence.GetInstance().GetB_3d_frac_vel(b, xv, yv, zv, ax, ay, az, dims, dx, "n")
     mg = api.CuMultigridFrac3D(ax, ay, az, dims, dx, "n")
-    p = torch.zeros_like(b)
-    res, cycles = float("inf"), 0
-    while cycles < max_cycles:
-        mg.Solve(p, p, b, 1, 2, 2, 0.8)
-        cycles += 1
-        res = mg.GetResdualError(p, b)
-        if res <= tol:
-            break
-    api.CuGradient.GetInstance().SubtractGradient3D_Frac_P(xv, yv, zv, p, ax, ay, az, dims, dx, "n")
+    p, p_lo = torch.empty_like(b), torch.empty_like(b)
+    cycles, res, _ = mg.SolveToTolerance(p, p_lo, None, b, tol, max_cycles)
+    grad = api.CuGradient.GetInstance()
+    # u <- a (u - grad p) for the high part, then u <- u - a grad p_lo
+    grad.SubtractGradient3D_Frac_P(xv, yv, zv, p, ax, ay, az, dims, dx, "n")
+    gx, gy, gz = torch.empty_like(xv), torch.empty_like(yv), torch.empty_like(zv)
+    grad.Gradient3D_Frac_P(gx, gy, gz, p_lo, ax, ay, az, dims, dx, "n")
+    xv.sub_(ax * gx)
+    yv.sub_(ay * gy)
+    zv.sub_(az * gz)
     return cycles, res
