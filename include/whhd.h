@@ -261,6 +261,35 @@ WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const flo
                                     void* stream);
 WHHD_API void whhd_mgfrac3d_destroy(whhd_mgfrac3d_t h);
 
+/* ---- learned initial guess ---------------------------------------------- */
+typedef struct whhd_unet_s* whhd_unet_t;
+/* 3-level 3D U-Net (16/32/64 channels, 3x3x3 convolutions on the BF16 matrix
+ * cores with FP32 accumulation, ReLU, 2x2x2 average pooling, nearest
+ * upsampling, concatenated skips) predicting the fractional multigrid's
+ * solution from (f, solid level set); its output is the pre_v of
+ * whhd_mgfrac3d_solve / _solve_tol.  No reference counterpart: the reference's
+ * multigrid takes any initial guess as pre_v (WaveletPotentials/
+ * cuMultigridFrac3D.cu:249-258).  Extents must be multiples of 4; weights are
+ * He-normal from `seed`.  Holds activations for one nx*ny*nz grid. */
+WHHD_API int whhd_unet_create(int nx, int ny, int nz, unsigned long long seed, whhd_unet_t* out);
+WHHD_API long long whhd_unet_num_params(whhd_unet_t h);
+/* Copy the FP32 master weights (which = 0) or the last gradients (1) to host dst. */
+WHHD_API int whhd_unet_get_params(whhd_unet_t h, float* dst, int which);
+/* Load FP32 master weights from host memory; resets the Adam moments. */
+WHHD_API int whhd_unet_set_params(whhd_unet_t h, const float* src);
+/* p0 = prediction for the cell arrays f (right-hand side) and ls (level set);
+ * `length` is the domain's x extent (inputs are scaled by rms(f) and length). */
+WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const float* ls,
+                               double length, void* stream);
+/* Forward and backward for the target p: loss = mean((e - mean e)^2) with
+ * e = prediction - p in the network's scaled units; gradients kept in the
+ * handle.  Writes the loss when `loss` is not NULL (synchronizes `stream`). */
+WHHD_API int whhd_unet_loss_grad(whhd_unet_t h, const float* f, const float* ls, const float* p,
+                                 double length, double* loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999) with the kept gradients. */
+WHHD_API int whhd_unet_adam(whhd_unet_t h, float lr, void* stream);
+WHHD_API void whhd_unet_destroy(whhd_unet_t h);
+
 /* ---- field operators and reductions ------------------------------------ */
 /* CuGradient::Gradient3D_Frac_P(xv, yv, zv, p, ax, ay, az, dim, dx, dy, dz, dx_e, bc)
  * (WaveletPotentials/cuGradient.cu:1841-1951, 2514-2545) for uniform spacing:

@@ -70,6 +70,9 @@ def _setup(L):
     L.orc_sweep3d.argtypes = [_D] * 5 + [i, i, i, d, d, d]
     L.orc_dst3d_solve.argtypes = [_D, _D, i, i, i, d, d, d]
     L.orc_project_zero.argtypes = [_D] * 3 + [i, i, i, d, d, d]
+    L.orc_conv3.argtypes = [_D, _D, i, _D, i, i, i, i, i, _D, _D, i]
+    L.orc_conv3_bwd_data.argtypes = [_D, _D, i, i, i, i, _D, i]
+    L.orc_conv3_bwd_weight.argtypes = [_D, _D, _D, i, _D, i, _D, i, i, i, i, i]
     L.orc_curl3d.restype = None
     L.orc_curl2d.restype = None
     L.orc_div3d.restype = None
@@ -346,3 +349,176 @@ def project_zero(qx, qy, qz, n, dx):
     _check(lib().orc_project_zero(_p(q[0]), _p(q[1]), _p(q[2]), nx, ny, nz, *map(float, dx)),
            "project_zero")
     return tuple(q)
+
+
+# ---- learned initial guess: the 3-level U-Net (unet_oracle.c + composition) ----
+# (channels from a, channels from b, a upsampled, output channels, level); the
+# last layer has 16 output rows of which only row 0 is nonzero.
+UNET_LAYERS = [(16, 0, False, 16, 0), (16, 0, False, 16, 0), (16, 0, False, 32, 1),
+               (32, 0, False, 32, 1), (32, 0, False, 64, 2), (64, 0, False, 64, 2),
+               (64, 32, True, 32, 1), (32, 0, False, 32, 1), (32, 16, True, 16, 0),
+               (16, 0, False, 16, 0), (16, 0, False, 16, 0)]
+
+
+def unet_layout():
+    """[(weight offset, bias offset)] per layer in the flat parameter vector
+    (weights [co][27][cin] then biases [co]) and its length."""
+    off, out = 0, []
+    for ca, cb, _, co, _ in UNET_LAYERS:
+        wo = off
+        off += co * 27 * (ca + cb)
+        out.append((wo, off))
+        off += co
+    return out, off
+
+
+def bf16_round(a):
+    """Round to the nearest BF16 value (ties to even), via FP32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64).reshape(np.shape(a))
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def conv3(a, ca, b, cb, up, dims, w, bias, co):
+    nz, nx, ny = dims
+    y = np.empty((nz * nx * ny, co))
+    a, b, w, bias = _c(a), _c(b), _c(w), _c(bias)
+    _check(lib().orc_conv3(_p(y), _p(a), ca, _p(b), cb, int(up), nz, nx, ny, _p(w), _p(bias), co),
+           "conv3")
+    return y
+
+
+def conv3_bwd_data(g, co, dims, w, cin):
+    nz, nx, ny = dims
+    dx = np.empty((nz * nx * ny, cin))
+    g, w = _c(g), _c(w)
+    _check(lib().orc_conv3_bwd_data(_p(dx), _p(g), co, nz, nx, ny, _p(w), cin), "conv3_bwd_data")
+    return dx
+
+
+def conv3_bwd_weight(g, co, a, ca, b, cb, up, dims):
+    nz, nx, ny = dims
+    dw = np.empty(co * 27 * (ca + cb))
+    db = np.empty(co)
+    g, a, b = _c(g), _c(a), _c(b)
+    _check(lib().orc_conv3_bwd_weight(_p(dw), _p(db), _p(g), co, _p(a), ca, _p(b), cb, int(up),
+                                      nz, nx, ny), "conv3_bwd_weight")
+    return dw, db
+
+
+def _pool(x, dims, C, mean=True):
+    nz, nx, ny = dims
+    t = x.reshape(nz // 2, 2, nx // 2, 2, ny // 2, 2, C).sum(axis=(1, 3, 5))
+    return (t / 8.0 if mean else t).reshape(-1, C)
+
+
+def _unpool(xc, dims, C):
+    nz, nx, ny = dims
+    t = xc.reshape(nz // 2, nx // 2, ny // 2, C)
+    return np.repeat(np.repeat(np.repeat(t, 2, 0), 2, 1), 2, 2).reshape(-1, C)
+
+
+def unet_forward(params, f, ls, length, bf16=False):
+    """Network output (scaled units, one value per cell) and the activations.
+    f, ls: cell arrays (nz, nx, ny).  With bf16, activations are rounded to
+    BF16 where the GPU stores them."""
+    R = bf16_round if bf16 else (lambda a: a)
+    lay, _ = unet_layout()
+    nz, nx, ny = f.shape
+    d0, d1, d2 = (nz, nx, ny), (nz // 2, nx // 2, ny // 2), (nz // 4, nx // 4, ny // 4)
+    ms = float(np.mean(np.asarray(f, dtype=np.float64) ** 2))
+    x = np.zeros((nz * nx * ny, 16))
+    x[:, 0] = np.ravel(f) / np.sqrt(ms)
+    x[:, 1] = np.ravel(ls) / length
+    A = {"x": R(x)}
+
+    def layer(l, a, b, dims, relu=True):
+        ca, cb, up, co, _ = UNET_LAYERS[l]
+        wo, bo = lay[l]
+        y = conv3(a, ca, b, cb, up, dims, params[wo:bo], params[bo:bo + co], co)
+        return R(np.maximum(y, 0.0)) if relu else y
+
+    A["e0a"] = layer(0, A["x"], None, d0)
+    A["e0"] = layer(1, A["e0a"], None, d0)
+    A["p0"] = R(_pool(A["e0"], d0, 16))
+    A["e1a"] = layer(2, A["p0"], None, d1)
+    A["e1"] = layer(3, A["e1a"], None, d1)
+    A["p1"] = R(_pool(A["e1"], d1, 32))
+    A["ba"] = layer(4, A["p1"], None, d2)
+    A["b"] = layer(5, A["ba"], None, d2)
+    A["d1a"] = layer(6, A["b"], A["e1"], d1)
+    A["d1"] = layer(7, A["d1a"], None, d1)
+    A["d0a"] = layer(8, A["d1"], A["e0"], d0)
+    A["d0"] = layer(9, A["d0a"], None, d0)
+    out = layer(10, A["d0"], None, d0, relu=False)[:, 0]
+    return out, A, ms
+
+
+def unet_predict(params, f, ls, length, bf16=False):
+    """The initial guess p0 = output * rms(f) * length^2, shaped like f."""
+    out, _, ms = unet_forward(params, f, ls, length, bf16)
+    return (out * np.sqrt(ms) * length ** 2).reshape(f.shape)
+
+
+def unet_loss_grad(params, f, ls, p, length, bf16=False):
+    """loss = mean((e - mean e)^2), e = output - p / (rms(f) length^2), and
+    its gradient with respect to the flat parameters."""
+    R = bf16_round if bf16 else (lambda a: a)
+    lay, n = unet_layout()
+    out, A, ms = unet_forward(params, f, ls, length, bf16)
+    nz, nx, ny = f.shape
+    d0, d1, d2 = (nz, nx, ny), (nz // 2, nx // 2, ny // 2), (nz // 4, nx // 4, ny // 4)
+    V0 = out.size
+    e = out - np.ravel(p) / (np.sqrt(ms) * length ** 2)
+    eb = float(e.mean())
+    loss = float(np.mean(e * e) - eb * eb)
+    g = np.zeros((V0, 16))
+    g[:, 0] = 2.0 * (e - eb) / V0
+    g = R(g)
+    grads = np.zeros(n)
+
+    def wgrad(l, G, a, b, dims):
+        ca, cb, up, co, _ = UNET_LAYERS[l]
+        wo, bo = lay[l]
+        grads[wo:bo], grads[bo:bo + co] = conv3_bwd_weight(G, co, a, ca, b, cb, up, dims)
+
+    def bdata(l, G, dims):
+        ca, cb, _, co, _ = UNET_LAYERS[l]
+        wo, bo = lay[l]
+        return R(conv3_bwd_data(G, co, dims, params[wo:bo], ca + cb))
+
+    def mask(G, Y):
+        return G * (Y > 0)
+
+    wgrad(10, g, A["d0"], None, d0)
+    g0b = mask(bdata(10, g, d0), A["d0"])
+    wgrad(9, g0b, A["d0a"], None, d0)
+    g0a = mask(bdata(9, g0b, d0), A["d0a"])
+    wgrad(8, g0a, A["d1"], A["e0"], d0)
+    t = bdata(8, g0a, d0)
+    u0, ge0 = t[:, :32], t[:, 32:]
+    g1a = mask(R(_pool(u0, d0, 32, mean=False)), A["d1"])
+    wgrad(7, g1a, A["d1a"], None, d1)
+    g1b = mask(bdata(7, g1a, d1), A["d1a"])
+    wgrad(6, g1b, A["b"], A["e1"], d1)
+    t = bdata(6, g1b, d1)
+    u1, ge1 = t[:, :64], t[:, 64:]
+    g2a = mask(R(_pool(u1, d1, 64, mean=False)), A["b"])
+    wgrad(5, g2a, A["ba"], None, d2)
+    g2b = mask(bdata(5, g2a, d2), A["ba"])
+    wgrad(4, g2b, A["p1"], None, d2)
+    g2c = bdata(4, g2b, d2)
+    ge1 = mask(R(ge1 + _unpool(g2c, d1, 32) / 8.0), A["e1"])
+    wgrad(3, ge1, A["e1a"], None, d1)
+    g1a = mask(bdata(3, ge1, d1), A["e1a"])
+    wgrad(2, g1a, A["p0"], None, d1)
+    g1c = bdata(2, g1a, d1)
+    ge0 = mask(R(ge0 + _unpool(g1c, d0, 16) / 8.0), A["e0"])
+    wgrad(1, ge0, A["e0a"], None, d0)
+    g0b = mask(bdata(1, ge0, d0), A["e0a"])
+    wgrad(0, g0b, A["x"], None, d0)
+    return loss, grads, out

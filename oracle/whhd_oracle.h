@@ -134,6 +134,17 @@ int orc_dst3d_solve(double* phi, const double* b, int n1, int n2, int n3, double
 /* CuParallelSweep3D::ProjectZero: q -= grad phi with (-L_h) phi = -div q at interior nodes. */
 int orc_project_zero(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
                      double dy, double dz);
+/* ---- U-Net convolutions (unet_oracle.c) ---- */
+/* 3x3x3 convolution, zero padding, channels-last; input = concat(a (ca ch, half
+ * resolution when up_a), b (cb ch)); w[co][27][ca+cb]; bias may be NULL. */
+int orc_conv3(double* y, const double* a, int ca, const double* b, int cb, int up_a, int nz,
+              int nx, int ny, const double* w, const double* bias, int co);
+/* Input gradient at full resolution (cin = ca + cb channels) from g[V][co]. */
+int orc_conv3_bwd_data(double* dx, const double* g, int co, int nz, int nx, int ny,
+                       const double* w, int cin);
+/* Weight and bias gradients (db may be NULL). */
+int orc_conv3_bwd_weight(double* dw, double* db, const double* g, int co, const double* a, int ca,
+                         const double* b, int cb, int up_a, int nz, int nx, int ny);
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,94 @@
+"""GPU parity of the learned initial guess's U-Net against the FP64 oracle.
+
+The GPU computes in BF16 with FP32 accumulation; the oracle runs the same
+network in FP64 with the GPU's BF16 weights and rounds activations and
+activation gradients to BF16 where the GPU stores them, so the remaining
+differences are FP32 accumulation order and the rounding flips it causes.
+Tolerances (relative L2): prediction 1e-2, loss 1e-2, each layer's weight and
+bias gradient 6e-2 -- BF16 rounding by itself moves the FP64 oracle's
+gradients by 1-2% per layer at 16^3, so the bar is a few times that noise.
+The last layer's bias gradient is zero in exact arithmetic (the loss removes
+the mean), so it is checked against the scale of that layer's weight gradient.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def effective(orc, params):
+    """The parameters the GPU computes with: BF16 weights, FP32 biases."""
+    p = params.astype(np.float64).copy()
+    lay, _ = orc.unet_layout()
+    for wo, bo in lay:
+        p[wo:bo] = orc.bf16_round(p[wo:bo])
+    return p
+
+
+def sample(dims, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(dims).astype(np.float32)
+    ls = rng.uniform(-0.1, 0.4, dims).astype(np.float32)
+    p = (rng.standard_normal(dims) * 1e-2).astype(np.float32)
+    return f, ls, p
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 20)])
+def test_prediction_matches_oracle(whhd, orc, dims):
+    nx, ny, nz = dims
+    net = whhd.UNet3D(dims, seed=11)
+    f, ls, _ = sample((nz, nx, ny), 1)
+    p0 = torch.empty((nz, nx, ny), device="cuda")
+    net.predict(p0, torch.from_numpy(f).cuda(), torch.from_numpy(ls).cuda(), 1.0)
+    torch.cuda.synchronize()
+    ref = orc.unet_predict(effective(orc, net.get_params()), f.astype(np.float64),
+                           ls.astype(np.float64), 1.0, bf16=True)
+    assert rel(p0.cpu().numpy(), ref) < 1e-2
+
+
+def test_gradients_match_oracle(whhd, orc):
+    dims = (16, 16, 16)
+    net = whhd.UNet3D(dims, seed=5)
+    f, ls, p = sample(dims, 2)
+    d = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    loss = net.loss_grad(d(f), d(ls), d(p), 1.0)
+    g = net.get_params(1).astype(np.float64)
+    rloss, rg, _ = orc.unet_loss_grad(effective(orc, net.get_params()), f.astype(np.float64),
+                                      ls.astype(np.float64), p.astype(np.float64), 1.0, bf16=True)
+    assert abs(loss - rloss) <= 1e-2 * rloss
+    lay, _ = orc.unet_layout()
+    for l, (wo, bo) in enumerate(lay):
+        co = orc.UNET_LAYERS[l][3]
+        assert rel(g[wo:bo], rg[wo:bo]) < 6e-2, f"layer {l} weights"
+        if l < len(lay) - 1:
+            assert rel(g[bo:bo + co], rg[bo:bo + co]) < 6e-2, f"layer {l} biases"
+        else:
+            assert np.abs(g[bo:bo + co]).max() <= 1e-2 * np.abs(g[wo:bo]).max()
+
+
+def test_adam_steps_reduce_loss_and_checkpoint_roundtrip(whhd, tmp_path):
+    dims = (16, 16, 16)
+    net = whhd.UNet3D(dims, seed=7)
+    f, ls, p = sample(dims, 3)
+    d = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    f, ls, p = d(f), d(ls), d(p)
+    first = net.loss_grad(f, ls, p, 1.0)
+    for _ in range(30):
+        net.loss_grad(f, ls, p, 1.0)
+        net.adam(1e-3)
+    last = net.loss_grad(f, ls, p, 1.0)
+    assert last < 0.5 * first
+    path = tmp_path / "unet.bin"
+    net.save(str(path))
+    other = whhd.UNet3D(dims, seed=99)
+    other.load(str(path))
+    assert np.array_equal(other.get_params(), net.get_params())
+    a, b = torch.empty_like(f), torch.empty_like(f)
+    net.predict(a, f, ls, 1.0)
+    other.predict(b, f, ls, 1.0)
+    assert torch.equal(a, b)

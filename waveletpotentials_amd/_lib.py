@@ -64,6 +64,14 @@ SIGNATURES = {
                                      _P]),
     "whhd_mgfrac3d_residual": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "whhd_mgfrac3d_destroy": (None, [_P]),
+    "whhd_unet_create": (_I, [_I, _I, _I, ctypes.c_ulonglong, ctypes.POINTER(_P)]),
+    "whhd_unet_num_params": (ctypes.c_longlong, [_P]),
+    "whhd_unet_get_params": (_I, [_P, _P, _I]),
+    "whhd_unet_set_params": (_I, [_P, _P]),
+    "whhd_unet_predict": (_I, [_P, _P, _P, _P, _D, _P]),
+    "whhd_unet_loss_grad": (_I, [_P, _P, _P, _P, _D, ctypes.POINTER(_D), _P]),
+    "whhd_unet_adam": (_I, [_P, ctypes.c_float, _P]),
+    "whhd_unet_destroy": (None, [_P]),
     "whhd_gradient3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
     "whhd_subtract_gradient_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
     "whhd_divergence3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _P]),

@@ -30,6 +30,7 @@ __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "st
            "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
            "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "CuBatchRecover3D",
+           "UNet3D",
            "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -513,6 +514,77 @@ class CuMultigridFrac3D(_Handle):
                                            _ptr(f, numel=self._n, name="f"), ctypes.byref(rel),
                                            _stream()), "GetResdualError")
         return rel.value
+
+
+class UNet3D(_Handle):
+    """Learned initial guess for CuMultigridFrac3D: a 3-level 3D U-Net (16/32/64
+    channels, BF16 matrix-core convolutions with FP32 accumulation) from (f,
+    solid level set) to the pressure; its prediction is the multigrid's pre_v.
+    No reference counterpart (the reference's Solve takes any pre_v,
+    WaveletPotentials/cuMultigridFrac3D.cu:249-258).  `dim` = (nx, ny, nz) cells,
+    multiples of 4; the activations it holds are sized for that grid."""
+    _destroy = "whhd_unet_destroy"
+    MAGIC = b"WHUN"
+
+    def __init__(self, dim, seed=0):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        check(lib().whhd_unet_create(*self.dim, int(seed), ctypes.byref(self._h)), "UNet3D")
+        self.n = int(self.dim[0] * self.dim[1] * self.dim[2])
+
+    def num_params(self):
+        return int(lib().whhd_unet_num_params(self._h))
+
+    def get_params(self, which=0):
+        """FP32 master weights (which=0) or the last gradients (which=1)."""
+        import numpy as np
+        out = np.empty(self.num_params(), dtype=np.float32)
+        check(lib().whhd_unet_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p), int(which)),
+              "UNet3D.get_params")
+        return out
+
+    def set_params(self, params):
+        import numpy as np
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        if a.size != self.num_params():
+            raise ValueError(f"expected {self.num_params()} parameters, got {a.size}")
+        check(lib().whhd_unet_set_params(self._h, a.ctypes.data_as(ctypes.c_void_p)),
+              "UNet3D.set_params")
+
+    def predict(self, p0, f, ls, length):
+        check(lib().whhd_unet_predict(self._h, _ptr(p0, numel=self.n, name="p0"),
+                                      _ptr(f, numel=self.n, name="f"),
+                                      _ptr(ls, numel=self.n, name="ls"), float(length), _stream()),
+              "UNet3D.predict")
+
+    def loss_grad(self, f, ls, p, length):
+        loss = ctypes.c_double(0.0)
+        check(lib().whhd_unet_loss_grad(self._h, _ptr(f, numel=self.n, name="f"),
+                                        _ptr(ls, numel=self.n, name="ls"),
+                                        _ptr(p, numel=self.n, name="p"), float(length),
+                                        ctypes.byref(loss), _stream()), "UNet3D.loss_grad")
+        return loss.value
+
+    def adam(self, lr):
+        check(lib().whhd_unet_adam(self._h, float(lr), _stream()), "UNet3D.adam")
+
+    def save(self, path):
+        """Checkpoint: b"WHUN", uint32 version 1, uint64 count, count FP32 weights
+        (little endian)."""
+        import numpy as np
+        w = self.get_params(0)
+        with open(path, "wb") as fh:
+            fh.write(self.MAGIC + np.array([1], "<u4").tobytes() +
+                     np.array([w.size], "<u8").tobytes() + w.astype("<f4").tobytes())
+
+    def load(self, path):
+        import numpy as np
+        with open(path, "rb") as fh:
+            raw = fh.read()
+        if raw[:4] != self.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 1:
+            raise ValueError(f"{path}: not a U-Net checkpoint of version 1")
+        count = int(np.frombuffer(raw[8:16], "<u8")[0])
+        self.set_params(np.frombuffer(raw[16:16 + 4 * count], "<f4"))
 
 
 def _face_ptrs(dim, xv, yv, zv, names=("xv", "yv", "zv")):
