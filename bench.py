@@ -33,9 +33,12 @@ def parse():
     p.add_argument("--n", type=int, default=256, help="cells per axis")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-runs", type=int, default=5)
-    p.add_argument("--workload", choices=("recover", "batch"), default="recover",
+    p.add_argument("--workload", choices=("recover", "batch", "solids", "learned"),
+                   default="recover",
                    help="recover: one N^3 grid (the headline); batch: 64 grids of 32^3..128^3 "
-                        "with spheres, one HIP graph per batch")
+                        "with spheres, one HIP graph per batch; solids: 192^3 with 8 spheres, "
+                        "fractional multigrid to 1e-6 and the velocity correction; learned: "
+                        "the same solve from the U-Net's prediction against from zero")
     return p.parse_args()
 
 
@@ -81,6 +84,12 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "solids":
+        bench_solids(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
 
     import numpy as np
@@ -315,6 +324,158 @@ def bench_batch(args):
                                "cores": threads, "kind": "port",
                                "sample": "one pass of the C oracle over all 64 problems "
                                          "(FP64, OpenMP)"}
+    print(json.dumps(out))
+
+
+def _line(metric, value, unit, args, ms, data, config, higher=True, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data,
+            "config": config}
+
+
+def bench_solids(args):
+    """192^3 with 8 spheres: a step is the fractional multigrid solve from zero
+    to a relative residual of 1e-6 (mixed-precision refinement, FP32 V-cycles
+    with 2 + 2 weighted-Jacobi sweeps) plus the velocity correction
+    u <- a (u - grad p).  The level set, fractions and f are built before timing."""
+    import numpy as np
+    import torch
+
+    import waveletpotentials_amd as W
+    from waveletpotentials_amd import inputs as I
+    from waveletpotentials_amd import workloads as WL
+
+    torch.cuda.set_device(0)
+    n = (192, 192, 192)
+    cells = 192 ** 3
+    P = WL.solid_problem(n, I.seed_for(2))
+    mg = W.CuMultigridFrac3D(P["ax"], P["ay"], P["az"], n, P["dx"], "n")
+    grad = W.CuGradient.GetInstance()
+    f = P["f"]
+    u0 = [P[k].clone() for k in ("xv", "yv", "zv")]
+    u = [t.clone() for t in u0]
+    g = [torch.empty_like(t) for t in u0]
+
+    def step():
+        for a, b in zip(u, u0):
+            a.copy_(b)
+        hi, lo, cyc, res = WL.mg_solve_from(mg, f, None, 1e-6, 400)
+        grad.SubtractGradient3D_Frac_P(*u, hi, P["ax"], P["ay"], P["az"], n, P["dx"], "n")
+        grad.Gradient3D_Frac_P(*g, lo, P["ax"], P["ay"], P["az"], n, P["dx"], "n")
+        for a, b, fr in zip(u, g, (P["ax"], P["ay"], P["az"])):
+            a.sub_(fr * b)
+        return cyc, res
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cycles, res = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # V-cycle time and the stage model's bytes per cycle (per finest cell: 4
+    # smoothing sweeps x 6 + residual 6 + restrict 2.625 + prolong 3.625 + clear
+    # 0.125 = 36.375 floats, x 8/7 for the coarser levels)
+    v = torch.zeros_like(f)
+    e0, e1 = torch.cuda.Event(True), torch.cuda.Event(True)
+    e0.record()
+    mg.Solve(v, v, f, 20, 2, 2, 0.8)
+    e1.record()
+    torch.cuda.synchronize()
+    vc_ms = e0.elapsed_time(e1) / 20
+    cyc_bytes = 36.375 * 8.0 / 7.0 * 4 * cells
+    achieved = cyc_bytes / (vc_ms * 1e-3) / 1e9
+    # divergence of the corrected velocity a (u - grad p), which the solve makes
+    # zero (the reference checks div(a (u - grad p)) the same way,
+    # TestLocalProjection.cpp:690-696)
+    b = torch.empty_like(f)
+    ones = [torch.ones_like(t) for t in u]
+    W.CuConvergence.GetInstance().GetB_3d_frac_vel(b, *u, *ones, n, P["dx"], "n")
+    out = _line("fractional-multigrid solve to 1e-6 + velocity correction, Mcells/s @192^3",
+                round(cells / (ms * 1e-3) / 1e6, 1), "Mcells/s", args, ms,
+                "synthetic (8 seeded spheres, fast-swept level set, spectral velocity |k|<=12)",
+                {"workload": "solid-obstacle projection 192^3, 8 spheres, V-cycles 2+2 "
+                             "weighted Jacobi w=0.8 to relative residual 1e-6", "cells": cells,
+                 "v_cycles": cycles, "fast_sweeps": P["sweeps"]})
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "one V-cycle (stage-model bytes / V-cycle time)",
+                       "v_cycle_ms": round(vc_ms, 4), "algorithmic_bytes_per_launch": int(cyc_bytes)}
+    out["accuracy"] = {"relative_residual": res,
+                       "max_abs_div_after_over_before": float(b.abs().max() / f.abs().max())}
+    out["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        O.build()
+        fr = [t.double().cpu().numpy() for t in (P["ax"], P["ay"], P["az"])]
+        fh = f.double().cpu().numpy()
+        t = time.perf_counter()
+        O.mg_solve(fh, *fr, P["dx"], "n", n_iters=2, nu1=2, nu2=2, weight=0.8)
+        cpu_cycle = (time.perf_counter() - t) / 2
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(cells / (cpu_cycle * cycles) / 1e6, 3),
+                               "unit": "Mcells/s", "cores": threads, "kind": "port",
+                               "sample": f"2 FP64 V-cycles of the C oracle at 192^3, "
+                                         f"scaled to the {cycles} cycles of the GPU solve"}
+    print(json.dumps(out))
+
+
+def bench_learned(args):
+    """The learned initial guess on held-out 192^3 solid-obstacle seeds: GPU time
+    and V-cycles to a relative residual of 1e-6 from the U-Net's prediction
+    (prediction included) against from zero; value = plain time / learned time."""
+    import statistics
+
+    import torch
+
+    import waveletpotentials_amd as W
+    from waveletpotentials_amd import train_unet as TU
+
+    torch.cuda.set_device(0)
+    n = (192, 192, 192)
+    net = W.UNet3D(n)
+    net.load(os.path.join(ROOT, "waveletpotentials_amd", "weights", "unet_solids192.bin"))
+    seeds = [TU.EVAL_SEED0 + i for i in range(max(1, args.steps))]
+    rows = TU.evaluate(net, n, seeds)
+    pm = statistics.mean(r["plain_ms"] for r in rows)
+    lm = statistics.mean(r["learned_ms"] for r in rows)
+    f = torch.empty(n[::-1], device="cuda")
+    e0, e1 = torch.cuda.Event(True), torch.cuda.Event(True)
+    p0 = torch.empty_like(f)
+    ls = torch.zeros_like(f)
+    f.normal_()
+    net.predict(p0, f, ls, 1.0)
+    e0.record()
+    for _ in range(5):
+        net.predict(p0, f, ls, 1.0)
+    e1.record()
+    torch.cuda.synchronize()
+    pred_ms = e0.elapsed_time(e1) / 5
+    flops = 2.0 * 192 ** 3 * 27 * sum(
+        (ca + cb) * co / (8 ** lev) for ca, cb, co, lev in
+        ((16, 0, 16, 0), (16, 0, 16, 0), (16, 0, 32, 1), (32, 0, 32, 1), (32, 0, 64, 2),
+         (64, 0, 64, 2), (64, 32, 32, 1), (32, 0, 32, 1), (32, 16, 16, 0), (16, 0, 16, 0),
+         (16, 0, 1, 0)))
+    out = _line("learned-init MG speedup (GPU time to 1e-6, plain / learned) @192^3",
+                round(pm / lm, 3), "x", args, lm,
+                "synthetic held-out seeds of the solid-obstacle workload",
+                {"workload": "fractional multigrid 192^3, 8 spheres, to relative residual 1e-6, "
+                             "from zero vs from the 3-level U-Net prediction",
+                 "seeds": len(rows),
+                 "plain_cycles_mean": statistics.mean(r["plain_cycles"] for r in rows),
+                 "learned_cycles_mean": statistics.mean(r["learned_cycles"] for r in rows),
+                 "plain_ms_mean": round(pm, 3), "learned_ms_mean": round(lm, 3),
+                 "prediction_rel_err_mean": statistics.mean(r["prediction_rel_err"] for r in rows)},
+                dtype="bf16")
+    achieved = flops / (pred_ms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0,
+                       "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
+                       "kernel": "U-Net forward (algorithmic FLOPs / prediction time)",
+                       "prediction_ms": round(pred_ms, 3)}
+    out["cpu_baseline"] = None
+    out["rows"] = rows
     print(json.dumps(out))
 
 

@@ -286,6 +286,10 @@ WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const f
  * handle.  Writes the loss when `loss` is not NULL (synchronizes `stream`). */
 WHHD_API int whhd_unet_loss_grad(whhd_unet_t h, const float* f, const float* ls, const float* p,
                                  double length, double* loss, void* stream);
+/* Units of the network output: p = output * rms(f) * length^2 * scale
+ * (default 1; training sets it from its data so the output is O(1)). */
+WHHD_API int whhd_unet_set_output_scale(whhd_unet_t h, double scale);
+WHHD_API double whhd_unet_output_scale(whhd_unet_t h);
 /* One Adam step (beta 0.9 / 0.999) with the kept gradients. */
 WHHD_API int whhd_unet_adam(whhd_unet_t h, float lr, void* stream);
 WHHD_API void whhd_unet_destroy(whhd_unet_t h);

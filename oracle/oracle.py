@@ -458,22 +458,22 @@ def unet_forward(params, f, ls, length, bf16=False):
     return out, A, ms
 
 
-def unet_predict(params, f, ls, length, bf16=False):
-    """The initial guess p0 = output * rms(f) * length^2, shaped like f."""
+def unet_predict(params, f, ls, length, bf16=False, out_scale=1.0):
+    """The initial guess p0 = output * rms(f) * length^2 * out_scale, shaped like f."""
     out, _, ms = unet_forward(params, f, ls, length, bf16)
-    return (out * np.sqrt(ms) * length ** 2).reshape(f.shape)
+    return (out * np.sqrt(ms) * length ** 2 * out_scale).reshape(f.shape)
 
 
-def unet_loss_grad(params, f, ls, p, length, bf16=False):
-    """loss = mean((e - mean e)^2), e = output - p / (rms(f) length^2), and
-    its gradient with respect to the flat parameters."""
+def unet_loss_grad(params, f, ls, p, length, bf16=False, out_scale=1.0):
+    """loss = mean((e - mean e)^2), e = output - p / (rms(f) length^2 out_scale),
+    and its gradient with respect to the flat parameters."""
     R = bf16_round if bf16 else (lambda a: a)
     lay, n = unet_layout()
     out, A, ms = unet_forward(params, f, ls, length, bf16)
     nz, nx, ny = f.shape
     d0, d1, d2 = (nz, nx, ny), (nz // 2, nx // 2, ny // 2), (nz // 4, nx // 4, ny // 4)
     V0 = out.size
-    e = out - np.ravel(p) / (np.sqrt(ms) * length ** 2)
+    e = out - np.ravel(p) / (np.sqrt(ms) * length ** 2 * out_scale)
     eb = float(e.mean())
     loss = float(np.mean(e * e) - eb * eb)
     g = np.zeros((V0, 16))

@@ -71,6 +71,8 @@ SIGNATURES = {
     "whhd_unet_predict": (_I, [_P, _P, _P, _P, _D, _P]),
     "whhd_unet_loss_grad": (_I, [_P, _P, _P, _P, _D, ctypes.POINTER(_D), _P]),
     "whhd_unet_adam": (_I, [_P, ctypes.c_float, _P]),
+    "whhd_unet_set_output_scale": (_I, [_P, _D]),
+    "whhd_unet_output_scale": (_D, [_P]),
     "whhd_unet_destroy": (None, [_P]),
     "whhd_gradient3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
     "whhd_subtract_gradient_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),

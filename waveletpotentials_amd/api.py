@@ -568,23 +568,31 @@ class UNet3D(_Handle):
     def adam(self, lr):
         check(lib().whhd_unet_adam(self._h, float(lr), _stream()), "UNet3D.adam")
 
+    def set_output_scale(self, scale):
+        check(lib().whhd_unet_set_output_scale(self._h, float(scale)), "UNet3D.set_output_scale")
+
+    def output_scale(self):
+        return float(lib().whhd_unet_output_scale(self._h))
+
     def save(self, path):
-        """Checkpoint: b"WHUN", uint32 version 1, uint64 count, count FP32 weights
-        (little endian)."""
+        """Checkpoint (little endian): b"WHUN", uint32 version 2, uint64 count,
+        float64 output scale, count FP32 weights."""
         import numpy as np
         w = self.get_params(0)
         with open(path, "wb") as fh:
-            fh.write(self.MAGIC + np.array([1], "<u4").tobytes() +
-                     np.array([w.size], "<u8").tobytes() + w.astype("<f4").tobytes())
+            fh.write(self.MAGIC + np.array([2], "<u4").tobytes() +
+                     np.array([w.size], "<u8").tobytes() +
+                     np.array([self.output_scale()], "<f8").tobytes() + w.astype("<f4").tobytes())
 
     def load(self, path):
         import numpy as np
         with open(path, "rb") as fh:
             raw = fh.read()
-        if raw[:4] != self.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 1:
-            raise ValueError(f"{path}: not a U-Net checkpoint of version 1")
+        if raw[:4] != self.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 2:
+            raise ValueError(f"{path}: not a U-Net checkpoint of version 2")
         count = int(np.frombuffer(raw[8:16], "<u8")[0])
-        self.set_params(np.frombuffer(raw[16:16 + 4 * count], "<f4"))
+        self.set_output_scale(float(np.frombuffer(raw[16:24], "<f8")[0]))
+        self.set_params(np.frombuffer(raw[24:24 + 4 * count], "<f4"))
 
 
 def _face_ptrs(dim, xv, yv, zv, names=("xv", "yv", "zv")):

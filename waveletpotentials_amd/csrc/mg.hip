@@ -48,45 +48,45 @@ struct LevelArgs {
 };
 
 // Diagonal and off-diagonal sum of cell (x, y, z) applied to the iterate,
-// read through src(j), in arithmetic T.  Z: walls carry a Dirichlet
+// read through src(j), accumulated in T; the face coefficients are FP32 (the
+// operator every level's kernels share).  Z: walls carry a Dirichlet
 // coefficient with the exterior spacing; otherwise closed.
 template <bool Z, class T, class Src>
 __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int y, int z, int64_t i,
                                         T& diag, T& off) {
     const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
-    const T hx = a.hx[x], hy = a.hy[y], hz = a.hz[z];
     diag = T(0);
     off = T(0);
     // y faces: ay index (y, x, z) with extents (nx, ny+1, nz)
     const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
     if (y > 0 || Z) {
-        const T cf = face_cf<T>(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, hy);
+        const T cf = face_cf<float>(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, a.hy[y]);
         diag += cf;
         if (y > 0) off -= cf * src(i - 1);
     }
     if (y < a.ny - 1 || Z) {
-        const T cf = face_cf<T>(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, hy);
+        const T cf = face_cf<float>(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, a.hy[y]);
         diag += cf;
         if (y < a.ny - 1) off -= cf * src(i + 1);
     }
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
     if (x > 0 || Z) {
-        const T cf = face_cf<T>(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, hx);
+        const T cf = face_cf<float>(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, a.hx[x]);
         diag += cf;
         if (x > 0) off -= cf * src(i - sx);
     }
     if (x < a.nx - 1 || Z) {
-        const T cf = face_cf<T>(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, hx);
+        const T cf = face_cf<float>(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, a.hx[x]);
         diag += cf;
         if (x < a.nx - 1) off -= cf * src(i + sx);
     }
     if (z > 0 || Z) {
-        const T cf = face_cf<T>(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, hz);
+        const T cf = face_cf<float>(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, a.hz[z]);
         diag += cf;
         if (z > 0) off -= cf * src(i - sz);
     }
     if (z < a.nz - 1 || Z) {
-        const T cf = face_cf<T>(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, hz);
+        const T cf = face_cf<float>(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, a.hz[z]);
         diag += cf;
         if (z < a.nz - 1) off -= cf * src(i + sz);
     }
@@ -116,6 +116,52 @@ __global__ void __launch_bounds__(256) coarsest_kernel(float* __restrict__ v,
                                                        double* __restrict__ buf, int sweeps,
                                                        double w) {
     const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    if (n <= 256) {
+        // one cell per thread: its coefficients in registers, the iterate in LDS
+        __shared__ double S[2][256];
+        const int i = threadIdx.x;
+        double cf[6] = {0, 0, 0, 0, 0, 0}, diag = 0.0, fi = 0.0;
+        int nb[6] = {-1, -1, -1, -1, -1, -1};
+        if (i < n) {
+            const int y = i % a.ny, x = (i / a.ny) % a.nx, z = i / (a.ny * a.nx);
+            const int sx = a.ny, sz = a.nx * a.ny;
+            const int iay = y + x * (a.ny + 1) + z * a.nx * (a.ny + 1);
+            const int iax = y + x * a.ny + z * (a.nx + 1) * a.ny;
+            // faces in the order of stencil(): y-, y+, x-, x+, z-, z+
+            const bool has[6] = {y > 0 || Z, y < a.ny - 1 || Z, x > 0 || Z, x < a.nx - 1 || Z,
+                                 z > 0 || Z, z < a.nz - 1 || Z};
+            const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
+            const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i],
+                                 a.az[i + sz]};
+            const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
+                                 x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
+                                 z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
+            const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
+            const int off[6] = {-1, 1, -sx, sx, -sz, sz};
+            for (int k = 0; k < 6; ++k)
+                if (has[k]) {
+                    cf[k] = face_cf<float>(fr[k], hn[k], hh[k]);
+                    diag += cf[k];
+                    if (in[k]) nb[k] = i + off[k];
+                }
+            fi = f[i];
+            S[0][i] = v[i];
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int it = 0; it < sweeps; ++it) {
+            if (i < n) {
+                double o = 0.0;
+                for (int k = 0; k < 6; ++k)
+                    if (nb[k] >= 0) o -= cf[k] * S[cur][nb[k]];
+                S[cur ^ 1][i] = diag > 1e-12 ? (1.0 - w) * S[cur][i] + w * (fi - o) / diag : 0.0;
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        if (i < n) v[i] = (float)S[cur][i];
+        return;
+    }
     double* A = buf;
     double* B = buf + n;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) A[i] = v[i];
@@ -137,23 +183,62 @@ __global__ void __launch_bounds__(256) coarsest_kernel(float* __restrict__ v,
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = (float)A[i];
 }
 
-// Residual of the two-float iterate hi + lo in FP64, r = f - A (hi + lo),
-// stored as FP32, with per-workgroup FP64 sums of r^2 and f^2 in part.
+// FP64 face factors of the finest level, 2 / ((h_nbr + h) h) for the low and
+// high face of each cell along each axis (exterior spacing at the walls).
+struct Factors64 {
+    const double *ylo, *yhi, *xlo, *xhi, *zlo, *zhi;
+};
+
+// Residual of the two-float iterate hi + lo in FP64, r = f - A (hi + lo), with
+// the operator's coefficients in FP64 (fraction times factor), stored as FP32,
+// with per-workgroup FP64 sums of r^2 and f^2 in part.
 template <bool Z>
 __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
                                                          const float* __restrict__ hi,
                                                          const float* __restrict__ lo,
                                                          const float* __restrict__ f, LevelArgs a,
-                                                         double* __restrict__ part) {
+                                                         Factors64 k, double* __restrict__ part) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
     double sr = 0.0, sf = 0.0;
     if (y < a.ny && x < a.nx) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+        const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
         auto val = [&](int64_t j) { return (double)hi[j] + (double)lo[j]; };
-        double diag, off;
-        stencil<Z, double>(a, val, x, y, z, i, diag, off);
+        double diag = 0.0, off = 0.0;
+        const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
+        const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
+        if (y > 0 || Z) {
+            const double cf = (double)a.ay[iay] * k.ylo[y];
+            diag += cf;
+            if (y > 0) off -= cf * val(i - 1);
+        }
+        if (y < a.ny - 1 || Z) {
+            const double cf = (double)a.ay[iay + 1] * k.yhi[y];
+            diag += cf;
+            if (y < a.ny - 1) off -= cf * val(i + 1);
+        }
+        if (x > 0 || Z) {
+            const double cf = (double)a.ax[iax] * k.xlo[x];
+            diag += cf;
+            if (x > 0) off -= cf * val(i - sx);
+        }
+        if (x < a.nx - 1 || Z) {
+            const double cf = (double)a.ax[iax + a.ny] * k.xhi[x];
+            diag += cf;
+            if (x < a.nx - 1) off -= cf * val(i + sx);
+        }
+        if (z > 0 || Z) {
+            const double cf = (double)a.az[i] * k.zlo[z];
+            diag += cf;
+            if (z > 0) off -= cf * val(i - sz);
+        }
+        if (z < a.nz - 1 || Z) {
+            const double cf = (double)a.az[i + sz] * k.zhi[z];
+            diag += cf;
+            if (z < a.nz - 1) off -= cf * val(i + sz);
+        }
         const double fi = f[i];
         const double ri = diag > 1e-12 ? fi - (diag * val(i) + off) : 0.0;
         r[i] = (float)ri;
@@ -335,8 +420,9 @@ __global__ void norm_partial_kernel(double* __restrict__ part, const float* __re
     }
 }
 
-__global__ void norm_final_kernel(double* __restrict__ out, const double* __restrict__ part,
-                                  int nparts) {
+__global__ void __launch_bounds__(1024) norm_final_kernel(double* __restrict__ out,
+                                                          const double* __restrict__ part,
+                                                          int nparts) {
     double sr = 0.0, sf = 0.0;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
         sr += part[2 * i];
@@ -346,15 +432,20 @@ __global__ void norm_final_kernel(double* __restrict__ out, const double* __rest
         sr += __shfl_xor(sr, o);
         sf += __shfl_xor(sf, o);
     }
-    __shared__ double lr[4], lf[4];
+    __shared__ double lr[16], lf[16];
     if ((threadIdx.x & 63) == 0) {
         lr[threadIdx.x >> 6] = sr;
         lf[threadIdx.x >> 6] = sf;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = lr[0] + lr[1] + lr[2] + lr[3];
-        out[1] = lf[0] + lf[1] + lf[2] + lf[3];
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            a += lr[k];
+            b += lf[k];
+        }
+        out[0] = a;
+        out[1] = b;
     }
 }
 
@@ -370,6 +461,7 @@ struct whhd_mgfrac3d_s {
     double* part = nullptr;   // norm partial sums (2 * kNormBlocks) + the two totals
     double* host = nullptr;   // pinned host copy of the totals
     double* cbuf = nullptr;   // FP64 ping-pong of the coarsest level (2 * its cells)
+    double* fac64 = nullptr;  // finest level's FP64 face factors: ylo, yhi, xlo, xhi, zlo, zhi
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
     std::vector<void*> owned;
@@ -577,8 +669,27 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         h->part = (double*)alloc((2 * kNormBlocks + 2) * sizeof(double));
         h->cbuf = (double*)alloc((size_t)cells(h->lv.back().d) * 2 * sizeof(double));
         h->rpart = (double*)alloc((size_t)(2 * h->rblocks + 2) * sizeof(double));
-        if (!h->r || !h->part || !h->cbuf || !h->rpart)
+        h->fac64 = (double*)alloc((size_t)2 * (nx + ny + nz) * sizeof(double));
+        if (!h->r || !h->part || !h->cbuf || !h->rpart || !h->fac64)
             st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+    }
+    if (st == WHHD_OK) {
+        // uniform finest spacing: interior faces 2 / (2 h h), wall faces 2 / ((e + h) h)
+        std::vector<double> k;
+        const int ns[3] = {ny, nx, nz};
+        const double hs[3] = {dy, dx, dz};
+        const bool dwall = bc == 'd';  // zero exterior spacing
+        for (int a = 0; a < 3; ++a) {
+            for (int side = 0; side < 2; ++side)
+                for (int c = 0; c < ns[a]; ++c) {
+                    const bool wall = side == 0 ? c == 0 : c == ns[a] - 1;
+                    const double hn = wall ? (dwall ? 0.0 : hs[a]) : hs[a];
+                    k.push_back(2.0 / ((hn + hs[a]) * hs[a]));
+                }
+        }
+        if (hipMemcpy(h->fac64, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            st = fail(WHHD_HIP_ERROR, "mgfrac3d_create: factor upload");
     }
     if (st == WHHD_OK && hipHostMalloc((void**)&h->host, 2 * sizeof(double)) != hipSuccess) {
         h->host = nullptr;
@@ -630,16 +741,19 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     TRY(fill(lo, 0.f, n, s));
     const LevelArgs a = args_of(h, L0);
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    const int nx = L0.d.x, ny = L0.d.y;
+    const Factors64 k{h->fac64, h->fac64 + ny, h->fac64 + 2 * ny, h->fac64 + 2 * ny + nx,
+                      h->fac64 + 2 * (ny + nx), h->fac64 + 2 * (ny + nx) + L0.d.z};
     double res = 0.0;
     int c = 0;
     for (;; ++c) {
         if (h->bc == 'z')
             hipLaunchKernelGGL(residual64_kernel<true>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
-                               h->rpart);
+                               k, h->rpart);
         else
             hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
-                               h->rpart);
-        hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, h->rpart + 2 * h->rblocks,
+                               k, h->rpart);
+        hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
                            h->rpart, (int)h->rblocks);
         hipError_t e = launch_status("mg_residual64_kernel");
         if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
@@ -668,7 +782,7 @@ int mgfrac3d_residual(whhd_mgfrac3d_s* h, const float* v, const float* f, double
     const int64_t n = cells(h->dim);
     TRY(residual(h, L0, h->r, v, f, s));
     hipLaunchKernelGGL(norm_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, h->r, f, n);
-    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, h->part + 2 * kNormBlocks,
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->part + 2 * kNormBlocks,
                        h->part, kNormBlocks);
     hipError_t e = launch_status("mg_norm_kernels");
     if (e != hipSuccess) return hip_fail(e, "mg norm kernels");

@@ -487,6 +487,7 @@ struct whhd_unet_s {
     double* host = nullptr;   // pinned copy of lsum
     int64_t step = 0;
     float gscale = 1.f;  // loss-gradient scale (divided out before Adam)
+    double out_scale = 1.0;  // output units: p = output * rms(f) * L^2 * out_scale
     std::vector<void*> owned;
 };
 
@@ -629,7 +630,7 @@ int backward(whhd_unet_s* h, const float* p, float len, hipStream_t st) {
     const int Z = h->nz, X = h->nx, Y = h->ny;
     const int Z1 = Z / 2, X1 = X / 2, Y1 = Y / 2, Z2 = Z / 4, X2 = X / 4, Y2 = Y / 4;
     const int64_t V0 = h->V[0], V1 = h->V[1], V2 = h->V[2];
-    const float len2 = len * len;
+    const float len2 = (float)((double)len * len * h->out_scale);
     hipLaunchKernelGGL(zero_kernel, g1(h->nparams), dim3(256), 0, st, h->grad, h->nparams);
     hipLaunchKernelGGL(sums_kernel<true>, dim3(kParts), dim3(256), 0, st, h->part, h->out, p,
                        stats(h), len2, V0);
@@ -871,7 +872,7 @@ WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const f
     int r = unet::forward(h, f, ls, (float)length, st);
     if (r != WHHD_OK) return r;
     hipLaunchKernelGGL(unet::output_kernel, unet::g1(h->V[0]), dim3(256), 0, st, p0, h->out,
-                       unet::stats(h), (float)(length * length), h->V[0]);
+                       unet::stats(h), (float)(length * length * h->out_scale), h->V[0]);
     return unet::check_launch("unet_output_kernel");
 }
 
@@ -906,6 +907,15 @@ WHHD_API int whhd_unet_adam(whhd_unet_t h, float lr, void* stream) {
     int r = unet::check_launch("unet_adam_kernel");
     return r == WHHD_OK ? unet::pack_all(h, st) : r;
 }
+
+WHHD_API int whhd_unet_set_output_scale(whhd_unet_t h, double scale) {
+    using namespace whhd;
+    if (!h || !(scale > 0)) return fail(WHHD_INVALID_ARG, "unet_set_output_scale: bad argument");
+    h->out_scale = scale;
+    return WHHD_OK;
+}
+
+WHHD_API double whhd_unet_output_scale(whhd_unet_t h) { return h ? h->out_scale : 0.0; }
 
 WHHD_API void whhd_unet_destroy(whhd_unet_t h) { unet_release(h); }
 
