@@ -435,8 +435,10 @@ def bench_learned(args):
 
     torch.cuda.set_device(0)
     n = (192, 192, 192)
-    net = W.UNet3D(n)
-    net.load(os.path.join(ROOT, "waveletpotentials_amd", "weights", "unet_solids192.bin"))
+    ckpt = os.path.join(ROOT, "waveletpotentials_amd", "weights", "unet_solids192.bin")
+    factor = W.UNet3D.checkpoint_factor(ckpt)
+    net = W.UNet3D(n, factor=factor)
+    net.load(ckpt)
     seeds = [TU.EVAL_SEED0 + i for i in range(max(1, args.steps))]
     rows = TU.evaluate(net, n, seeds)
     pm = statistics.mean(r["plain_ms"] for r in rows)
@@ -453,7 +455,7 @@ def bench_learned(args):
     e1.record()
     torch.cuda.synchronize()
     pred_ms = e0.elapsed_time(e1) / 5
-    flops = 2.0 * 192 ** 3 * 27 * sum(
+    flops = 2.0 * (192 // factor) ** 3 * 27 * sum(
         (ca + cb) * co / (8 ** lev) for ca, cb, co, lev in
         ((16, 0, 16, 0), (16, 0, 16, 0), (16, 0, 32, 1), (32, 0, 32, 1), (32, 0, 64, 2),
          (64, 0, 64, 2), (64, 32, 32, 1), (32, 0, 32, 1), (32, 16, 16, 0), (16, 0, 16, 0),
@@ -463,6 +465,7 @@ def bench_learned(args):
                 "synthetic held-out seeds of the solid-obstacle workload",
                 {"workload": "fractional multigrid 192^3, 8 spheres, to relative residual 1e-6, "
                              "from zero vs from the 3-level U-Net prediction",
+                 "network_grid": f"{192 // factor}^3 (factor {factor})",
                  "seeds": len(rows),
                  "plain_cycles_mean": statistics.mean(r["plain_cycles"] for r in rows),
                  "learned_cycles_mean": statistics.mean(r["learned_cycles"] for r in rows),

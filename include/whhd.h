@@ -269,16 +269,20 @@ typedef struct whhd_unet_s* whhd_unet_t;
  * solution from (f, solid level set); its output is the pre_v of
  * whhd_mgfrac3d_solve / _solve_tol.  No reference counterpart: the reference's
  * multigrid takes any initial guess as pre_v (WaveletPotentials/
- * cuMultigridFrac3D.cu:249-258).  Extents must be multiples of 4; weights are
- * He-normal from `seed`.  Holds activations for one nx*ny*nz grid. */
-WHHD_API int whhd_unet_create(int nx, int ny, int nz, unsigned long long seed, whhd_unet_t* out);
+ * cuMultigridFrac3D.cu:249-258).  nx, ny, nz are the solve's cells; the
+ * network runs on the grid coarser by `factor` (inputs averaged over factor^3
+ * blocks, the prediction interpolated back trilinearly), so the extents must
+ * be multiples of 4 * factor.  Weights are He-normal from `seed`. */
+WHHD_API int whhd_unet_create(int nx, int ny, int nz, int factor, unsigned long long seed,
+                              whhd_unet_t* out);
 WHHD_API long long whhd_unet_num_params(whhd_unet_t h);
 /* Copy the FP32 master weights (which = 0) or the last gradients (1) to host dst. */
 WHHD_API int whhd_unet_get_params(whhd_unet_t h, float* dst, int which);
 /* Load FP32 master weights from host memory; resets the Adam moments. */
 WHHD_API int whhd_unet_set_params(whhd_unet_t h, const float* src);
-/* p0 = prediction for the cell arrays f (right-hand side) and ls (level set);
- * `length` is the domain's x extent (inputs are scaled by rms(f) and length). */
+/* p0 = prediction for the cell arrays f (right-hand side) and ls (level set),
+ * all on the solve's grid; `length` is the domain's x extent (inputs are
+ * scaled by rms(f) and length). */
 WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const float* ls,
                                double length, void* stream);
 /* Forward and backward for the target p: loss = mean((e - mean e)^2) with

@@ -458,8 +458,38 @@ def unet_forward(params, f, ls, length, bf16=False):
     return out, A, ms
 
 
-def unet_predict(params, f, ls, length, bf16=False, out_scale=1.0):
-    """The initial guess p0 = output * rms(f) * length^2 * out_scale, shaped like f."""
+def unet_down(a, F):
+    """Means of the F^3 blocks of a (nz, nx, ny) cell array."""
+    nz, nx, ny = a.shape
+    return a.reshape(nz // F, F, nx // F, F, ny // F, F).mean(axis=(1, 3, 5))
+
+
+def unet_up(c, F):
+    """Trilinear interpolation of cell-centred values on the coarse grid at the
+    fine cell centres (coarse coordinate (i + 1/2)/F - 1/2, clamped)."""
+    def axis(nc):
+        u = np.clip((np.arange(nc * F) + 0.5) / F - 0.5, 0, nc - 1)
+        i0 = np.floor(u).astype(int)
+        i1 = np.minimum(i0 + 1, nc - 1)
+        return i0, i1, u - i0
+    out = np.asarray(c, dtype=np.float64)
+    for ax in range(3):
+        i0, i1, w = axis(out.shape[ax])
+        shape = [1, 1, 1]
+        shape[ax] = -1
+        out = (np.take(out, i0, axis=ax) * (1 - w).reshape(shape)
+               + np.take(out, i1, axis=ax) * w.reshape(shape))
+    return out
+
+
+def unet_predict(params, f, ls, length, bf16=False, out_scale=1.0, factor=1):
+    """The initial guess p0 = output * rms(f) * length^2 * out_scale, shaped like
+    f; with factor > 1 the network runs on the block means of f and ls and its
+    output is interpolated back."""
+    if factor > 1:
+        fc, lc = unet_down(f, factor), unet_down(ls, factor)
+        out, _, ms = unet_forward(params, fc, lc, length, bf16)
+        return unet_up(out.reshape(fc.shape), factor) * np.sqrt(ms) * length ** 2 * out_scale
     out, _, ms = unet_forward(params, f, ls, length, bf16)
     return (out * np.sqrt(ms) * length ** 2 * out_scale).reshape(f.shape)
 

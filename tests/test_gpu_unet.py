@@ -51,6 +51,20 @@ def test_prediction_matches_oracle(whhd, orc, dims):
     assert rel(p0.cpu().numpy(), ref) < 1e-2
 
 
+def test_prediction_on_a_coarser_grid_matches_oracle(whhd, orc):
+    dims = (16, 16, 24)      # (nx, ny, nz) of the solve; network on 8 x 8 x 12
+    nx, ny, nz = dims
+    net = whhd.UNet3D(dims, seed=4, factor=2)
+    net.set_output_scale(3e-3)
+    f, ls, _ = sample((nz, nx, ny), 6)
+    p0 = torch.empty((nz, nx, ny), device="cuda")
+    net.predict(p0, torch.from_numpy(f).cuda(), torch.from_numpy(ls).cuda(), 1.0)
+    torch.cuda.synchronize()
+    ref = orc.unet_predict(effective(orc, net.get_params()), f.astype(np.float64),
+                           ls.astype(np.float64), 1.0, bf16=True, out_scale=3e-3, factor=2)
+    assert rel(p0.cpu().numpy(), ref) < 1e-2
+
+
 def test_gradients_match_oracle(whhd, orc):
     dims = (16, 16, 16)
     net = whhd.UNet3D(dims, seed=5)

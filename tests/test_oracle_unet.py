@@ -7,7 +7,9 @@
   the weight gradient is the derivative of the (linear in w) convolution;
 * the composed network's gradient matches central finite differences of its
   loss, layer by layer, biases included;
-* BF16 rounding is round-to-nearest-even on the FP32 bit pattern.
+* BF16 rounding is round-to-nearest-even on the FP32 bit pattern;
+* the resampling around a coarser network keeps constants, and the trilinear
+  interpolation reproduces a linear field away from the walls.
 """
 import numpy as np
 import pytest
@@ -102,3 +104,20 @@ def test_bf16_round(orc):
     assert r[3] == -2.5
     assert abs(r[4] - 1e-30) <= 1e-30 * 2.0 ** -8
     assert r[5] == 3.140625
+
+
+def test_resampling(orc):
+    c = np.full((3, 4, 5), 2.5)
+    assert np.allclose(orc.unet_up(c, 4), 2.5)
+    assert np.allclose(orc.unet_down(np.full((8, 12, 4), -1.0), 4), -1.0)
+    # linear field sampled at coarse cell centres; interior fine cells exact
+    F, nc = 2, (4, 5, 6)
+    zc, xc, yc = np.meshgrid(*[np.arange(n) + 0.5 for n in nc], indexing="ij")
+    coarse = 0.3 * zc - 1.2 * xc + 0.7 * yc
+    fine = orc.unet_up(coarse, F)
+    zf, xf, yf = np.meshgrid(*[(np.arange(n * F) + 0.5) / F for n in nc], indexing="ij")
+    ref = 0.3 * zf - 1.2 * xf + 0.7 * yf
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(fine[inner], ref[inner], atol=1e-12)
+    # block means of the linear fine field are the coarse values
+    assert np.allclose(orc.unet_down(ref, F), coarse, atol=1e-12)

@@ -64,7 +64,7 @@ SIGNATURES = {
                                      _P]),
     "whhd_mgfrac3d_residual": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "whhd_mgfrac3d_destroy": (None, [_P]),
-    "whhd_unet_create": (_I, [_I, _I, _I, ctypes.c_ulonglong, ctypes.POINTER(_P)]),
+    "whhd_unet_create": (_I, [_I, _I, _I, _I, ctypes.c_ulonglong, ctypes.POINTER(_P)]),
     "whhd_unet_num_params": (ctypes.c_longlong, [_P]),
     "whhd_unet_get_params": (_I, [_P, _P, _I]),
     "whhd_unet_set_params": (_I, [_P, _P]),

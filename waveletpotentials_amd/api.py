@@ -521,15 +521,18 @@ class UNet3D(_Handle):
     channels, BF16 matrix-core convolutions with FP32 accumulation) from (f,
     solid level set) to the pressure; its prediction is the multigrid's pre_v.
     No reference counterpart (the reference's Solve takes any pre_v,
-    WaveletPotentials/cuMultigridFrac3D.cu:249-258).  `dim` = (nx, ny, nz) cells,
-    multiples of 4; the activations it holds are sized for that grid."""
+    WaveletPotentials/cuMultigridFrac3D.cu:249-258).  `dim` = (nx, ny, nz) cells of
+    the solve; the network runs `factor` times coarser (block means in,
+    trilinear interpolation out), so the extents are multiples of 4 * factor."""
     _destroy = "whhd_unet_destroy"
     MAGIC = b"WHUN"
 
-    def __init__(self, dim, seed=0):
+    def __init__(self, dim, seed=0, factor=1):
         super().__init__()
         self.dim = tuple(int(v) for v in dim)
-        check(lib().whhd_unet_create(*self.dim, int(seed), ctypes.byref(self._h)), "UNet3D")
+        self.factor = int(factor)
+        check(lib().whhd_unet_create(*self.dim, self.factor, int(seed), ctypes.byref(self._h)),
+              "UNet3D")
         self.n = int(self.dim[0] * self.dim[1] * self.dim[2])
 
     def num_params(self):
@@ -575,24 +578,37 @@ class UNet3D(_Handle):
         return float(lib().whhd_unet_output_scale(self._h))
 
     def save(self, path):
-        """Checkpoint (little endian): b"WHUN", uint32 version 2, uint64 count,
-        float64 output scale, count FP32 weights."""
+        """Checkpoint (little endian): b"WHUN", uint32 version 3, uint64 count,
+        float64 output scale, uint32 factor, uint32 0, count FP32 weights."""
         import numpy as np
         w = self.get_params(0)
         with open(path, "wb") as fh:
-            fh.write(self.MAGIC + np.array([2], "<u4").tobytes() +
+            fh.write(self.MAGIC + np.array([3], "<u4").tobytes() +
                      np.array([w.size], "<u8").tobytes() +
-                     np.array([self.output_scale()], "<f8").tobytes() + w.astype("<f4").tobytes())
+                     np.array([self.output_scale()], "<f8").tobytes() +
+                     np.array([self.factor, 0], "<u4").tobytes() + w.astype("<f4").tobytes())
+
+    @staticmethod
+    def checkpoint_factor(path):
+        import numpy as np
+        with open(path, "rb") as fh:
+            raw = fh.read(32)
+        if raw[:4] != UNet3D.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 3:
+            raise ValueError(f"{path}: not a U-Net checkpoint of version 3")
+        return int(np.frombuffer(raw[24:28], "<u4")[0])
 
     def load(self, path):
         import numpy as np
         with open(path, "rb") as fh:
             raw = fh.read()
-        if raw[:4] != self.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 2:
-            raise ValueError(f"{path}: not a U-Net checkpoint of version 2")
+        if raw[:4] != self.MAGIC or np.frombuffer(raw[4:8], "<u4")[0] != 3:
+            raise ValueError(f"{path}: not a U-Net checkpoint of version 3")
         count = int(np.frombuffer(raw[8:16], "<u8")[0])
+        factor = int(np.frombuffer(raw[24:28], "<u4")[0])
+        if factor != self.factor:
+            raise ValueError(f"{path}: trained with factor {factor}, this network has {self.factor}")
         self.set_output_scale(float(np.frombuffer(raw[16:24], "<f8")[0]))
-        self.set_params(np.frombuffer(raw[24:24 + 4 * count], "<f4"))
+        self.set_params(np.frombuffer(raw[32:32 + 4 * count], "<f4"))
 
 
 def _face_ptrs(dim, xv, yv, zv, names=("xv", "yv", "zv")):

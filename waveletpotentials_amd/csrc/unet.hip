@@ -430,6 +430,59 @@ __global__ void pack_kernel(bf16* __restrict__ wk, bf16* __restrict__ wt,
     wt[(int64_t)ci * kpad(CO) + (kTaps - 1 - tap) * CO + co] = v;
 }
 
+// Network grid from a finer one: the mean of each F^3 block of fine cells.
+__global__ void down_kernel(float* __restrict__ out, const float* __restrict__ in, int Zc, int Xc,
+                            int Yc, int F) {
+    const int64_t n = (int64_t)Zc * Xc * Yc;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int yc = (int)(i % Yc);
+    const int64_t r = i / Yc;
+    const int xc = (int)(r % Xc), zc = (int)(r / Xc);
+    const int64_t X = (int64_t)Xc * F, Y = (int64_t)Yc * F;
+    float s = 0.f;
+    for (int a = 0; a < F; ++a)
+        for (int b = 0; b < F; ++b) {
+            const float* row = in + (((int64_t)zc * F + a) * X + (int64_t)xc * F + b) * Y + (int64_t)yc * F;
+            for (int d = 0; d < F; ++d) s += row[d];
+        }
+    out[i] = s / (float)(F * F * F);
+}
+
+// Fine cell values by trilinear interpolation of the network output at cell
+// centres (coarse coordinate (i + 1/2) / F - 1/2, clamped at the walls), times
+// rms(f) * L^2 * scale.
+__global__ void up_kernel(float* __restrict__ p, const float* __restrict__ out,
+                          const double* __restrict__ stats, float len2, int Zc, int Xc, int Yc,
+                          int F) {
+    const int64_t Z = (int64_t)Zc * F, X = (int64_t)Xc * F, Y = (int64_t)Yc * F;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Z * X * Y) return;
+    const int y = (int)(i % Y);
+    const int64_t r = i / Y;
+    const int x = (int)(r % X), z = (int)(r / X);
+    auto axis = [&](int c, int nc, int& c0, int& c1, float& w) {
+        float u = ((float)c + 0.5f) / (float)F - 0.5f;
+        u = fminf(fmaxf(u, 0.f), (float)(nc - 1));
+        c0 = (int)u;
+        c1 = c0 + 1 < nc ? c0 + 1 : c0;
+        w = u - (float)c0;
+    };
+    int z0, z1, x0, x1, y0, y1;
+    float wz, wx, wy;
+    axis(z, Zc, z0, z1, wz);
+    axis(x, Xc, x0, x1, wx);
+    axis(y, Yc, y0, y1, wy);
+    auto at = [&](int a, int b, int c) { return out[((int64_t)a * Xc + b) * Yc + c]; };
+    const float v0 = (1.f - wx) * ((1.f - wy) * at(z0, x0, y0) + wy * at(z0, x0, y1)) +
+                     wx * ((1.f - wy) * at(z0, x1, y0) + wy * at(z0, x1, y1));
+    const float v1 = (1.f - wx) * ((1.f - wy) * at(z1, x0, y0) + wy * at(z1, x0, y1)) +
+                     wx * ((1.f - wy) * at(z1, x1, y0) + wy * at(z1, x1, y1));
+    const double Vc = (double)Zc * Xc * Yc;
+    const double ms = stats[1] / Vc;
+    p[i] = (float)((double)((1.f - wz) * v0 + wz * v1) * sqrt(ms) * (double)len2);
+}
+
 __global__ void zero_kernel(float* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.f;
@@ -468,7 +521,9 @@ constexpr LayerDef kNet[kLayers] = {
 }  // namespace
 
 struct whhd_unet_s {
-    int nx, ny, nz;
+    int nx, ny, nz;   // network grid
+    int factor = 1;   // fine grid = factor * network grid
+    float *fc = nullptr, *lsc = nullptr, *pc = nullptr;  // inputs averaged to the network grid
     int64_t V[3];
     int64_t woff[kLayers], boff[kLayers], nparams = 0;
     float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
@@ -686,6 +741,14 @@ int backward(whhd_unet_s* h, const float* p, float len, hipStream_t st) {
     return bwd_weight(h, 0, {h->x, nullptr, Z, X, Y}, h->g0b, st);
 }
 
+// `in` on the network grid: itself, or its F^3 block means in `buf`.
+const float* coarse(whhd_unet_s* h, float* buf, const float* in, hipStream_t st) {
+    if (h->factor == 1) return in;
+    hipLaunchKernelGGL(down_kernel, g1(h->V[0]), dim3(256), 0, st, buf, in, h->nz, h->nx, h->ny,
+                       h->factor);
+    return buf;
+}
+
 int pack_all(whhd_unet_s* h, hipStream_t st) {
     for (int l = 0; l < kLayers; ++l) {
         const int cin = kNet[l].ca + kNet[l].cb, co = kNet[l].co;
@@ -739,16 +802,23 @@ void unet_release(whhd_unet_s* h) {
 
 extern "C" {
 
-WHHD_API int whhd_unet_create(int nx, int ny, int nz, unsigned long long seed, whhd_unet_t* out) {
+WHHD_API int whhd_unet_create(int nx, int ny, int nz, int factor, unsigned long long seed,
+                              whhd_unet_t* out) {
     using namespace whhd;
     using namespace whhd::unet;
     if (!out) return fail(WHHD_INVALID_ARG, "unet_create: null handle pointer");
     *out = nullptr;
-    if (nx < 4 || ny < 4 || nz < 4 || nx % 4 || ny % 4 || nz % 4)
-        return fail(WHHD_INVALID_ARG, "unet_create: extents must be multiples of 4");
+    if (factor < 1 || factor > 16) return fail(WHHD_INVALID_ARG, "unet_create: factor out of range");
+    const int q = 4 * factor;
+    if (nx < q || ny < q || nz < q || nx % q || ny % q || nz % q)
+        return fail(WHHD_INVALID_ARG, "unet_create: extents must be multiples of 4 * factor");
     if ((int64_t)nx * ny * nz * 96 >= ((int64_t)1 << 40))
         return fail(WHHD_INVALID_ARG, "unet_create: grid too large");
     auto* h = new whhd_unet_s;
+    h->factor = factor;
+    nx /= factor;
+    ny /= factor;
+    nz /= factor;
     h->nx = nx; h->ny = ny; h->nz = nz;
     h->V[0] = (int64_t)nx * ny * nz;
     h->V[1] = h->V[0] / 8;
@@ -794,6 +864,11 @@ WHHD_API int whhd_unet_create(int nx, int ny, int nz, unsigned long long seed, w
     h->g1a = act(V1, 32); h->g1b = act(V1, 32); h->u1 = act(V1, 64); h->ge1 = act(V1, 32);
     h->g1c = act(V1, 16); h->g2a = act(V2, 64); h->g2b = act(V2, 64); h->g2c = act(V2, 32);
     h->part = (double*)alloc((2 * kParts + 4) * sizeof(double));
+    if (factor > 1) {
+        h->fc = (float*)alloc((size_t)V0 * 4);
+        h->lsc = (float*)alloc((size_t)V0 * 4);
+        h->pc = (float*)alloc((size_t)V0 * 4);
+    }
     if (ok && hipHostMalloc((void**)&h->host, 4 * sizeof(double)) != hipSuccess) {
         h->host = nullptr;
         ok = false;
@@ -869,10 +944,18 @@ WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const f
     using namespace whhd;
     if (!h || !p0 || !f || !ls || !(length > 0)) return fail(WHHD_INVALID_ARG, "unet_predict: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    int r = unet::forward(h, f, ls, (float)length, st);
+    const float* fin = unet::coarse(h, h->fc, f, st);
+    const float* lin = unet::coarse(h, h->lsc, ls, st);
+    int r = unet::forward(h, fin, lin, (float)length, st);
     if (r != WHHD_OK) return r;
-    hipLaunchKernelGGL(unet::output_kernel, unet::g1(h->V[0]), dim3(256), 0, st, p0, h->out,
-                       unet::stats(h), (float)(length * length * h->out_scale), h->V[0]);
+    const float len2 = (float)(length * length * h->out_scale);
+    if (h->factor == 1)
+        hipLaunchKernelGGL(unet::output_kernel, unet::g1(h->V[0]), dim3(256), 0, st, p0, h->out,
+                           unet::stats(h), len2, h->V[0]);
+    else
+        hipLaunchKernelGGL(unet::up_kernel, unet::g1(h->V[0] * h->factor * h->factor * h->factor),
+                           dim3(256), 0, st, p0, h->out, unet::stats(h), len2, h->nz, h->nx, h->ny,
+                           h->factor);
     return unet::check_launch("unet_output_kernel");
 }
 
@@ -881,8 +964,11 @@ WHHD_API int whhd_unet_loss_grad(whhd_unet_t h, const float* f, const float* ls,
     using namespace whhd;
     if (!h || !f || !ls || !p || !(length > 0)) return fail(WHHD_INVALID_ARG, "unet_loss_grad: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    int r = unet::forward(h, f, ls, (float)length, st);
-    if (r == WHHD_OK) r = unet::backward(h, p, (float)length, st);
+    const float* fin = unet::coarse(h, h->fc, f, st);
+    const float* lin = unet::coarse(h, h->lsc, ls, st);
+    const float* pin = unet::coarse(h, h->pc, p, st);
+    int r = unet::forward(h, fin, lin, (float)length, st);
+    if (r == WHHD_OK) r = unet::backward(h, pin, (float)length, st);
     if (r != WHHD_OK) return r;
     if (loss) {
         hipError_t e = hipMemcpyAsync(h->host, unet::lsum(h), 2 * sizeof(double),

@@ -42,9 +42,13 @@ def scale_set(net):
     return net.output_scale() != 1.0
 
 
-def train(net, n, samples, epochs, lr, log=sys.stderr, bf16_cache=True):
+def train(net, n, samples, epochs, lr, log=sys.stderr, bf16_cache=True, flips=True):
     """Adam on `samples` seeds for `epochs` passes; the first pass generates and
-    solves the samples (kept on the device, BF16 when bf16_cache)."""
+    solves the samples (kept on the device, BF16 when bf16_cache).  With flips,
+    each step mirrors (f, level set, p) along a random subset of the axes: the
+    walls are closed on every side, so a mirrored problem is another valid
+    problem with the mirrored solution."""
+    gen = torch.Generator().manual_seed(12345)
     cache = []
     losses = []
     t0 = time.time()
@@ -67,6 +71,10 @@ def train(net, n, samples, epochs, lr, log=sys.stderr, bf16_cache=True):
             step = ep * samples + i
             total = epochs * samples
             lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+            if flips:
+                axes = [d for d in range(3) if torch.rand(1, generator=gen).item() < 0.5]
+                if axes:
+                    f, ls, p = (t.flip(axes).contiguous() for t in (f, ls, p))
             loss = net.loss_grad(f, ls, p, length)
             net.adam(lr_t)
             losses.append(loss)
@@ -117,18 +125,20 @@ def main():
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--factor", type=int, default=4,
+                    help="the network runs on the grid this many times coarser")
     ap.add_argument("--eval", type=int, default=8, help="held-out seeds to evaluate")
     ap.add_argument("--init", default=None, help="start from this checkpoint")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     n = (a.n, a.n, a.n)
-    net = api.UNet3D(n, seed=a.seed)
+    net = api.UNet3D(n, seed=a.seed, factor=a.factor)
     if a.init:
         net.load(a.init)
     losses = train(net, n, a.samples, a.epochs, a.lr) if a.samples > 0 else []
     net.save(a.out)
     rows = evaluate(net, n, [EVAL_SEED0 + i for i in range(a.eval)])
-    print(json.dumps({"n": a.n, "samples": a.samples, "epochs": a.epochs,
+    print(json.dumps({"n": a.n, "factor": a.factor, "samples": a.samples, "epochs": a.epochs,
                       "final_loss": losses[-1] if losses else None, "eval": rows}))
 
 
