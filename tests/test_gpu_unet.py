@@ -106,3 +106,21 @@ def test_adam_steps_reduce_loss_and_checkpoint_roundtrip(whhd, tmp_path):
     net.predict(a, f, ls, 1.0)
     other.predict(b, f, ls, 1.0)
     assert torch.equal(a, b)
+
+
+def test_trained_network_saves_vcycles(whhd):
+    """The committed checkpoint (trained on 2000 generated 192^3 solid-obstacle
+    solves) against the plain multigrid on two held-out seeds: the solve from
+    its prediction reaches 1e-6 in fewer V-cycles than from zero."""
+    import os
+    from waveletpotentials_amd import train_unet as TU
+    ckpt = os.path.join(os.path.dirname(whhd.__file__), "weights", "unet_solids192.bin")
+    n = (192, 192, 192)
+    net = whhd.UNet3D(n, factor=whhd.UNet3D.checkpoint_factor(ckpt))
+    net.load(ckpt)
+    rows = TU.evaluate(net, n, [TU.EVAL_SEED0, TU.EVAL_SEED0 + 1])
+    for r in rows:
+        assert r["plain_res"] <= 1e-6 and r["learned_res"] <= 1e-6
+        assert r["learned_cycles"] <= r["plain_cycles"]
+        assert r["prediction_rel_err"] < 0.6
+    assert sum(r["learned_cycles"] for r in rows) < sum(r["plain_cycles"] for r in rows)
