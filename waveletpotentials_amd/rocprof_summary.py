@@ -11,6 +11,8 @@ bench.py runs on a known size (an elementwise multiply reading and writing
 
 writes OUT_PREFIX.md (per-kernel durations and bytes) and OUT_PREFIX.json
 (per-kernel fetch/write bytes, read by bench.py for ``roofline.traffic``).
+With ``--trace-only TRACE_DIR OUT_PREFIX`` it writes OUT_PREFIX.md with the
+per-kernel durations of one ``--kernel-trace --stats`` run.
 """
 import argparse
 import collections
@@ -34,9 +36,21 @@ def kernel_label(name):
         return (f"lp_kernel<{_WT.get(m.group(1), m.group(1))},L={m.group(2)},"
                 f"{'fwd' if m.group(4) == '1' else 'inv'},"
                 f"{'contiguous' if m.group(5) == '1' else 'strided'}>")
-    m = re.match(r"(?:void )?whhd::(\w+)", name)
+    m = re.search(r"lp_kernel<.*Up(Cdf\d\d),.*>, (\d), (\d), (true|false), (true|false)>", name)
     if m:
-        return m.group(1)
+        return (f"lp_kernel<{_WT.get(m.group(1), m.group(1))},L={m.group(2)},"
+                f"{'fwd' if m.group(4) == 'true' else 'inv'},"
+                f"{'contiguous' if m.group(5) == 'true' else 'strided'}>")
+    m = re.search(r"unet\d+(conv3_wgrad_kernel|conv3_kernel)I((?:L[ib]\d+E)+)E", name)
+    if m:
+        args = [v if k == "i" else ("true" if v == "1" else "false")
+                for k, v in re.findall(r"L([ib])(\d+)E", m.group(2))]
+        return f"unet::{m.group(1)}<{', '.join(args)}>"
+    m = re.match(r"(?:void )?whhd::((?:\w+::)*)(?:\(anonymous namespace\)::)?(\w+(?:<[^>]*>)?)",
+                 name)
+    if m:
+        ns = "unet::" if m.group(1).startswith("unet") else ""
+        return ns + m.group(2)
     return name.split("(")[0][:80]
 
 
@@ -103,16 +117,39 @@ def write_outputs(kernels, f_scale, w_scale, prefix, command):
         f.write("\n".join(lines) + "\n")
 
 
+def write_trace_only(trace_dir, prefix, command, top=30):
+    rows = []
+    with open(_one(os.path.join(trace_dir, "*kernel_stats.csv"))) as f:
+        for r in csv.DictReader(f):
+            rows.append((kernel_label(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"])))
+    merged = {}
+    for k, c, t in rows:
+        m = merged.setdefault(k, [0, 0.0])
+        m[0] += c
+        m[1] += t
+    total = sum(v[1] for v in merged.values())
+    lines = [f"# rocprofv3 summary: {os.path.basename(prefix)}", "", "Command:", "", "```",
+             command, "```", "", "| kernel | calls | avg us | total ms | share |",
+             "|---|---:|---:|---:|---:|"]
+    for k, (c, t) in sorted(merged.items(), key=lambda kv: -kv[1][1])[:top]:
+        lines.append(f"| `{k}` | {c} | {t / c / 1000:.2f} | {t / 1e6:.3f} | {100 * t / total:.1f}% |")
+    with open(prefix + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("trace_dir")
-    p.add_argument("fetch_dir")
-    p.add_argument("write_dir")
-    p.add_argument("out_prefix")
+    p.add_argument("dirs", nargs="+", help="TRACE FETCH WRITE OUT_PREFIX, or TRACE OUT_PREFIX "
+                                           "with --trace-only")
     p.add_argument("--command", required=True)
+    p.add_argument("--trace-only", action="store_true")
     a = p.parse_args()
-    kernels, fs, ws = summarize(a.trace_dir, a.fetch_dir, a.write_dir)
-    write_outputs(kernels, fs, ws, a.out_prefix, a.command)
+    if a.trace_only:
+        write_trace_only(a.dirs[0], a.dirs[1], a.command)
+        return
+    trace_dir, fetch_dir, write_dir, out_prefix = a.dirs
+    kernels, fs, ws = summarize(trace_dir, fetch_dir, write_dir)
+    write_outputs(kernels, fs, ws, out_prefix, a.command)
 
 
 if __name__ == "__main__":
