@@ -107,6 +107,57 @@ __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__
     dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
 }
 
+// Coarsest-level sweeps in FP64 for at most 256 cells: one cell per thread,
+// its coefficients in registers, the iterate in LDS.  All threads of the
+// workgroup must call it.
+template <bool Z>
+__device__ void coarsest_small(float* __restrict__ v, const float* __restrict__ f, const LevelArgs& a,
+                               int sweeps, double w) {
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    __shared__ double S[2][256];
+    const int i = threadIdx.x;
+    double cf[6] = {0, 0, 0, 0, 0, 0}, diag = 0.0, fi = 0.0;
+    int nb[6] = {-1, -1, -1, -1, -1, -1};
+    if (i < n) {
+        const int y = i % a.ny, x = (i / a.ny) % a.nx, z = i / (a.ny * a.nx);
+        const int sx = a.ny, sz = a.nx * a.ny;
+        const int iay = y + x * (a.ny + 1) + z * a.nx * (a.ny + 1);
+        const int iax = y + x * a.ny + z * (a.nx + 1) * a.ny;
+        // faces in the order of stencil(): y-, y+, x-, x+, z-, z+
+        const bool has[6] = {y > 0 || Z, y < a.ny - 1 || Z, x > 0 || Z, x < a.nx - 1 || Z,
+                             z > 0 || Z, z < a.nz - 1 || Z};
+        const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
+        const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i],
+                             a.az[i + sz]};
+        const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
+                             x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
+                             z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
+        const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
+        const int off[6] = {-1, 1, -sx, sx, -sz, sz};
+        for (int k = 0; k < 6; ++k)
+            if (has[k]) {
+                cf[k] = face_cf<float>(fr[k], hn[k], hh[k]);
+                diag += cf[k];
+                if (in[k]) nb[k] = i + off[k];
+            }
+        fi = f[i];
+        S[0][i] = v[i];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int it = 0; it < sweeps; ++it) {
+        if (i < n) {
+            double o = 0.0;
+            for (int k = 0; k < 6; ++k)
+                if (nb[k] >= 0) o -= cf[k] * S[cur][nb[k]];
+            S[cur ^ 1][i] = diag > 1e-12 ? (1.0 - w) * S[cur][i] + w * (fi - o) / diag : 0.0;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (i < n) v[i] = (float)S[cur][i];
+}
+
 // The coarsest level's sweeps in FP64 (the reference solves it with one block,
 // cuWeightedJacobi3D.cu:1473-1722): one workgroup, FP64 ping-pong in `buf`
 // (2 * cells doubles), starting from and ending in v.
@@ -117,49 +168,7 @@ __global__ void __launch_bounds__(256) coarsest_kernel(float* __restrict__ v,
                                                        double w) {
     const int64_t n = (int64_t)a.nx * a.ny * a.nz;
     if (n <= 256) {
-        // one cell per thread: its coefficients in registers, the iterate in LDS
-        __shared__ double S[2][256];
-        const int i = threadIdx.x;
-        double cf[6] = {0, 0, 0, 0, 0, 0}, diag = 0.0, fi = 0.0;
-        int nb[6] = {-1, -1, -1, -1, -1, -1};
-        if (i < n) {
-            const int y = i % a.ny, x = (i / a.ny) % a.nx, z = i / (a.ny * a.nx);
-            const int sx = a.ny, sz = a.nx * a.ny;
-            const int iay = y + x * (a.ny + 1) + z * a.nx * (a.ny + 1);
-            const int iax = y + x * a.ny + z * (a.nx + 1) * a.ny;
-            // faces in the order of stencil(): y-, y+, x-, x+, z-, z+
-            const bool has[6] = {y > 0 || Z, y < a.ny - 1 || Z, x > 0 || Z, x < a.nx - 1 || Z,
-                                 z > 0 || Z, z < a.nz - 1 || Z};
-            const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
-            const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i],
-                                 a.az[i + sz]};
-            const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
-                                 x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
-                                 z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
-            const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
-            const int off[6] = {-1, 1, -sx, sx, -sz, sz};
-            for (int k = 0; k < 6; ++k)
-                if (has[k]) {
-                    cf[k] = face_cf<float>(fr[k], hn[k], hh[k]);
-                    diag += cf[k];
-                    if (in[k]) nb[k] = i + off[k];
-                }
-            fi = f[i];
-            S[0][i] = v[i];
-        }
-        __syncthreads();
-        int cur = 0;
-        for (int it = 0; it < sweeps; ++it) {
-            if (i < n) {
-                double o = 0.0;
-                for (int k = 0; k < 6; ++k)
-                    if (nb[k] >= 0) o -= cf[k] * S[cur][nb[k]];
-                S[cur ^ 1][i] = diag > 1e-12 ? (1.0 - w) * S[cur][i] + w * (fi - o) / diag : 0.0;
-            }
-            __syncthreads();
-            cur ^= 1;
-        }
-        if (i < n) v[i] = (float)S[cur][i];
+        coarsest_small<Z>(v, f, a, sweeps, w);
         return;
     }
     double* A = buf;
@@ -449,6 +458,149 @@ __global__ void __launch_bounds__(1024) norm_final_kernel(double* __restrict__ o
     }
 }
 
+// ---- the coarse levels of a V-cycle in one workgroup -----------------------
+// Below a few thousand cells a level's launches are latency-bound (a few
+// microseconds each, six per level per cycle).  This kernel runs the V-cycle
+// from level `first` to the coarsest and back in one workgroup of 1024
+// threads, with the same sweep, residual, restriction and prolongation as the
+// per-level kernels (workgroup barriers between steps) and the coarsest
+// level's sweeps in FP64.  The residual of a level goes to its Jacobi partner
+// array, which is free once the smoothing has ended in v.
+struct LevelDev {
+    LevelArgs a;
+    float *f, *v, *t;
+};
+
+template <bool Z>
+__device__ void blk_jacobi(const LevelDev& L, float* __restrict__ dst, const float* __restrict__ src,
+                           float w) {
+    const LevelArgs& a = L.a;
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx), z = (int)(i / ((int64_t)a.ny * a.nx));
+        float diag, off;
+        stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
+        dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (L.f[i] - off) / diag : 0.f;
+    }
+    __syncthreads();
+}
+
+template <bool Z>
+__device__ void blk_smooth(const LevelDev& L, int nu, float w) {
+    float* a = L.v;
+    float* b = L.t;
+    for (int k = 0; k < nu; ++k) {
+        blk_jacobi<Z>(L, b, a, w);
+        float* t = a;
+        a = b;
+        b = t;
+    }
+    if (a != L.v) {
+        const int64_t n = (int64_t)L.a.nx * L.a.ny * L.a.nz;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) L.v[i] = a[i];
+        __syncthreads();
+    }
+}
+
+template <bool Z>
+__global__ void __launch_bounds__(1024) vcycle_coarse_kernel(const LevelDev* __restrict__ lv,
+                                                             int first, int nlev, int nu1, int nu2,
+                                                             float w, double* __restrict__ cbuf,
+                                                             int csweeps) {
+    const int last = nlev - 1;
+    for (int l = first; l < last; ++l) {
+        const LevelDev L = lv[l], C = lv[l + 1];
+        blk_smooth<Z>(L, nu1, w);
+        const LevelArgs& a = L.a;
+        const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx), z = (int)(i / ((int64_t)a.ny * a.nx));
+            float diag, off;
+            stencil<Z, float>(a, [&](int64_t j) { return L.v[j]; }, x, y, z, i, diag, off);
+            L.t[i] = diag > 1e-12f ? L.f[i] - (diag * L.v[i] + off) : 0.f;
+        }
+        __syncthreads();
+        // restriction of the residual (restrict_kernel) and a zero coarse iterate
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const int cx = C.a.nx, cy = C.a.ny, cz = C.a.nz;
+        const int64_t nc = (int64_t)cx * cy * cz;
+        for (int64_t i = threadIdx.x; i < nc; i += blockDim.x) {
+            const int y = (int)(i % cy), x = (int)((i / cy) % cx), z = (int)(i / ((int64_t)cy * cx));
+            const int x0 = 2 * x, y0 = 2 * y, z0 = 2 * z;
+            const int x1 = x0 + 1 < nx ? x0 + 1 : -1, y1 = y0 + 1 < ny ? y0 + 1 : -1,
+                      z1 = z0 + 1 < nz ? z0 + 1 : -1;
+            const int64_t sx = ny, sz = (int64_t)nx * ny;
+            float accz = 0.f, wz = 0.f;
+            for (int kz = 0; kz < 2; ++kz) {
+                const int zz = kz ? z1 : z0;
+                if (zz < 0) continue;
+                float accx = 0.f, wx = 0.f;
+                for (int kx = 0; kx < 2; ++kx) {
+                    const int xx = kx ? x1 : x0;
+                    if (xx < 0) continue;
+                    const int64_t b = xx * sx + zz * sz;
+                    float accy = a.hy[y0] * L.t[b + y0], wy = a.hy[y0];
+                    if (y1 >= 0) {
+                        accy += a.hy[y1] * L.t[b + y1];
+                        wy += a.hy[y1];
+                    }
+                    accx += a.hx[xx] * (accy / wy);
+                    wx += a.hx[xx];
+                }
+                accz += a.hz[zz] * (accx / wx);
+                wz += a.hz[zz];
+            }
+            C.f[i] = accz / wz;
+            C.v[i] = 0.f;
+        }
+        __syncthreads();
+    }
+    {
+        // coarsest level: FP64 sweeps (coarsest_kernel); small levels in LDS,
+        // larger ones in cbuf
+        const LevelDev L = lv[last];
+        const LevelArgs& a = L.a;
+        const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+        if (n <= 256) {
+            coarsest_small<Z>(L.v, L.f, a, csweeps, (double)w);
+        } else {
+            double* A = cbuf;
+            double* B = cbuf + n;
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) A[i] = L.v[i];
+            __syncthreads();
+            for (int it = 0; it < csweeps; ++it) {
+                for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                    const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx),
+                              z = (int)(i / ((int64_t)a.ny * a.nx));
+                    double diag, off;
+                    stencil<Z, double>(a, [&](int64_t j) { return A[j]; }, x, y, z, i, diag, off);
+                    B[i] = diag > 1e-12
+                               ? (1.0 - (double)w) * A[i] + (double)w * ((double)L.f[i] - off) / diag
+                               : 0.0;
+                }
+                __syncthreads();
+                double* t = A;
+                A = B;
+                B = t;
+            }
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) L.v[i] = (float)A[i];
+        }
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= first; --l) {
+        const LevelDev L = lv[l], C = lv[l + 1];
+        const int64_t n = (int64_t)L.a.nx * L.a.ny * L.a.nz;
+        const int cx = C.a.nx, cy = C.a.ny;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int y = (int)(i % L.a.ny), x = (int)((i / L.a.ny) % L.a.nx),
+                      z = (int)(i / ((int64_t)L.a.ny * L.a.nx));
+            L.v[i] += C.v[(y >> 1) + (int64_t)(x >> 1) * cy + (int64_t)(z >> 1) * cx * cy];
+        }
+        __syncthreads();
+        blk_smooth<Z>(L, nu2, w);
+    }
+}
+
 }  // namespace
 }  // namespace whhd
 
@@ -462,6 +614,8 @@ struct whhd_mgfrac3d_s {
     double* host = nullptr;   // pinned host copy of the totals
     double* cbuf = nullptr;   // FP64 ping-pong of the coarsest level (2 * its cells)
     double* fac64 = nullptr;  // finest level's FP64 face factors: ylo, yhi, xlo, xhi, zlo, zhi
+    whhd::LevelDev* dlev = nullptr;  // device copy of the level table
+    int fuse_from = -1;              // levels from here down run in one workgroup
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
     std::vector<void*> owned;
@@ -554,6 +708,17 @@ int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
 
 int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s) {
     const Level& L = h->lv[l];
+    if (h->fuse_from >= 0 && (int)l >= h->fuse_from && l + 1 < h->lv.size()) {
+        const int cs = h->bc == 'z' ? 20 : 50;
+        if (h->bc == 'z')
+            hipLaunchKernelGGL(vcycle_coarse_kernel<true>, dim3(1), dim3(1024), 0, s, h->dlev, (int)l,
+                               (int)h->lv.size(), nu1, nu2, w, h->cbuf, cs);
+        else
+            hipLaunchKernelGGL(vcycle_coarse_kernel<false>, dim3(1), dim3(1024), 0, s, h->dlev, (int)l,
+                               (int)h->lv.size(), nu1, nu2, w, h->cbuf, cs);
+        hipError_t e = launch_status("mg_vcycle_coarse_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg coarse V-cycle kernel");
+    }
     if (l + 1 >= h->lv.size()) return coarsest(h, L, w, s);
     const Level& C = h->lv[l + 1];
     TRY(smooth(h, L, nu1, w, s));
@@ -672,6 +837,20 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         h->fac64 = (double*)alloc((size_t)2 * (nx + ny + nz) * sizeof(double));
         if (!h->r || !h->part || !h->cbuf || !h->rpart || !h->fac64)
             st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+    }
+    if (st == WHHD_OK) {
+        // levels of at most kFuseCells cells (not the finest) run fused
+        constexpr int64_t kFuseCells = 4096;
+        std::vector<LevelDev> dl;
+        for (size_t l = 0; l < h->lv.size(); ++l) {
+            const Level& L = h->lv[l];
+            dl.push_back(LevelDev{args_of(h, L), L.f, L.v, L.t});
+            if (h->fuse_from < 0 && l >= 1 && cells(L.d) <= kFuseCells) h->fuse_from = (int)l;
+        }
+        h->dlev = (LevelDev*)alloc(dl.size() * sizeof(LevelDev));
+        if (!h->dlev ||
+            hipMemcpy(h->dlev, dl.data(), dl.size() * sizeof(LevelDev), hipMemcpyHostToDevice) != hipSuccess)
+            st = fail(WHHD_OOM, "mgfrac3d_create: level table");
     }
     if (st == WHHD_OK) {
         // uniform finest spacing: interior faces 2 / (2 h h), wall faces 2 / ((e + h) h)
