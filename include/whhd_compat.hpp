@@ -11,6 +11,14 @@
 //   CuWHHDForward::Solve1D       WaveletPotentials/cuWHHDForward.cuh:11-23
 //   CuDivLocalProject3D          WaveletPotentials/cuDivLocalProject3D.cuh:7-21
 //   CuGetInnerProduct::GetIp     WaveletPotentials/cuGetInnerProduct.cuh:7-20
+//   CuSolidLevelSet3D            WaveletPotentials/cuSolidLevelSet3D.cuh:8-25
+//   CuFastSweeping::Solve        WaveletPotentials/cuFastSweeping.cuh:6-13
+//   CuMultigridFrac3D            WaveletPotentials/cuMultigridFrac3D.cuh:12-18
+//   CuGradient::Gradient3D_Frac_P, CuConvergence::GetB_3d_frac_vel
+//                                WaveletPotentials/cuGradient.cuh:42-43, cuConvergence.cuh:16-17
+//   CudaMatAdd/MatMul/ArraySum/AbsSum/NormSum/FindMaxValue
+//                                WaveletPotentials/cudaMath.cuh:127-139, 202
+//   CuDst3D, CuParallelSweep3D   WaveletPotentials/cuDst3D.cuh:6-13, cuParallelSweep3D.cuh:9-23
 // Differences, all deliberate:
 //   - DTYPE is float (the reference builds FP64 or FP32 by macro); arrays are
 //     device memory, y fastest, as in the reference.
@@ -209,6 +217,177 @@ public:
 
 private:
     std::map<std::pair<std::string, int>, double*> tables_;
+};
+
+// ---- solid boundary, fractional multigrid and field operators -------------
+
+// Owns the cell-centred level set ls_ (device), as the reference does.
+class CuSolidLevelSet3D {
+public:
+    CuSolidLevelSet3D() = delete;
+    CuSolidLevelSet3D(int3 dim, DTYPE3 dx) : dim_(dim), dx_(dx) {
+        if (hipMalloc(&ls_, sizeof(DTYPE) * (size_t)dim.x * dim.y * dim.z) != hipSuccess)
+            throw Error(WHHD_OOM, "CuSolidLevelSet3D");
+    }
+    CuSolidLevelSet3D(const CuSolidLevelSet3D&) = delete;
+    CuSolidLevelSet3D& operator=(const CuSolidLevelSet3D&) = delete;
+    ~CuSolidLevelSet3D() { (void)hipFree(ls_); }
+    void InitLsSphere(DTYPE3 center, DTYPE radius) { InitLsSphere(ls_, center, radius, dim_, dx_); }
+    void GetFrac(DTYPE* ax, DTYPE* ay, DTYPE* az) { GetFrac(ax, ay, az, ls_, dim_, dx_); }
+    static void InitLsSphere(DTYPE* ls, DTYPE3 center, DTYPE radius, int3 dim, DTYPE3 dx) {
+        const double c[3] = {center.x, center.y, center.z}, r = radius;
+        check(whhd_levelset_spheres(ls, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, c, &r, 1, nullptr),
+              "InitLsSphere");
+    }
+    static void GetFrac(DTYPE* ax, DTYPE* ay, DTYPE* az, DTYPE* ls, int3 dim, DTYPE3 dx) {
+        check(whhd_frac_from_ls(ax, ay, az, ls, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, nullptr),
+              "GetFrac");
+    }
+    DTYPE* ls_ = nullptr;
+
+private:
+    int3 dim_;
+    DTYPE3 dx_;
+};
+
+// Eikonal relaxation; its scratch array is kept between calls.
+class CuFastSweeping {
+public:
+    static CuFastSweeping* GetInstance() {
+        static CuFastSweeping inst;
+        return &inst;
+    }
+    ~CuFastSweeping() { (void)hipFree(scratch_); }
+    static void Solve(DTYPE* ls, int3 dim, DTYPE dx, int n_iters = 5) {
+        CuFastSweeping* s = GetInstance();
+        const size_t n = (size_t)dim.x * dim.y * dim.z;
+        if (n > s->size_) {
+            (void)hipFree(s->scratch_);
+            s->scratch_ = nullptr;
+            s->size_ = 0;
+            if (hipMalloc(&s->scratch_, n * sizeof(DTYPE)) != hipSuccess)
+                throw Error(WHHD_OOM, "CuFastSweeping::Solve");
+            s->size_ = n;
+        }
+        check(whhd_eikonal_sweep(ls, s->scratch_, dim.x, dim.y, dim.z, dx, n_iters, nullptr),
+              "CuFastSweeping::Solve");
+    }
+
+private:
+    DTYPE* scratch_ = nullptr;
+    size_t size_ = 0;
+};
+
+class CuMultigridFrac3D : public Owned<whhd_mgfrac3d_t, whhd_mgfrac3d_destroy> {
+public:
+    CuMultigridFrac3D(DTYPE* Ax, DTYPE* Ay, DTYPE* Az, int3 dim, DTYPE3 dx, char bc = 'n') {
+        check(whhd_mgfrac3d_create(Ax, Ay, Az, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, bc, &h_),
+              "CuMultigridFrac3D");
+    }
+    void Solve(DTYPE* dst, DTYPE* pre_v, DTYPE* f, int n_iters = 10, int nu1 = 2, int nu2 = 2,
+               DTYPE weight = 0.8f) {
+        check(whhd_mgfrac3d_solve(h_, dst, pre_v, f, n_iters, nu1, nu2, weight, stream_), "Solve");
+    }
+    DTYPE GetResdualError(DTYPE* dst, DTYPE* f) {
+        double rel = 0.0;
+        check(whhd_mgfrac3d_residual(h_, dst, f, &rel, stream_), "GetResdualError");
+        return (DTYPE)rel;
+    }
+};
+
+class CuGradient {
+public:
+    static CuGradient* GetInstance() {
+        static CuGradient inst;
+        return &inst;
+    }
+    void SetStream(hipStream_t s) { stream_ = s; }
+    void Gradient3D_Frac_P(DTYPE* xv, DTYPE* yv, DTYPE* zv, DTYPE* p, DTYPE* ax, DTYPE* ay,
+                           DTYPE* az, int3 dim, DTYPE3 dx, char bc = 'n') {
+        check(whhd_gradient3d_frac(xv, yv, zv, p, ax, ay, az, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z,
+                                   bc, stream_),
+              "Gradient3D_Frac_P");
+    }
+
+private:
+    hipStream_t stream_ = nullptr;
+};
+
+class CuConvergence {
+public:
+    static CuConvergence* GetInstance() {
+        static CuConvergence inst;
+        return &inst;
+    }
+    void SetStream(hipStream_t s) { stream_ = s; }
+    // b = -div(a u); bc only selects the reference's overload (the walls are closed)
+    void GetB_3d_frac_vel(DTYPE* b, DTYPE* xv, DTYPE* yv, DTYPE* zv, DTYPE* ax, DTYPE* ay,
+                          DTYPE* az, int3 dim, DTYPE3 dx, char bc = 'n') {
+        (void)bc;
+        check(whhd_divergence3d_frac(b, xv, yv, zv, ax, ay, az, dim.x, dim.y, dim.z, dx.x, dx.y,
+                                     dx.z, stream_),
+              "GetB_3d_frac_vel");
+    }
+
+private:
+    hipStream_t stream_ = nullptr;
+};
+
+inline void CudaMatAdd(DTYPE* dst, DTYPE* src, int size, DTYPE scale = 1.f) {
+    check(whhd_axpy(dst, src, size, scale, nullptr), "CudaMatAdd");
+}
+inline void CudaMatMul(DTYPE* dst, DTYPE* lsrc, DTYPE* rsrc, int size) {
+    check(whhd_mul(dst, lsrc, rsrc, size, nullptr), "CudaMatMul");
+}
+namespace detail {
+// device scratch of the reductions, kept for the process (the reference's
+// CuMemoryManager keeps its buffers too)
+inline DTYPE reduce(const DTYPE* x, int size, int op, const char* what) {
+    static double* scratch = nullptr;
+    if (!scratch && hipMalloc(&scratch, sizeof(double) * whhd_reduce_scratch_doubles()) != hipSuccess)
+        throw Error(WHHD_OOM, what);
+    double out = 0.0;
+    check(whhd_reduce(x, size, op, scratch, &out, nullptr), what);
+    return (DTYPE)out;
+}
+}  // namespace detail
+inline DTYPE CudaArraySum(DTYPE* src, int size) {
+    return detail::reduce(src, size, WHHD_REDUCE_SUM, "CudaArraySum");
+}
+inline DTYPE CudaArrayAbsSum(DTYPE* src, int size) {
+    return detail::reduce(src, size, WHHD_REDUCE_SUM_ABS, "CudaArrayAbsSum");
+}
+inline DTYPE CudaArrayNormSum(DTYPE* src, int size) {
+    return detail::reduce(src, size, WHHD_REDUCE_SUM_SQ, "CudaArrayNormSum");
+}
+inline DTYPE CudaFindMaxValue(DTYPE* src, int size) {
+    return detail::reduce(src, size, WHHD_REDUCE_MAX_ABS, "CudaFindMaxValue");
+}
+
+class CuDst3D : public Owned<whhd_dst3d_t, whhd_dst3d_destroy> {
+public:
+    CuDst3D() = delete;
+    CuDst3D(int3 dim, DTYPE3 dx) {
+        check(whhd_dst3d_create(dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, &h_), "CuDst3D");
+    }
+    void Solve(DTYPE* dst, DTYPE* f) { check(whhd_dst3d_solve(h_, dst, f, stream_), "Solve"); }
+};
+
+// bc 'n' only; the DST solve is direct, so tol is not used.
+class CuParallelSweep3D : public Owned<whhd_sweep3d_t, whhd_sweep3d_destroy> {
+public:
+    CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) {
+        check(whhd_sweep3d_create(dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, bc, &h_),
+              "CuParallelSweep3D");
+    }
+    void Solve(DTYPE* qx, DTYPE* qy, DTYPE* qz, DTYPE* xv, DTYPE* yv, DTYPE* zv) {
+        check(whhd_sweep3d_solve(h_, qx, qy, qz, xv, yv, zv, stream_), "Solve");
+    }
+    void ProjectZero(DTYPE* qx, DTYPE* qy, DTYPE* qz, DTYPE tol = 1e-6f) {
+        (void)tol;
+        check(whhd_sweep3d_project(h_, qx, qy, qz, stream_), "ProjectZero");
+    }
+    void Project(DTYPE* qx, DTYPE* qy, DTYPE* qz, DTYPE tol = 1e-6f) { ProjectZero(qx, qy, qz, tol); }
 };
 
 }  // namespace whhd_compat

@@ -5,6 +5,14 @@
 //   compat_recover3d N DIR   reads DIR/{xv,yv,zv}.f32 (raw FP32, y fastest),
 //                            writes DIR/{qx,qy,qz}.f32 and DIR/solve1d.f32
 //                            (forward CDF3.7 'n' transform of xv along z)
+//   compat_recover3d solids N DIR cx cy cz r
+//                            the solid-obstacle steps of the reference's test
+//                            driver on DIR/{xv,yv,zv}.f32 (sphere at c, radius
+//                            r, unit cube): level set, fractions, f, 10
+//                            V-cycles, gradient correction; writes DIR/ls.f32,
+//                            DIR/f.f32, DIR/p.f32 and prints the residual and
+//                            max |f|; then the sweep + DST comparator on the
+//                            input velocity, writing DIR/{sx,sy,sz}.f32
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -48,7 +56,73 @@ static std::vector<float> to_host(const float* d, size_t n) {
     return h;
 }
 
+static int solids(int argc, char** argv) {
+    if (argc != 8) return 2;
+    const int N = std::atoi(argv[2]);
+    const std::string dir = argv[3];
+    if (N < 4 || N > 256) return 2;
+    const float3 c = make_float3(std::atof(argv[4]), std::atof(argv[5]), std::atof(argv[6]));
+    const float r = std::atof(argv[7]);
+    const size_t nv = (size_t)(N + 1) * N * N, nc = (size_t)N * N * N,
+                 nq = (size_t)(N + 1) * (N + 1) * N;
+    std::vector<float> hx = read_f32(dir + "/xv.f32", nv), hy = read_f32(dir + "/yv.f32", nv),
+                       hz = read_f32(dir + "/zv.f32", nv);
+    float *xv = to_dev(hx), *yv = to_dev(hy), *zv = to_dev(hz);
+    float *ax, *ay, *az, *f, *p, *gx, *gy, *gz, *qx, *qy, *qz;
+    for (float** a : {&ax, &ay, &az, &gx, &gy, &gz})
+        if (hipMalloc(a, nv * 4)) return 3;
+    for (float** a : {&f, &p})
+        if (hipMalloc(a, nc * 4)) return 3;
+    for (float** a : {&qx, &qy, &qz})
+        if (hipMalloc(a, nq * 4)) return 3;
+    std::vector<float> ls;
+    float res = 0.f, fmax0 = 0.f, fmax1 = 0.f;
+    try {
+        const int3 dim = make_int3(N, N, N);
+        const float h = 1.0f / N;
+        const float3 dx = make_float3(h, h, h);
+        CuSolidLevelSet3D sls(dim, dx);
+        sls.InitLsSphere(c, r);
+        sls.GetFrac(ax, ay, az);
+        ls = to_host(sls.ls_, nc);
+        CuConvergence::GetInstance()->GetB_3d_frac_vel(f, xv, yv, zv, ax, ay, az, dim, dx, 'n');
+        fmax0 = CudaFindMaxValue(f, (int)nc);
+        CuMultigridFrac3D mg(ax, ay, az, dim, dx, 'n');
+        mg.Solve(p, nullptr, f, 10, 2, 2, 0.8f);
+        res = mg.GetResdualError(p, f);
+        // u <- a (u - grad p) as the reference's driver does it, then the
+        // divergence of u - grad p again
+        CuGradient::GetInstance()->Gradient3D_Frac_P(gx, gy, gz, p, ax, ay, az, dim, dx, 'n');
+        CudaMatAdd(xv, gx, (int)nv, -1.f);
+        CudaMatAdd(yv, gy, (int)nv, -1.f);
+        CudaMatAdd(zv, gz, (int)nv, -1.f);
+        CuConvergence::GetInstance()->GetB_3d_frac_vel(gx, xv, yv, zv, ax, ay, az, dim, dx, 'n');
+        fmax1 = CudaFindMaxValue(gx, (int)nc);
+        CudaMatMul(xv, xv, ax, (int)nv);
+        // the sweep + DST comparator on the original velocity
+        float *ox = to_dev(hx), *oy = to_dev(hy), *oz = to_dev(hz);
+        CuParallelSweep3D sweep(dim, dx, 'n');
+        sweep.Solve(qx, qy, qz, ox, oy, oz);
+        sweep.ProjectZero(qx, qy, qz);
+        if (hipDeviceSynchronize() != hipSuccess) return 3;
+        for (float* a : {ox, oy, oz}) (void)hipFree(a);
+    } catch (const Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    write_f32(dir + "/ls.f32", ls);
+    write_f32(dir + "/f.f32", to_host(f, nc));
+    write_f32(dir + "/p.f32", to_host(p, nc));
+    write_f32(dir + "/sx.f32", to_host(qx, (size_t)N * (N + 1) * (N + 1)));
+    write_f32(dir + "/sy.f32", to_host(qy, (size_t)(N + 1) * N * (N + 1)));
+    write_f32(dir + "/sz.f32", to_host(qz, nq));
+    for (float* a : {xv, yv, zv, ax, ay, az, f, p, gx, gy, gz, qx, qy, qz}) (void)hipFree(a);
+    std::printf("%.9e %.9e %.9e\n", res, fmax0, fmax1);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "solids") return solids(argc, argv);
     if (argc != 3) {
         std::fprintf(stderr, "usage: %s N DIR\n", argv[0]);
         return 2;
