@@ -368,6 +368,32 @@ WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* q
 WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz, void* stream);
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h);
 
+/* ---- field I/O on host arrays ------------------------------------------ */
+/* The reference's dumps (WaveletPotentials/Utils.cpp:812-990); arrays are host
+ * memory, y fastest.  Text: one row of ny values per (x, z) in memory order,
+ * `digits` after the point, fixed or scientific (WriteDataTxt, WriteDataETxt);
+ * read back with whhd_read_txt (ReadsDataTxt). */
+WHHD_API int whhd_write_txt(const char* path, const float* data, int nx, int ny, int nz,
+                            int digits, int scientific);
+WHHD_API int whhd_read_txt(const char* path, float* data, int nx, int ny, int nz);
+/* "x,y,z,value" lines for |value| >= threshold (WriteDataTxt_sparse) and
+ * "x,y,z,vx,vy,vz" for |v| >= threshold (WriteVectorTxt_sparse). */
+WHHD_API int whhd_write_txt_sparse(const char* path, const float* data, int nx, int ny, int nz,
+                                   double threshold);
+WHHD_API int whhd_write_vector_txt_sparse(const char* path, const float* vx, const float* vy,
+                                          const float* vz, int nx, int ny, int nz,
+                                          double threshold);
+/* Raw FP32 values (WriteDataBinary) and their reader. */
+WHHD_API int whhd_write_binary(const char* path, const float* data, long long n);
+WHHD_API int whhd_read_binary(const char* path, float* data, long long n);
+/* PLY point cloud (no reference counterpart; the reference includes a PLY
+ * library, WaveletPotentials/happly.h, but writes none): a vertex with
+ * property "value" at each cell centre where |value| >= threshold; ASCII or
+ * binary little endian; the vertex count to *written (may be NULL). */
+WHHD_API int whhd_write_ply(const char* path, const float* data, int nx, int ny, int nz, double dx,
+                            double dy, double dz, double threshold, int binary,
+                            long long* written);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,13 @@ STATUS = {0: "WHHD_OK", 1: "WHHD_INVALID_ARG", 2: "WHHD_NOT_POW2", 3: "WHHD_UNSU
 _P, _I, _C, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_char, ctypes.c_double
 _S = ctypes.c_char_p
 SIGNATURES = {
+    "whhd_write_txt": (_I, [_S, _P, _I, _I, _I, _I, _I]),
+    "whhd_read_txt": (_I, [_S, _P, _I, _I, _I]),
+    "whhd_write_txt_sparse": (_I, [_S, _P, _I, _I, _I, _D]),
+    "whhd_write_vector_txt_sparse": (_I, [_S, _P, _P, _P, _I, _I, _I, _D]),
+    "whhd_write_binary": (_I, [_S, _P, ctypes.c_longlong]),
+    "whhd_read_binary": (_I, [_S, _P, ctypes.c_longlong]),
+    "whhd_write_ply": (_I, [_S, _P, _I, _I, _I, _D, _D, _D, _D, _I, ctypes.POINTER(ctypes.c_longlong)]),
     "whhd_status_string": (_S, [_I]),
     "whhd_last_error": (_S, []),
     "whhd_version": (_S, []),

@@ -30,7 +30,8 @@ __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "st
            "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
            "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "CuBatchRecover3D",
-           "UNet3D",
+           "UNet3D", "WriteDataTxt", "WriteDataETxt", "ReadsDataTxt", "WriteDataTxt_sparse",
+           "WriteVectorTxt_sparse", "WriteDataBinary", "ReadDataBinary", "WritePly",
            "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -756,3 +757,84 @@ class CuParallelSweep3D(_Handle):
 
     def Project(self, qx, qy, qz):
         check(lib().whhd_sweep3d_project(self._h, *self._q(qx, qy, qz), _stream()), "Project")
+
+
+# ---- field I/O on host arrays (WaveletPotentials/Utils.cpp:812-990) ------------
+def _host(a, name):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 3:
+        raise ValueError(f"{name}: expected a (nz, nx, ny) array")
+    return a
+
+
+def _dims(a):
+    nz, nx, ny = a.shape
+    return nx, ny, nz
+
+
+def _path(p):
+    return str(p).encode()
+
+
+def WriteDataTxt(path, data, digits=12, scientific=False):
+    """One row of ny values per (x, z), fixed notation with `digits` decimals."""
+    a = _host(data, "data")
+    check(lib().whhd_write_txt(_path(path), a.ctypes.data_as(ctypes.c_void_p), *_dims(a),
+                               int(digits), int(bool(scientific))), "WriteDataTxt")
+
+
+def WriteDataETxt(path, data):
+    """WriteDataTxt in scientific notation with 14 digits."""
+    WriteDataTxt(path, data, 14, True)
+
+
+def ReadsDataTxt(path, dims):
+    """Read an (nx, ny, nz) array written by WriteDataTxt; returns (nz, nx, ny) FP32."""
+    import numpy as np
+    nx, ny, nz = (int(v) for v in dims)
+    out = np.empty((nz, nx, ny), dtype=np.float32)
+    check(lib().whhd_read_txt(_path(path), out.ctypes.data_as(ctypes.c_void_p), nx, ny, nz),
+          "ReadsDataTxt")
+    return out
+
+
+def WriteDataTxt_sparse(path, data, threshold=1e-6):
+    a = _host(data, "data")
+    check(lib().whhd_write_txt_sparse(_path(path), a.ctypes.data_as(ctypes.c_void_p), *_dims(a),
+                                      float(threshold)), "WriteDataTxt_sparse")
+
+
+def WriteVectorTxt_sparse(path, vx, vy, vz, threshold=1e-6):
+    a, b, c = _host(vx, "vx"), _host(vy, "vy"), _host(vz, "vz")
+    if not (a.shape == b.shape == c.shape):
+        raise ValueError("WriteVectorTxt_sparse: the three arrays differ in shape")
+    check(lib().whhd_write_vector_txt_sparse(_path(path), *(t.ctypes.data_as(ctypes.c_void_p)
+                                                            for t in (a, b, c)), *_dims(a),
+                                             float(threshold)), "WriteVectorTxt_sparse")
+
+
+def WriteDataBinary(path, data):
+    import numpy as np
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    check(lib().whhd_write_binary(_path(path), a.ctypes.data_as(ctypes.c_void_p), a.size),
+          "WriteDataBinary")
+
+
+def ReadDataBinary(path, shape):
+    import numpy as np
+    out = np.empty(shape, dtype=np.float32)
+    check(lib().whhd_read_binary(_path(path), out.ctypes.data_as(ctypes.c_void_p), out.size),
+          "ReadDataBinary")
+    return out
+
+
+def WritePly(path, data, dx, threshold=0.0, binary=True):
+    """PLY point cloud of a cell field (vertex at each cell centre with property
+    `value`, where |value| >= threshold).  Returns the vertex count."""
+    a = _host(data, "data")
+    n = ctypes.c_longlong(0)
+    check(lib().whhd_write_ply(_path(path), a.ctypes.data_as(ctypes.c_void_p), *_dims(a),
+                               *(float(v) for v in dx), float(threshold), int(bool(binary)),
+                               ctypes.byref(n)), "WritePly")
+    return n.value
