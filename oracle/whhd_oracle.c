@@ -784,7 +784,8 @@ int orc_project3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, 
                   const double* xv, const double* yv, const double* zv, int NX, int NY, int NZ,
                   double dx, double dy, double dz, int wt_div, char bc_div)
 {
-    if (bc_div != 'd') return ORC_UNSUPPORTED; /* the 'n' (nq_ccc) kernel is not restated */
+    if (bc_div != 'd' && bc_div != 'n') return ORC_UNSUPPORTED;
+    const int nq = bc_div == 'n';
     const int Lx = ilog2(NX), Ly = ilog2(NY), Lz = ilog2(NZ);
     if ((1 << Lx) != NX || (1 << Ly) != NY || (1 << Lz) != NZ) return ORC_INVALID;
     int wt_proj = orc_proj_wavelet(wt_div);
@@ -800,6 +801,9 @@ int orc_project3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, 
     if ((r = gram_diag(ippy, wt_proj, bc_proj, NY))) rc = r;
     if ((r = gram_diag(ippz, wt_proj, bc_proj, NZ))) rc = r;
     const double ix = 1. / dx, iy = 1. / dy, iz = 1. / dz;
+    /* domain lengths NX dx ... and their inverses (L_inv in the reference) */
+    const double lenx = NX * dx, leny = NY * dy, lenz = NZ * dz;
+    const double Lix = 1. / lenx, Liy = 1. / leny, Liz = 1. / lenz;
     const long sxv_x = NY, sxv_z = (long)(NX + 1) * NY;       /* xv: (NX+1, NY, NZ) */
     const long syv_x = NY + 1, syv_z = (long)NX * (NY + 1);   /* yv: (NX, NY+1, NZ) */
     const long szv_x = NY, szv_z = (long)NX * NY;             /* zv: (NX, NY, NZ+1) */
@@ -836,6 +840,93 @@ int orc_project3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, 
                 double fy = vz * sxBxBz - vx * szBxBz;
                 double fz = vx * syBxBy - vy * sxBxBy;
                 double lx = 0., ly = 0., lz = 0.;
+                if (nq) {
+                    /* cuProjectLocal_q_3d_nq_ccc, WaveletPotentials/cuDivLocalProject3D.cu:
+                     * 4474-4877.  Interior as below; on an index-0 plane the two
+                     * index-0 neighbours of the in-plane component along the normal
+                     * axis give two coefficients directly (cuCalculateP0Val_single,
+                     * :156-180, returns them divided by the in-plane scale), the
+                     * remaining in-plane component is corrected by their difference
+                     * times 1/length and projected with the Gram weights.  Lines use
+                     * the direct form twice; the corner integrates with the domain
+                     * lengths. */
+                    if (x > 0 && y > 0 && z > 0) {
+                        double inv = 1. / (sy * sy + sx * sx + sz * sz);
+                        lz = (sy * vx - sx * vy) * inv;
+                        lx = (sz * vy - sy * vz) * inv;
+                        ly = (sx * vz - sz * vx) * inv;
+                    } else if (y > 0 && x > 0 && z == 0) {          /* :4552-4585 */
+                        double z0 = zv[izv - szv_z], z1 = vz;
+                        qx[iqx - sqx_z] = 0.;
+                        qy[iqy - sqy_z] = z0 / sx;
+                        ly = z1 / sx;
+                        double vx2 = vx + (z1 / sx - z0 / sx) * Liz;
+                        lz = (vx2 * syBxBy - vy * sxBxBy) * mz;
+                    } else if (y == 0 && x > 0 && z > 0) {          /* :4586-4611 */
+                        double y0 = yv[iyv - 1], y1 = vy;
+                        qx[iqx - 1] = y0 / sz;
+                        lx = y1 / sz;
+                        double vz2 = vz + (y1 / sz - y0 / sz) * Liy;
+                        ly = (vz2 * sxBxBz - vx * szBxBz) * my;
+                        qz[iqz - 1] = 0.;
+                    } else if (y > 0 && x == 0 && z > 0) {          /* :4612-4637 */
+                        double x0 = xv[ixv - sxv_x], x1 = vx;
+                        qz[iqz - sqz_x] = x0 / sy;
+                        lz = x1 / sy;
+                        double vy2 = vy + (x1 / sy - x0 / sy) * Lix;
+                        lx = (vy2 * szByBz - vz * syByBz) * mx;
+                        qy[iqy - sqy_x] = 0.;
+                    } else if (y == 0 && x > 0 && z == 0) {         /* :4638-4680 */
+                        double y0 = yv[iyv - 1], y1 = vy, z0 = zv[izv - szv_z], z1 = vz;
+                        qy[iqy - sqy_z] = z0 / sx;
+                        ly = z1 / sx;
+                        qz[iqz - 1] = -y0 / sx;
+                        lz = -y1 / sx;
+                        qx[iqx - 1] = 0.;
+                        qx[iqx - sqx_z - 1] = 0.;
+                        qx[iqx - sqx_z] = 0.;
+                    } else if (y > 0 && x == 0 && z == 0) {         /* :4681-4728 */
+                        double z0 = zv[izv - szv_z], z1 = vz, x0 = xv[ixv - sxv_x], x1 = vx;
+                        qx[iqx - sqx_z] = -z0 / sy;
+                        lx = -z1 / sy;
+                        qz[iqz - sqz_x] = x0 / sy;
+                        lz = x1 / sy;
+                        qy[iqy - sqy_x - sqy_z] = 0.;
+                        qy[iqy - sqy_z] = 0.;
+                        qy[iqy - sqy_x] = 0.;
+                    } else if (y == 0 && x == 0 && z > 0) {         /* :4729-4775 */
+                        double y0 = yv[iyv - 1], y1 = vy, x0 = xv[ixv - sxv_x], x1 = vx;
+                        qx[iqx - 1] = y0 / sz;
+                        lx = y1 / sz;
+                        qy[iqy - sqy_x] = -x0 / sz;
+                        ly = -x1 / sz;
+                        qz[iqz - sqz_x - 1] = 0.;
+                        qz[iqz - sqz_x] = 0.;
+                        qz[iqz - 1] = 0.;
+                    } else {                                        /* :4776-4868 */
+                        double y0 = yv[iyv - 1], y1 = vy, x0 = xv[ixv - sxv_x], x1 = vx;
+                        double z0 = zv[izv - szv_z];
+                        qz[iqz - 1 - sqz_x] = 0.;
+                        qz[iqz - 1] = 0.;
+                        qz[iqz - sqz_x] = 0.;
+                        double qx010 = -z0 * leny;
+                        qx[iqx - 1 - sqx_z] = 0.;
+                        qx[iqx - sqx_z] = qx010;
+                        qx[iqx - 1] = y0 * lenz;
+                        lx = qx010 + y1 * lenz;
+                        qy[iqy - sqy_x - sqy_z] = 0.;
+                        qy[iqy - sqy_z] = 0.;
+                        qy[iqy - sqy_x] = -x0 * lenz;
+                        ly = -x1 * lenz;
+                    }
+                    qx[iqx] = lx;
+                    qy[iqy] = ly;
+                    qz[iqz] = lz;
+                    /* the reference's v_out form of this kernel (:5311-5715) leaves
+                     * the transformed velocity as it is */
+                    if (xvo) { xvo[ixv] = vx; yvo[iyv] = vy; zvo[izv] = vz; }
+                    continue;
+                }
                 if (x > 0 && y > 0 && z > 0) {
                     double inv = 1. / (sy * sy + sx * sx + sz * sz);
                     lz = (sy * vx - sx * vy) * inv;

@@ -53,8 +53,9 @@ int orc_emulate_line(double* dst, const double* src, int n, int levels, char bc,
  * reference leaves it unwritten). */
 int orc_gram_table(double* out8n, int wt, char bc, int n);
 
-/* 3D per-cell projection, velocity BC 'n' (bc_div 'd'): cuProjectLocal_q_3d_dq_ccc.
- * N = 2^L cells per axis (possibly different per axis). ip_* are Gram diagonals. */
+/* 3D per-cell projection: bc_div 'd' (velocity BC 'n') cuProjectLocal_q_3d_dq_ccc,
+ * bc_div 'n' (velocity BC 'd') cuProjectLocal_q_3d_nq_ccc.  N = 2^L cells per axis
+ * (possibly different per axis). ip_* are Gram diagonals. */
 int orc_project3d(double* qx, double* qy, double* qz, double* xv_out, double* yv_out,
                   double* zv_out, const double* xv, const double* yv, const double* zv,
                   int nx, int ny, int nz, double dx, double dy, double dz, int wt_div,

@@ -39,9 +39,9 @@ def fill(batch, seed):
             a[key].copy_(torch.from_numpy(v).to(device="cuda", dtype=torch.float32))
 
 
-def make(whhd, dims):
+def make(whhd, dims, bc="n"):
     T = whhd.WaveletType
-    return whhd.CuBatchRecover3D(dims, [tuple(1.0 / v for v in d) for d in dims], "n",
+    return whhd.CuBatchRecover3D(dims, [tuple(1.0 / v for v in d) for d in dims], bc,
                                  T.WT_CDF3_7, T.WT_CDF4_6)
 
 
@@ -49,10 +49,10 @@ def snapshot(batch):
     return [[batch.arrays(i)[k].clone() for k in ("qx", "qy", "qz")] for i in range(len(batch.dims))]
 
 
-def single(whhd, d, a):
+def single(whhd, d, a, bc="n"):
     T = whhd.WaveletType
     dx = tuple(1.0 / v for v in d)
-    rec = whhd.CuWaveletPotentialRecover3D(d, dx, "n", T.WT_CDF3_7, T.WT_CDF4_6)
+    rec = whhd.CuWaveletPotentialRecover3D(d, dx, bc, T.WT_CDF3_7, T.WT_CDF4_6)
     q = [torch.full_like(a[k], float("nan")) for k in ("qx", "qy", "qz")]
     rec.Solve(*q, a["xv"], a["yv"], a["zv"])
     return q
@@ -73,6 +73,20 @@ def test_batch_equals_single_solves(whhd, kernel_mode):
         ref = single(whhd, d, b.arrays(i))
         for g, r in zip(got[i], ref):
             assert torch.isfinite(g).all(), f"problem {i} {d}: coefficient never written"
+            assert torch.equal(g, r), f"problem {i} {d} differs from the single-grid solve"
+
+
+def test_batch_velocity_bc_d_equals_single_solves(whhd):
+    """Velocity bc 'd' runs the windowed transforms and the divergence bc 'n'
+    projection per size class; still bit for bit the single-grid solves."""
+    dims = [(32, 32, 32), (16, 16, 16), (32, 32, 32), (16, 32, 8)]
+    b = make(whhd, dims, "d")
+    fill(b, 3)
+    b.Solve()
+    got = snapshot(b)
+    for i, d in enumerate(dims):
+        ref = single(whhd, d, b.arrays(i), "d")
+        for g, r in zip(got[i], ref):
             assert torch.equal(g, r), f"problem {i} {d} differs from the single-grid solve"
 
 

@@ -97,6 +97,78 @@ def test_project3d_matches_oracle(whhd, orc):
         np.testing.assert_allclose(g, r, rtol=2e-5, atol=2e-6 * np.abs(r).max())
 
 
+def free_potential(n, seed):
+    """Edge potential with no boundary constraint (smooth modes plus noise)."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    out = []
+    for sh in [(nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny), (nz, nx + 1, ny + 1)]:
+        z, x, y = np.meshgrid(*[np.linspace(0, 1, k) for k in sh], indexing="ij")
+        a, b, c, d = rng.uniform(0.5, 3.0, 4)
+        out.append(np.sin(a * x + b * y + c * z + d) + 0.1 * rng.standard_normal(sh))
+    return out
+
+
+@pytest.mark.parametrize("n,dx", [((8, 8, 8), (1 / 8,) * 3), ((16, 8, 32), (0.05, 0.09, 0.02)),
+                                  ((32, 32, 32), (1 / 32,) * 3), ((64, 32, 64), (1 / 64,) * 3)])
+def test_recover3d_velocity_bc_d_matches_oracle(whhd, orc, kernel_mode, n, dx):
+    """Velocity bc 'd' (divergence bc 'n', cuProjectLocal_q_3d_nq_ccc): the curl of
+    an unconstrained potential, recovered on the GPU, matches the FP64 oracle and
+    reproduces the velocity."""
+    xv, yv, zv = I.curl3d(*free_potential(n, n[0] + 7), dx)
+    ref = orc.recover3d(xv, yv, zv, n, dx, bc="d")
+    T = whhd.WaveletType
+    rec = whhd.CuWaveletPotentialRecover3D(n, dx, "d", T.WT_CDF3_7, T.WT_CDF4_6)
+    qx, qy, qz = alloc_q(n)
+    rec.Solve(qx, qy, qz, cuda(xv), cuda(yv), cuda(zv))
+    got = host(qx, qy, qz)
+    assert all(np.isfinite(g).all() for g in got), "some potential coefficient was never written"
+    assert rel(got, ref) < TOL_Q
+    c = orc.curl3d(*got, n, dx)
+    assert rel(list(c), [xv, yv, zv]) < TOL_Q
+
+
+def test_recover3d_velocity_bc_d_vout(whhd, orc):
+    """The v_out form for velocity bc 'd' returns the velocity through the forward
+    and inverse transforms unchanged, as the oracle does."""
+    N = 16
+    n = (N, N, N)
+    dx = (1.0 / N,) * 3
+    rng = np.random.default_rng(3)
+    v = [rng.standard_normal(s) for s in ((N, N + 1, N), (N, N, N + 1), (N + 1, N, N))]
+    ref = orc.recover3d(*v, n, dx, bc="d", with_vout=True)
+    T = whhd.WaveletType
+    rec = whhd.CuWaveletPotentialRecover3D(n, dx, "d", T.WT_CDF3_7, T.WT_CDF4_6)
+    qx, qy, qz = alloc_q(n)
+    xo, yo, zo = (torch.empty_like(cuda(a)) for a in v)
+    rec.Solve(qx, qy, qz, xo, yo, zo, *(cuda(a) for a in v))
+    got = host(qx, qy, qz, xo, yo, zo)
+    assert rel(got[:3], ref[:3]) < TOL_Q
+    assert rel(got[3:], ref[3:]) < TOL_Q
+    assert rel(got[3:], v) < TOL_Q
+
+
+def test_project3d_bc_n_matches_oracle(whhd, orc):
+    n = (16, 8, 32)
+    nx, ny, nz = n
+    dx = (0.05, 0.09, 0.02)
+    rng = np.random.default_rng(8)
+    xv = rng.standard_normal((nz, nx + 1, ny))
+    yv = rng.standard_normal((nz, nx, ny + 1))
+    zv = rng.standard_normal((nz + 1, nx, ny))
+    ref = orc.project3d(xv, yv, zv, n, dx, bc_div="n", with_vout=True)
+    T = whhd.WaveletType
+    lv = (4, 3, 5)
+    p = whhd.CuDivLocalProject3D((nx + 1, ny + 1, nz + 1), lv, dx, T.WT_CDF4_6, "n")
+    qx, qy, qz = alloc_q(n)
+    xo, yo, zo = cuda(xv), cuda(yv), cuda(zv)
+    p.ProjectLocal_q_ccc(qx, qy, qz, xo, yo, zo, cuda(xv), cuda(yv), cuda(zv), lv)
+    got = host(qx, qy, qz, xo, yo, zo)
+    assert all(np.isfinite(g).all() for g in got[:3])
+    for g, r in zip(got, ref):
+        np.testing.assert_allclose(g, r, rtol=2e-5, atol=2e-6 * np.abs(r).max())
+
+
 def test_cut_matches_oracle(whhd, orc, kernel_mode):
     N = 16
     n = (N, N, N)

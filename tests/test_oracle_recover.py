@@ -98,3 +98,65 @@ def test_cut_level_zero_is_identity(orc):
     q = orc.recover3d(xv, yv, zv, n, dx)
     c = orc.cut3d(*q, n, 0)
     assert orc.rel_l2(list(q), list(c)) < 1e-12
+
+
+def _free_potential(n, seed):
+    """An edge potential with no boundary constraint: smooth random modes plus
+    noise, every boundary component left as it is."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    shapes = [(nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny), (nz, nx + 1, ny + 1)]
+    out = []
+    for sh in shapes:
+        z, x, y = np.meshgrid(*[np.linspace(0, 1, k) for k in sh], indexing="ij")
+        a, b, c, d = rng.uniform(0.5, 3.0, 4)
+        out.append(np.sin(a * x + b * y + c * z + d) + 0.1 * rng.standard_normal(sh))
+    return out
+
+
+@pytest.mark.parametrize("n,dx", [((16, 16, 16), (1 / 16,) * 3),
+                                  ((16, 8, 32), (0.05, 0.09, 0.02)),
+                                  ((8, 32, 16), (0.1, 0.01, 0.03))])
+def test_recover3d_velocity_bc_d_reproduces_any_curl(orc, n, dx):
+    """Velocity bc 'd' (divergence bc 'n', cuProjectLocal_q_3d_nq_ccc): the normal
+    velocity is free at the walls, so the curl of ANY edge potential is
+    representable, and the recovered potential's curl reproduces it to roundoff.
+    Every index-0 plane, line and corner branch contributes; on unequal domain
+    lengths a swapped length in the corner branch, a dropped correction on a
+    plane or a sign or scale slip on a line each leave errors of 1e-3 or more."""
+    q0 = _free_potential(n, 11)
+    u = I.curl3d(*q0, dx)
+    r = orc.recover3d(*u, n, dx, bc="d")
+    c = orc.curl3d(*r, n, dx)
+    assert orc.rel_l2(list(u), list(c)) < 1e-10
+
+
+def test_recover3d_velocity_bc_d_zero_flux_input(orc):
+    """The zero-flux velocities of the bc 'n' workload are also bc 'd' velocities."""
+    n = (16, 8, 32)
+    dx = (1 / 16, 1 / 8, 1 / 32)
+    q = I.spectral_potential3d(n, 3, 5)
+    u = I.curl3d(*q, dx)
+    r = orc.recover3d(*u, n, dx, bc="d")
+    assert orc.rel_l2(list(u), list(orc.curl3d(*r, n, dx))) < 1e-10
+
+
+def test_velocity_bc_n_cannot_represent_wall_flux(orc):
+    """The same free potential recovered with velocity bc 'n' (walls closed) does
+    not reproduce its curl: the two kernels differ exactly in the boundary rows."""
+    n = (16, 8, 32)
+    dx = (0.05, 0.09, 0.02)
+    u = I.curl3d(*_free_potential(n, 11), dx)
+    r = orc.recover3d(*u, n, dx, bc="n")
+    assert orc.rel_l2(list(u), list(orc.curl3d(*r, n, dx))) > 1e-2
+
+
+def test_project3d_bc_n_vout_is_input(orc):
+    """The v_out form of the divergence bc 'n' kernel leaves the velocity as read."""
+    N = 8
+    n = (N, N, N)
+    rng = np.random.default_rng(4)
+    v = [rng.standard_normal(s) for s in ((N, N + 1, N), (N, N, N + 1), (N + 1, N, N))]
+    out = orc.project3d(*v, n, (1 / N,) * 3, bc_div="n", with_vout=True)
+    for a, b in zip(out[3:], v):
+        assert np.array_equal(a, b)

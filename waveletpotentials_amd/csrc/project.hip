@@ -1,12 +1,23 @@
 // Per-cell local projection in wavelet space.
 //
-// 3D: cuProjectLocal_q_3d_dq_ccc (WaveletPotentials/cuDivLocalProject3D.cu:
-// 4013-4246; 9-argument form with v_out = s x q :5717-5965).  For cell
-// (x, y, z) the kernel reads xv(x+1,y,z), yv(x,y+1,z), zv(x,y,z+1) and writes
-// qx(x,y+1,z+1), qy(x+1,y,z+1), qz(x+1,y+1,z): interior cells get
-// q = (s x v)/|s|^2, cells on one index-0 plane get one Gram-weighted
-// component, and the index-0 lines and corner write zeros, covering every
-// potential coefficient exactly once.
+// 3D, divergence bc 'd' (velocity bc 'n'): cuProjectLocal_q_3d_dq_ccc
+// (WaveletPotentials/cuDivLocalProject3D.cu:4013-4246; 9-argument form with
+// v_out = s x q :5717-5965).  For cell (x, y, z) the kernel reads xv(x+1,y,z),
+// yv(x,y+1,z), zv(x,y,z+1) and writes qx(x,y+1,z+1), qy(x+1,y,z+1),
+// qz(x+1,y+1,z): interior cells get q = (s x v)/|s|^2, cells on one index-0
+// plane get one Gram-weighted component, and the index-0 lines and corner
+// write zeros, covering every potential coefficient exactly once.
+//
+// 3D, divergence bc 'n' (velocity bc 'd'): cuProjectLocal_q_3d_nq_ccc
+// (cuDivLocalProject3D.cu:4474-4877).  Same interior and the same coverage; a
+// cell on an index-0 plane also reads the index-0 sample of the in-plane
+// component along the plane's normal (xv(0,y,z), yv(x,0,z) or zv(x,y,0)),
+// whose two samples give two coefficients directly (divided by the in-plane
+// scale, cuCalculateP0Val_single :156-180); the remaining component is
+// corrected by their difference over the domain length and projected with the
+// Gram weights.  Lines take the direct form twice, the corner integrates with
+// the domain lengths.  Its v_out form (:5311-5715) leaves the transformed
+// velocity unchanged, so v_out receives the velocity as read.
 //
 // 2D: cuProjectLocal_q_single (WaveletPotentials/cuDivLocalProject.cu:765-1045):
 // node (X, Y) with X, Y > 1 gets
@@ -55,6 +66,9 @@ struct Proj3Args {
     int nx, ny, nz;
     int zblocks;  // grid z rows per problem
     int64_t bv[3], bq[3];
+    int nq;                       // divergence bc 'n' boundary cells
+    float lix, liy, liz;          // 1 / domain length per axis
+    float lenx, leny, lenz;       // domain length per axis
 };
 
 // One thread per (x, y) column walking ZB consecutive z cells; offsets in
@@ -105,7 +119,84 @@ __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
             const float ip0y = p.cy.b0[y], ip1y = p.cy.b1[y];
             const float ip0z = p.cz.b0[z], ip1z = p.cz.b1[z];
             const float eps = (float)kEps;
-            if (y > 0 && z > 0 && x == 0) {
+            if (p.nq) {
+                // index-0 samples of the in-plane components (read only where used)
+                if (x > 0 && y > 0 && z == 0) {
+                    const float z0 = p.zv[izv - szv_z];
+                    const float sxBxBy = sx * ip0x * ip1y * ip0z;
+                    const float syBxBy = sy * ip1x * ip0y * ip0z;
+                    const float mz = 1.f / (sx * sxBxBy + sy * syBxBy + eps);
+                    const float a0 = z0 / sx, a1 = vz / sx;
+                    p.qx[iqx - sqx_z] = 0.f;
+                    p.qy[iqy - sqy_z] = a0;
+                    ly = a1;
+                    const float vx2 = vx + (a1 - a0) * p.liz;
+                    lz = (vx2 * syBxBy - vy * sxBxBy) * mz;
+                } else if (x > 0 && z > 0 && y == 0) {
+                    const float y0 = p.yv[iyv - 1];
+                    const float sxBxBz = sx * ip0x * ip0y * ip1z;
+                    const float szBxBz = sz * ip1x * ip0y * ip0z;
+                    const float my = 1.f / (sx * sxBxBz + sz * szBxBz + eps);
+                    const float a0 = y0 / sz, a1 = vy / sz;
+                    p.qx[iqx - 1] = a0;
+                    lx = a1;
+                    const float vz2 = vz + (a1 - a0) * p.liy;
+                    ly = (vz2 * sxBxBz - vx * szBxBz) * my;
+                    p.qz[iqz - 1] = 0.f;
+                } else if (y > 0 && z > 0 && x == 0) {
+                    const float x0 = p.xv[ixv - NY];
+                    const float syByBz = sy * ip0x * ip0y * ip1z;
+                    const float szByBz = sz * ip0x * ip1y * ip0z;
+                    const float mx = 1.f / (sy * syByBz + sz * szByBz + eps);
+                    const float a0 = x0 / sy, a1 = vx / sy;
+                    p.qz[iqz - sqz_x] = a0;
+                    lz = a1;
+                    const float vy2 = vy + (a1 - a0) * p.lix;
+                    lx = (vy2 * szByBz - vz * syByBz) * mx;
+                    p.qy[iqy - sqy_x] = 0.f;
+                } else if (x > 0 && y == 0 && z == 0) {
+                    const float y0 = p.yv[iyv - 1], z0 = p.zv[izv - szv_z];
+                    p.qy[iqy - sqy_z] = z0 / sx;
+                    ly = vz / sx;
+                    p.qz[iqz - 1] = -y0 / sx;
+                    lz = -vy / sx;
+                    p.qx[iqx - 1] = 0.f;
+                    p.qx[iqx - sqx_z - 1] = 0.f;
+                    p.qx[iqx - sqx_z] = 0.f;
+                } else if (y > 0 && x == 0 && z == 0) {
+                    const float z0 = p.zv[izv - szv_z], x0 = p.xv[ixv - NY];
+                    p.qx[iqx - sqx_z] = -z0 / sy;
+                    lx = -vz / sy;
+                    p.qz[iqz - sqz_x] = x0 / sy;
+                    lz = vx / sy;
+                    p.qy[iqy - sqy_x - sqy_z] = 0.f;
+                    p.qy[iqy - sqy_z] = 0.f;
+                    p.qy[iqy - sqy_x] = 0.f;
+                } else if (z > 0 && x == 0 && y == 0) {
+                    const float y0 = p.yv[iyv - 1], x0 = p.xv[ixv - NY];
+                    p.qx[iqx - 1] = y0 / sz;
+                    lx = vy / sz;
+                    p.qy[iqy - sqy_x] = -x0 / sz;
+                    ly = -vx / sz;
+                    p.qz[iqz - sqz_x - 1] = 0.f;
+                    p.qz[iqz - sqz_x] = 0.f;
+                    p.qz[iqz - 1] = 0.f;
+                } else {
+                    const float y0 = p.yv[iyv - 1], x0 = p.xv[ixv - NY], z0 = p.zv[izv - szv_z];
+                    p.qz[iqz - 1 - sqz_x] = 0.f;
+                    p.qz[iqz - 1] = 0.f;
+                    p.qz[iqz - sqz_x] = 0.f;
+                    const float q010 = -z0 * p.leny;
+                    p.qx[iqx - 1 - sqx_z] = 0.f;
+                    p.qx[iqx - sqx_z] = q010;
+                    p.qx[iqx - 1] = y0 * p.lenz;
+                    lx = q010 + vy * p.lenz;
+                    p.qy[iqy - sqy_x - sqy_z] = 0.f;
+                    p.qy[iqy - sqy_z] = 0.f;
+                    p.qy[iqy - sqy_x] = -x0 * p.lenz;
+                    ly = -vx * p.lenz;
+                }
+            } else if (y > 0 && z > 0 && x == 0) {
                 const float syByBz = sy * ip0x * ip0y * ip1z;
                 const float szByBz = sz * ip0x * ip1y * ip0z;
                 const float mx = 1.f / (sy * syByBz + sz * szByBz + eps);
@@ -160,9 +251,15 @@ __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
         p.qy[iqy] = ly;
         p.qz[iqz] = lz;
         if (p.xo) {
-            p.xo[ixv] = sy * lz - sz * ly;
-            p.yo[iyv] = sz * lx - sx * lz;
-            p.zo[izv] = sx * ly - sy * lx;
+            if (p.nq) {
+                p.xo[ixv] = vx;
+                p.yo[iyv] = vy;
+                p.zo[izv] = vz;
+            } else {
+                p.xo[ixv] = sy * lz - sz * ly;
+                p.yo[iyv] = sz * lx - sx * lz;
+                p.zo[izv] = sx * ly - sy * lx;
+            }
         }
         ixv += sxv_z; iyv += syv_z; izv += szv_z;
         iqx += sqx_z; iqy += sqy_z; iqz += sqz_z;
@@ -241,6 +338,8 @@ static int axis_coef(std::vector<float>& rows, int N, double h, int wt_div, char
 
 struct whhd_project3d_s {
     int nx, ny, nz;
+    char bc;                // divergence bc
+    double len[3];          // domain lengths nx dx, ny dy, nz dz
     float* coef = nullptr;  // 3 axes x 3 rows x (n+1)
     whhd::AxisCoef cx, cy, cz;
 };
@@ -262,9 +361,9 @@ int project3d_create(int nx, int ny, int nz, double dx, double dy, double dz, in
     if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))
         return fail(WHHD_NOT_POW2, "project3d: cell counts must be powers of two (>= 4)");
     if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "project3d: spacing must be > 0");
-    if (bc != 'd')
+    if (bc != 'd' && bc != 'n')
         return fail(WHHD_UNSUPPORTED_WAVELET,
-                    "project3d: only the divergence bc 'd' (velocity bc 'n') is implemented");
+                    "project3d: the divergence bc must be 'd' (velocity bc 'n') or 'n' (velocity bc 'd')");
     int st = check_wavelet(wt, bc);
     if (st != WHHD_OK) return st;
     if (proj_wavelet(wt) < 0) return fail(WHHD_UNSUPPORTED_WAVELET, "project3d: no projection type");
@@ -274,6 +373,8 @@ int project3d_create(int nx, int ny, int nz, double dx, double dy, double dz, in
     if ((st = axis_coef(rz, nz, dz, wt, bc, true)) != WHHD_OK) return st;
     auto* h = new whhd_project3d_s;
     h->nx = nx; h->ny = ny; h->nz = nz;
+    h->bc = bc;
+    h->len[0] = nx * dx; h->len[1] = ny * dy; h->len[2] = nz * dz;
     const size_t tot = rx.size() + ry.size() + rz.size();
     if (hipMalloc(&h->coef, tot * sizeof(float)) != hipSuccess) {
         delete h;
@@ -321,7 +422,10 @@ int project3d_run(const whhd_project3d_s* h, float* qx, float* qy, float* qz, fl
     Proj3Args a{qx, qy, qz, xo, yo, zo, xv, yv, zv, h->cx, h->cy, h->cz, h->nx, h->ny, h->nz,
                 (h->nz + zb - 1) / zb,
                 {(X + 1) * Y * Z, X * (Y + 1) * Z, X * Y * (Z + 1)},
-                {X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z}};
+                {X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z},
+                h->bc == 'n',
+                (float)(1.0 / h->len[0]), (float)(1.0 / h->len[1]), (float)(1.0 / h->len[2]),
+                (float)h->len[0], (float)h->len[1], (float)h->len[2]};
     if (nb < 1 || (int64_t)a.zblocks * nb > 65535)
         return fail(WHHD_INVALID_ARG, "project3d_run: batch too large for one launch");
     const bool wide = (X + 1) * (Y + 1) * (Z + 1) >= ((int64_t)1 << 31);
