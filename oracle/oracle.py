@@ -76,6 +76,8 @@ def _setup(L):
     L.orc_curl3d.restype = None
     L.orc_curl2d.restype = None
     L.orc_div3d.restype = None
+    L.orc_div_l0_matrix.argtypes = [_D, i, i]
+    L.orc_div_l0_matrix.restype = None
     L.orc_wavelet_supported.argtypes = [i, c]
     L.orc_wavelet_halo.argtypes = [i]
     L.orc_proj_wavelet.argtypes = [i]
@@ -153,6 +155,13 @@ def project2d(xv, yv, n, dx, wt_div="cdf4.6", bc_div="d"):
     _check(lib().orc_project2d(_p(qz), _p(xv), _p(yv), nx, ny, dx[0], dx[1], _wt(wt_div),
                                _b(bc_div)), "project2d")
     return qz
+
+
+def div_l0_matrix(lx, ly):
+    """GetDivL0Matrix for level counts (lx, ly): 16 entries."""
+    out = np.zeros(16)
+    lib().orc_div_l0_matrix(_p(out), int(lx), int(ly))
+    return out
 
 
 def recover3d(xv, yv, zv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6", with_vout=False):

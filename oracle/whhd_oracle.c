@@ -996,10 +996,117 @@ int orc_project3d(double* qx, double* qy, double* qz, double* xvo, double* yvo, 
 /* The kernel evaluates the same node formula once per tile orientation and   */
 /* adds the halves; the neighbour terms it computes are not added.            */
 /* ------------------------------------------------------------------------- */
+/* GetDivL0Matrix (WaveletPotentials/DivL0Matrix.cpp:3-50): the corner 2x2
+ * block's four entries by the difference of the two level counts, mirrored
+ * when the y axis has more levels, expanded to the 4x4 sign pattern.  The row
+ * for a difference of 5 initialises only its first entry (the four values are
+ * summed there, DivL0Matrix.cpp:27); kept as the reference has it. */
+void orc_div_l0_matrix(double* dst, int lx, int ly)
+{
+    static const double rows[9][4] = {
+        {-0.374999999999990, -0.124999999999991, -0.375000000000009, -0.125000000000009},
+        {-0.600000000000000, -0.400000000000000, -0.450000000000000, -0.050000000000000},
+        {-1.058823529411764, -0.941176470588235, -0.485294117647059, -0.014705882352941},
+        {-2.030769230769231, -1.969230769230771, -0.496153846153847, -0.003846153846153},
+        {-4.015564202334636, -3.984435797665360, -0.499027237354084, -0.000972762645915},
+        {-8.007804878048813 - 7.992195121951190 - 0.499756097560973 - 0.000243902439025, 0., 0., 0.},
+        {-16.003905296558496, -15.996094703441514, -0.499938979741274, -0.000061020258726},
+        {-32.001953005797915, -31.998046994202070, -0.499984742142203, -0.000015257857796},
+        {-64.000976547601056, -63.999023452398980, -0.499996185360941, -0.000003814639067}};
+    int m = lx < ly ? lx : ly;
+    int ex = lx - m, ey = ly - m;
+    int d = ex > ey ? ex : ey;
+    const double* r = rows[d <= 8 ? d : 0];
+    double a = r[0], b = r[1], c = r[2], e = r[3];
+    if (ex < ey) { double t = a; a = c; c = t; t = b; b = e; e = t; }
+    const double v[16] = {a, b, c, e, -a, -b, e, c, b, a, -c, -e, -b, -a, -e, -c};
+    memcpy(dst, v, sizeof v);
+}
+
+/* cuProjectLocal_q_single_n (WaveletPotentials/cuDivLocalProject.cu:1047-1478),
+ * divergence bc 'n' (velocity bc 'd').  The kernel sums an x-oriented and a
+ * y-oriented pass.  Interior nodes (X, Y > 1) get half the closed form in each.
+ * Columns X = 0, 1 (Y > 1) get, in the x pass, one unknown of the 2x2 system
+ * coupling the two x scaling functions (Gram entries <b0,b0>, <b0,b1> of the
+ * divergence basis, scale 1/length); rows Y = 0, 1 (X > 1) likewise in the y
+ * pass; the four corner nodes come from the level-difference table applied to
+ * xv(0,1), xv(1,1), yv(1,0), yv(1,1).  xv/yv here are node-indexed: xv(X, Y)
+ * is xv[(Y-1) + X NY] (0 at Y = 0), yv(X, Y) is yv[Y + (X-1)(NY+1)] (0 at X = 0). */
+static int project2d_n(double* qz, const double* xv, const double* yv, int NX, int NY,
+                       double dx, double dy, int wt_div)
+{
+    const int Lx = ilog2(NX), Ly = ilog2(NY);
+    int wt_proj = orc_proj_wavelet(wt_div);
+    double *gbx = malloc(sizeof(double) * 8 * (NX + 1)), *gby = malloc(sizeof(double) * 8 * (NY + 1)),
+           *ippx = malloc(sizeof(double) * NX), *ippy = malloc(sizeof(double) * NY);
+    int rc = ORC_OK, r;
+    if ((r = orc_gram_table(gbx, wt_div, 'n', NX + 1))) rc = r;
+    if ((r = orc_gram_table(gby, wt_div, 'n', NY + 1))) rc = r;
+    if ((r = gram_diag(ippx, wt_proj, orc_proj_bc('n'), NX))) rc = r;
+    if ((r = gram_diag(ippy, wt_proj, orc_proj_bc('n'), NY))) rc = r;
+    const double* ipbx = gbx;
+    const double* ipby = gby;
+    const double bx00 = gbx[0], bx01 = gbx[3 * (NX + 1)];  /* rows 0 and 3 (<b_i, b_i+1>) */
+    const double by00 = gby[0], by01 = gby[3 * (NY + 1)];
+    double D[16];
+    orc_div_l0_matrix(D, Lx, Ly);
+    const double ix = 1. / dx, iy = 1. / dy;
+    const double sxL = ix / NX, syL = iy / NY;
+#define XV(X, Y) ((Y) >= 1 ? xv[((Y) - 1) + (long)(X) * NY] : 0.)
+#define YV(X, Y) ((X) >= 1 ? yv[(Y) + (long)((X) - 1) * (NY + 1)] : 0.)
+    for (int X = 0; X <= NX; ++X)
+        for (int Y = 0; Y <= NY; ++Y) {
+            int jy = Ly - bitlen(Y > 0 ? Y - 1 : 0) + 1, jx = Lx - bitlen(X > 0 ? X - 1 : 0) + 1;
+            double sy = 4. * iy / ((double)(1 << jy) + EPS);
+            double sx = 4. * ix / ((double)(1 << jx) + EPS);
+            double h1 = 0., h2 = 0.;
+            if (X > 1 && Y > 1) {                                   /* :1148-1205, :1356-1398 */
+                double ip0x = ippx[X - 1], ip1x = ipbx[X], ip0y = ippy[Y - 1], ip1y = ipby[Y];
+                double mid = sx * sx * ip0x * ip1y + sy * sy * ip1x * ip0y + EPS;
+                double P = (XV(X, Y) * sy * ip1x * ip0y - YV(X, Y) * sx * ip0x * ip1y) / mid;
+                h1 = 0.5 * P;
+                h2 = 0.5 * P;
+            } else if (Y > 1) {                                     /* :1214-1268, X = 0, 1 */
+                double ip0x = ippx[0], ip0y = ippy[Y - 1], ip1y = ipby[Y];
+                double xv0 = XV(0, Y), xv1 = XV(1, Y), v = YV(1, Y);
+                double p0 = sxL * sxL * ip0x * ip1y + sy * sy * bx00 * ip0y;
+                double p1 = -sxL * sxL * ip0x * ip1y + sy * sy * bx01 * ip0y;
+                double den = p0 * p0 - p1 * p1 + EPS;
+                double m0 = p0 / den, m1 = -p1 / den;
+                double f1 = v * sxL * ip0x * ip1y + xv0 * sy * bx00 * ip0y + xv1 * sy * bx01 * ip0y;
+                double f2 = -v * sxL * ip0x * ip1y + xv0 * sy * bx01 * ip0y + xv1 * sy * bx00 * ip0y;
+                h1 = X == 0 ? m0 * f1 + m1 * f2 : m1 * f1 + m0 * f2;
+            } else if (X > 1) {                                     /* :1400-1431, Y = 0, 1 */
+                double ip0mx = ippx[X - 1], ip1mx = ipbx[X], ip0y = ippy[0];
+                double v = XV(X, 1), yv0 = YV(X, 0), yv1 = YV(X, 1);
+                double p0 = sx * sx * ip0mx * by00 + syL * syL * ip1mx * ip0y;
+                double p1 = sx * sx * ip0mx * by01 - syL * syL * ip1mx * ip0y;
+                double den = p0 * p0 - p1 * p1 + EPS;
+                double m0 = p0 / den, m1 = -p1 / den;
+                double f1 = -v * syL * ip1mx * ip0y - yv0 * sx * ip0mx * by00 - yv1 * sx * ip0mx * by01;
+                double f2 = v * syL * ip1mx * ip0y - yv0 * sx * ip0mx * by01 - yv1 * sx * ip0mx * by00;
+                h2 = Y == 0 ? m0 * f1 + m1 * f2 : m1 * f1 + m0 * f2;
+            } else {                                                /* :1432-1451, corner */
+                const double* a = D + (2 * Y + X) * 4;
+                h2 = a[2] * XV(0, 1) + a[3] * XV(1, 1) - a[0] * YV(1, 0) - a[1] * YV(1, 1);
+            }
+            qz[Y + (long)X * (NY + 1)] = h1 + h2;
+        }
+#undef XV
+#undef YV
+    free(gbx); free(gby); free(ippx); free(ippy);
+    return rc;
+}
+
 int orc_project2d(double* qz, const double* xv, const double* yv, int NX, int NY, double dx,
                   double dy, int wt_div, char bc_div)
 {
-    if (bc_div != 'd') return ORC_UNSUPPORTED; /* _single_n is not restated */
+    if (bc_div == 'n') {
+        const int Lx = ilog2(NX), Ly = ilog2(NY);
+        if ((1 << Lx) != NX || (1 << Ly) != NY) return ORC_INVALID;
+        return project2d_n(qz, xv, yv, NX, NY, dx, dy, wt_div);
+    }
+    if (bc_div != 'd') return ORC_UNSUPPORTED;
     const int Lx = ilog2(NX), Ly = ilog2(NY);
     if ((1 << Lx) != NX || (1 << Ly) != NY) return ORC_INVALID;
     int wt_proj = orc_proj_wavelet(wt_div);

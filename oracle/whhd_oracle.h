@@ -61,7 +61,8 @@ int orc_project3d(double* qx, double* qy, double* qz, double* xv_out, double* yv
                   int nx, int ny, int nz, double dx, double dy, double dz, int wt_div,
                   char bc_div);
 
-/* 2D node projection cuProjectLocal_q_single (bc_base 'd'). */
+/* 2D node projection: bc_div 'd' cuProjectLocal_q_single, bc_div 'n'
+ * cuProjectLocal_q_single_n. */
 int orc_project2d(double* qz, const double* xv, const double* yv, int nx, int ny,
                   double dx, double dy, int wt_div, char bc_div);
 
@@ -70,6 +71,9 @@ int orc_recover3d(double* qx, double* qy, double* qz, double* xv_out, double* yv
                   double* zv_out, const double* xv, const double* yv, const double* zv,
                   int nx, int ny, int nz, double dx, double dy, double dz, char bc,
                   int wt_curl, int wt_div);
+
+/* GetDivL0Matrix: the 16-entry corner table for level counts (lx, ly). */
+void orc_div_l0_matrix(double* dst, int lx, int ly);
 
 /* CuWaveletPotentialRecover2D::Solve */
 int orc_recover2d(double* qz, const double* xv, const double* yv, int nx, int ny,

@@ -199,6 +199,52 @@ def test_recover2d_matches_oracle(whhd, orc, N):
     assert rel([got], [psi]) < TOL_Q
 
 
+def free_stream(n, seed):
+    nx, ny = n
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), indexing="ij")
+    a, b, c = rng.uniform(0.8, 3.5, 3)
+    return np.sin(a * X + b * Y + 0.4) + 0.3 * np.cos(c * X * Y) + 0.05 * rng.standard_normal(X.shape)
+
+
+@pytest.mark.parametrize("n,dx", [((16, 16), (1 / 16, 1 / 16)), ((128, 128), (1 / 128, 1 / 128)),
+                                  ((32, 16), (1 / 32, 1 / 16)), ((64, 8), (1 / 64, 1 / 8)),
+                                  ((8, 256), (1 / 8, 1 / 256))])
+def test_recover2d_velocity_bc_d_matches_oracle(whhd, orc, n, dx):
+    """Velocity bc 'd' in 2D (cuProjectLocal_q_single_n) against the oracle,
+    including level differences 1, 3 and 5 (the last with the reference's
+    one-entry table row); on the unit square also curl q = u."""
+    nx, ny = n
+    u = orc.curl2d(free_stream(n, nx + ny), n, dx)
+    ref = orc.recover2d(*u, n, dx, bc="d")[0]
+    T = whhd.WaveletType
+    rec = whhd.CuWaveletPotentialRecover2D(n, dx, "d", T.WT_CDF3_7, T.WT_CDF4_6)
+    qz = torch.full((nx + 1, ny + 1), np.nan, device="cuda")
+    rec.Solve(qz, cuda(u[0][0]), cuda(u[1][0]))
+    (got,) = host(qz)
+    assert np.isfinite(got).all()
+    assert rel([got], [ref]) < TOL_Q
+    if nx == ny and dx[0] == 1 / nx:
+        assert rel(list(orc.curl2d(got, n, dx)), list(u)) < TOL_Q
+
+
+def test_project2d_bc_n_matches_oracle(whhd, orc):
+    n = (32, 16)
+    nx, ny = n
+    dx = (0.03, 0.07)
+    rng = np.random.default_rng(12)
+    xv = rng.standard_normal((nx + 1, ny))
+    yv = rng.standard_normal((nx, ny + 1))
+    ref = orc.project2d(xv, yv, n, dx, bc_div="n")
+    T = whhd.WaveletType
+    p = whhd.CuDivLocalProject((nx + 1, ny + 1), (5, 4), dx, T.WT_CDF4_6, "n")
+    qz = torch.full((nx + 1, ny + 1), np.nan, device="cuda")
+    p.ProjectSingle(qz, cuda(xv), cuda(yv))
+    (got,) = host(qz)
+    ref = ref.reshape(got.shape)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=2e-6 * np.abs(ref).max())
+
+
 def test_recover3d_full_size_properties(whhd, orc, kernel_mode):
     """256^3 (the headline workload): FP64 oracle parity plus size-independent checks."""
     N = 256

@@ -160,3 +160,93 @@ def test_project3d_bc_n_vout_is_input(orc):
     out = orc.project3d(*v, n, (1 / N,) * 3, bc_div="n", with_vout=True)
     for a, b in zip(out[3:], v):
         assert np.array_equal(a, b)
+
+
+def _free_stream(n, seed):
+    """A stream function with no boundary constraint."""
+    nx, ny = n
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), indexing="ij")
+    a, b, c = rng.uniform(0.8, 3.5, 3)
+    return np.sin(a * X + b * Y + 0.4) + 0.3 * np.cos(c * X * Y) + 0.05 * rng.standard_normal(X.shape)
+
+
+@pytest.mark.parametrize("N", [16, 32, 64])
+def test_recover2d_velocity_bc_d_exact_on_unit_square(orc, N):
+    """Velocity bc 'd' in 2D (cuProjectLocal_q_single_n): on the unit square the
+    curl of any stream function (wall flux allowed) is reproduced to roundoff;
+    this exercises the interior, both 2x2 edge systems and the corner table's
+    equal-level row."""
+    n = (N, N)
+    dx = (1 / N, 1 / N)
+    u = orc.curl2d(_free_stream(n, N), n, dx)
+    q = orc.recover2d(*u, n, dx, bc="d")
+    assert orc.rel_l2(list(u), list(orc.curl2d(q[0], n, dx))) < 1e-10
+
+
+def _pipeline2d_n(orc, xv, yv, n, dx, corner=None):
+    """The velocity bc 'd' recovery written out stage by stage, with the four
+    corner coefficients optionally replaced."""
+    nx, ny = n
+    lx, ly = int(np.log2(nx)), int(np.log2(ny))
+    tx, _ = orc.transform(xv.reshape(1, nx + 1, ny), lx, "x", "n", "cdf4.6")
+    tx, _ = orc.transform(tx, ly, "y", "d", "cdf3.7")
+    ty, _ = orc.transform(yv.reshape(1, nx, ny + 1), lx, "x", "d", "cdf3.7")
+    ty, _ = orc.transform(ty, ly, "y", "n", "cdf4.6")
+    qw = orc.project2d(tx[0], ty[0], n, dx, bc_div="n").reshape(1, nx + 1, ny + 1).copy()
+    if corner is not None:
+        for (X, Y), v in zip(((0, 0), (1, 0), (0, 1), (1, 1)), corner):
+            qw[0, X, Y] = v
+    q, _ = orc.transform(qw, lx, "x", "n", "cdf4.6", False)
+    q, _ = orc.transform(q, ly, "y", "n", "cdf4.6", False)
+    return q[0], qw[0]
+
+
+@pytest.mark.parametrize("n,dx", [((32, 16), (1 / 32, 1 / 16)), ((16, 16), (0.1, 0.1)),
+                                  ((64, 8), (1 / 64, 1 / 8)), ((16, 32), (1 / 32, 1 / 32))])
+def test_recover2d_velocity_bc_d_off_unit_square(orc, n, dx):
+    """Off the unit square the reference's corner table (no length factor, one row
+    per level difference) leaves an O(1) curl error, while the rest of the path
+    stays exact: with the four corner coefficients set to their least-squares
+    values the curl is reproduced to roundoff.  This pins the interior and the
+    2x2 edge systems on anisotropic grids, and records the reference's limit."""
+    nx, ny = n
+    u = orc.curl2d(_free_stream(n, 7), n, dx)
+    q, qw = _pipeline2d_n(orc, *u, n, dx)
+    assert np.allclose(q, orc.recover2d(*u, n, dx, bc="d")[0], rtol=0, atol=1e-12 * np.abs(q).max())
+    assert orc.rel_l2(list(u), list(orc.curl2d(q, n, dx))) > 1e-3
+
+    def resid(c):
+        qq, _ = _pipeline2d_n(orc, *u, n, dx, c)
+        return np.concatenate([(a - b).ravel() for a, b in zip(orc.curl2d(qq, n, dx), u)])
+
+    base = qw[[0, 1, 0, 1], [0, 0, 1, 1]]
+    r0 = resid(base)
+    J = np.stack([resid(base + np.eye(4)[k]) - r0 for k in range(4)], 1)
+    dc, *_ = np.linalg.lstsq(J, -r0, rcond=None)
+    q2, _ = _pipeline2d_n(orc, *u, n, dx, base + dc)
+    assert orc.rel_l2(list(u), list(orc.curl2d(q2, n, dx))) < 1e-10
+
+
+def test_div_l0_table(orc):
+    """The transcribed level-difference table equals the closed form
+    a, b = -2^(d-2) (1 +- 1/(4^d+1)), c, e = -(1/4 +- 4^(d-1)/(4^d+1)) digit for
+    digit (d = 1..8), (-3/8, -1/8, -3/8, -1/8) for d = 0, with the d = 5 row
+    holding only the sum; pairs swap when y has more levels."""
+    for lx, ly in [(4, 4), (5, 4), (6, 4), (7, 4), (8, 4), (9, 4), (10, 4), (11, 4), (12, 4),
+                   (13, 4), (4, 5), (4, 7), (3, 11)]:
+        m = min(lx, ly)
+        ex, ey = lx - m, ly - m
+        d = max(ex, ey)
+        if 1 <= d <= 8:
+            r = 1 / (4 ** d + 1)
+            a, b = -2.0 ** (d - 2) * (1 + r), -2.0 ** (d - 2) * (1 - r)
+            c, e = -(0.25 + 4 ** (d - 1) * r), -(0.25 - 4 ** (d - 1) * r)
+            if d == 5:
+                a, b, c, e = a + b + c + e, 0.0, 0.0, 0.0
+        else:
+            a, b, c, e = -0.375, -0.125, -0.375, -0.125
+        if ex < ey:
+            a, c, b, e = c, a, e, b
+        want = [a, b, c, e, -a, -b, e, c, b, a, -c, -e, -b, -a, -e, -c]
+        np.testing.assert_allclose(orc.div_l0_matrix(lx, ly), want, rtol=0, atol=2e-14 * 2 ** d)

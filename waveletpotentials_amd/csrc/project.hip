@@ -23,11 +23,20 @@
 // node (X, Y) with X, Y > 1 gets
 // (xv*sy*B1x*B0y - yv*sx*B0x*B1y) / (sx^2*B0x*B1y + sy^2*B1x*B0y + eps);
 // the kernel there evaluates this in two tile orientations and adds halves.
+// Divergence bc 'n' (velocity bc 'd'), cuProjectLocal_q_single_n (:1047-1478):
+// the same interior; nodes X = 0, 1 (Y > 1) solve the 2x2 system that couples
+// the two x scaling functions (Gram entries <b0,b0>, <b0,b1>, scale
+// 1/length), nodes Y = 0, 1 (X > 1) the y analogue, and the four corner nodes
+// apply a fixed table (by level-count difference) to xv(0,1), xv(1,1),
+// yv(1,0), yv(1,1).  The table carries no length, so that path reproduces
+// curl q = u exactly only on the unit square, as the reference does.
 //
 // Both are single streaming passes, one thread per cell, y fastest so that
 // every wavefront touches contiguous rows.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "whhd_internal.h"
@@ -271,40 +280,110 @@ struct Proj2Args {
     const float *xv, *yv;
     AxisCoef cx, cy;  // indexed by node X (resp. Y): s from X-1, B0[X-1], B1[X]
     int nx, ny;
+    int nq;                             // divergence bc 'n'
+    float sxl, syl;                     // 1 / domain length
+    float bx00, bx01, by00, by01;       // <b0,b0>, <b0,b1> of the divergence basis
+    float p0x, p0y;                     // projection-basis Gram diagonal at 0
+    float corner[16];                   // level-difference table
 };
 
 __global__ void __launch_bounds__(256) project2d_kernel(Proj2Args p) {
     const int Y = blockIdx.x * 64 + threadIdx.x;
     const int X = blockIdx.y * 4 + threadIdx.y;
     if (Y > p.ny || X > p.nx) return;
+    const int64_t NY = p.ny;
+    // node-indexed velocities: xv(X, Y) = xv[(Y-1) + X NY] (0 at Y = 0),
+    // yv(X, Y) = yv[Y + (X-1)(NY+1)] (0 at X = 0)
+    auto XV = [&](int x, int y) { return y >= 1 ? p.xv[(y - 1) + x * NY] : 0.f; };
+    auto YV = [&](int x, int y) { return x >= 1 ? p.yv[y + (x - 1) * (NY + 1)] : 0.f; };
     float q = 0.f;
     if (X > 1 && Y > 1) {
-        const float vx = p.xv[(Y - 1) + (int64_t)X * p.ny];
-        const float vy = p.yv[Y + (int64_t)(X - 1) * (p.ny + 1)];
+        const float vx = XV(X, Y), vy = YV(X, Y);
         const float sx = p.cx.s[X], sy = p.cy.s[Y];
         const float ip0x = p.cx.b0[X], ip1x = p.cx.b1[X];
         const float ip0y = p.cy.b0[Y], ip1y = p.cy.b1[Y];
         const float mid = sx * sx * ip0x * ip1y + sy * sy * ip1x * ip0y + (float)kEps;
         q = (vx * sy * ip1x * ip0y - vy * sx * ip0x * ip1y) / mid;
+    } else if (p.nq) {
+        const float eps = (float)kEps;
+        if (Y > 1) {  // X = 0, 1
+            const float sy = p.cy.s[Y], ip0y = p.cy.b0[Y], ip1y = p.cy.b1[Y];
+            const float xv0 = XV(0, Y), xv1 = XV(1, Y), v = YV(1, Y);
+            const float ax = p.sxl * p.sxl * p.p0x * ip1y;
+            const float p0 = ax + sy * sy * p.bx00 * ip0y;
+            const float p1 = -ax + sy * sy * p.bx01 * ip0y;
+            const float den = p0 * p0 - p1 * p1 + eps;
+            const float m0 = p0 / den, m1 = -p1 / den;
+            const float g = v * p.sxl * p.p0x * ip1y;
+            const float f1 = g + xv0 * sy * p.bx00 * ip0y + xv1 * sy * p.bx01 * ip0y;
+            const float f2 = -g + xv0 * sy * p.bx01 * ip0y + xv1 * sy * p.bx00 * ip0y;
+            q = X == 0 ? m0 * f1 + m1 * f2 : m1 * f1 + m0 * f2;
+        } else if (X > 1) {  // Y = 0, 1
+            const float sx = p.cx.s[X], ip0x = p.cx.b0[X], ip1x = p.cx.b1[X];
+            const float v = XV(X, 1), yv0 = YV(X, 0), yv1 = YV(X, 1);
+            const float ay = p.syl * p.syl * ip1x * p.p0y;
+            const float p0 = sx * sx * ip0x * p.by00 + ay;
+            const float p1 = sx * sx * ip0x * p.by01 - ay;
+            const float den = p0 * p0 - p1 * p1 + eps;
+            const float m0 = p0 / den, m1 = -p1 / den;
+            const float g = v * p.syl * ip1x * p.p0y;
+            const float f1 = -g - yv0 * sx * ip0x * p.by00 - yv1 * sx * ip0x * p.by01;
+            const float f2 = g - yv0 * sx * ip0x * p.by01 - yv1 * sx * ip0x * p.by00;
+            q = Y == 0 ? m0 * f1 + m1 * f2 : m1 * f1 + m0 * f2;
+        } else {  // corner
+            const float* a = p.corner + (2 * Y + X) * 4;
+            q = a[2] * XV(0, 1) + a[3] * XV(1, 1) - a[0] * YV(1, 0) - a[1] * YV(1, 1);
+        }
     }
-    p.qz[Y + (int64_t)X * (p.ny + 1)] = q;
+    p.qz[Y + X * (NY + 1)] = q;
 }
 
 // ---------------------------------------------------------------------------
 // Host: coefficient tables
 // ---------------------------------------------------------------------------
-static int gram_diag_host(std::vector<double>& diag, int wt, char bc, int n) {
+// The first `rows` rows of the 8-row Gram table (row 0: <b_i, b_i>, row 3:
+// <b_i, b_i+1>), row r at r * n.
+static int gram_rows_host(std::vector<double>& out, int wt, char bc, int n, int rows) {
     double* d = nullptr;
     if (hipMalloc(&d, sizeof(double) * 8 * (size_t)n) != hipSuccess)
-        return fail(WHHD_OOM, "gram diagonal allocation failed");
+        return fail(WHHD_OOM, "gram table allocation failed");
     int st = gram_table_device(d, wt, bc, n, nullptr);
     if (st == WHHD_OK) {
-        diag.resize(n);
-        hipError_t e = hipMemcpy(diag.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = hip_fail(e, "gram diagonal copy");
+        out.resize((size_t)rows * n);
+        hipError_t e = hipMemcpy(out.data(), d, sizeof(double) * rows * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "gram table copy");
     }
     hipFree(d);
     return st;
+}
+
+static int gram_diag_host(std::vector<double>& diag, int wt, char bc, int n) {
+    return gram_rows_host(diag, wt, bc, n, 1);
+}
+
+// GetDivL0Matrix (WaveletPotentials/DivL0Matrix.cpp:3-50): four numbers (a, b,
+// c, e) by the difference d of the two level counts, swapped pairwise when the
+// y axis has more levels, expanded to the 4x4 sign pattern.  For d = 1..8 the
+// reference's table equals a, b = -2^(d-2) (1 +- 1/(4^d + 1)),
+// c, e = -(1/4 +- 4^(d-1)/(4^d + 1)), computed here from that form; d = 0 or
+// d > 8 take (-3/8, -1/8, -3/8, -1/8).  At d = 5 the reference initialises
+// only the first entry, with the sum of the four (DivL0Matrix.cpp:27), and so
+// does this table.
+static void div_l0_table(float out[16], int lx, int ly) {
+    const int m = lx < ly ? lx : ly;
+    const int ex = lx - m, ey = ly - m;
+    const int d = ex > ey ? ex : ey;
+    double a = -0.375, b = -0.125, c = -0.375, e = -0.125;
+    if (d >= 1 && d <= 8) {
+        const double r = 1.0 / (std::ldexp(1.0, 2 * d) + 1.0);
+        const double h = std::ldexp(1.0, d - 2), t = std::ldexp(1.0, 2 * d - 2) * r;
+        a = -h * (1.0 + r); b = -h * (1.0 - r);
+        c = -(0.25 + t); e = -(0.25 - t);
+        if (d == 5) { a = a + b + c + e; b = c = e = 0.0; }
+    }
+    if (ex < ey) { std::swap(a, c); std::swap(b, e); }
+    const double v[16] = {a, b, c, e, -a, -b, e, c, b, a, -c, -e, -b, -a, -e, -c};
+    for (int i = 0; i < 16; ++i) out[i] = (float)v[i];
 }
 
 // rows: [s | B0 | B1], each of `len` floats.  mode3d: index c is the cell
@@ -348,6 +427,10 @@ struct whhd_project2d_s {
     int nx, ny;
     float* coef = nullptr;
     whhd::AxisCoef cx, cy;
+    bool nq = false;  // divergence bc 'n': boundary constants below
+    float sxl = 0.f, syl = 0.f, bx00 = 0.f, bx01 = 0.f, by00 = 0.f, by01 = 0.f, p0x = 0.f,
+          p0y = 0.f;
+    float corner[16] = {};
 };
 
 namespace whhd {
@@ -446,17 +529,33 @@ int project2d_create(int nx, int ny, double dx, double dy, int wt, char bc, whhd
     if (!is_pow2(nx) || !is_pow2(ny))
         return fail(WHHD_NOT_POW2, "project2d: cell counts must be powers of two (>= 4)");
     if (!(dx > 0 && dy > 0)) return fail(WHHD_INVALID_ARG, "project2d: spacing must be > 0");
-    if (bc != 'd')
+    if (bc != 'd' && bc != 'n')
         return fail(WHHD_UNSUPPORTED_WAVELET,
-                    "project2d: only the divergence bc 'd' (velocity bc 'n') is implemented");
+                    "project2d: the divergence bc must be 'd' (velocity bc 'n') or 'n' (velocity bc 'd')");
     int st = check_wavelet(wt, bc);
     if (st != WHHD_OK) return st;
     if (proj_wavelet(wt) < 0) return fail(WHHD_UNSUPPORTED_WAVELET, "project2d: no projection type");
     std::vector<float> rx, ry;
     if ((st = axis_coef(rx, nx, dx, wt, bc, false)) != WHHD_OK) return st;
     if ((st = axis_coef(ry, ny, dy, wt, bc, false)) != WHHD_OK) return st;
+    std::vector<double> gx, gy, px, py;
+    if (bc == 'n') {
+        if ((st = gram_rows_host(gx, wt, bc, nx + 1, 4)) != WHHD_OK) return st;
+        if ((st = gram_rows_host(gy, wt, bc, ny + 1, 4)) != WHHD_OK) return st;
+        if ((st = gram_diag_host(px, proj_wavelet(wt), proj_bc(bc), nx)) != WHHD_OK) return st;
+        if ((st = gram_diag_host(py, proj_wavelet(wt), proj_bc(bc), ny)) != WHHD_OK) return st;
+    }
     auto* h = new whhd_project2d_s;
     h->nx = nx; h->ny = ny;
+    if (bc == 'n') {
+        h->nq = true;
+        h->sxl = (float)(1.0 / (nx * dx));
+        h->syl = (float)(1.0 / (ny * dy));
+        h->bx00 = (float)gx[0]; h->bx01 = (float)gx[3 * (size_t)(nx + 1)];
+        h->by00 = (float)gy[0]; h->by01 = (float)gy[3 * (size_t)(ny + 1)];
+        h->p0x = (float)px[0]; h->p0y = (float)py[0];
+        div_l0_table(h->corner, ilog2(nx), ilog2(ny));
+    }
     const size_t tot = rx.size() + ry.size();
     if (hipMalloc(&h->coef, tot * sizeof(float)) != hipSuccess) {
         delete h;
@@ -481,7 +580,9 @@ int project2d_create(int nx, int ny, double dx, double dy, int wt, char bc, whhd
 int project2d_run(const whhd_project2d_s* h, float* qz, const float* xv, const float* yv,
                   hipStream_t st) {
     if (!h || !qz || !xv || !yv) return fail(WHHD_INVALID_ARG, "project2d_run: null argument");
-    Proj2Args a{qz, xv, yv, h->cx, h->cy, h->nx, h->ny};
+    Proj2Args a{qz, xv, yv, h->cx, h->cy, h->nx, h->ny, h->nq ? 1 : 0, h->sxl, h->syl,
+                h->bx00, h->bx01, h->by00, h->by01, h->p0x, h->p0y, {}};
+    for (int i = 0; i < 16; ++i) a.corner[i] = h->corner[i];
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, 1);
     hipLaunchKernelGGL(project2d_kernel, grid, block, 0, st, a);
