@@ -349,22 +349,32 @@ WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, dou
  * dx.x*dx.y, the same for cubic cells).  phi may equal b. */
 WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream);
 WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h);
+/* CuDst(int2 dim, DTYPE2 dx) (WaveletPotentials/cuDst.cu:4-36): the 2D solver on an
+ * (n1, n2) node grid; solve with whhd_dst3d_solve (CuDst::Solve,
+ * WaveletPotentials/cuDst.cu:207-311: (-L_h) phi = b for the 5-point Laplacian,
+ * each axis with its own spacing), release with whhd_dst3d_destroy. */
+WHHD_API int whhd_dst2d_create(int n1, int n2, double dx, double dy, whhd_dst3d_t* out);
 
 typedef struct whhd_sweep3d_s* whhd_sweep3d_t;
 /* CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cu:
- * 876-895) for nx*ny*nz cells; only velocity bc 'n' is implemented.  Owns a
- * DST solver on the (nx-1, ny-1, nz-1) interior nodes. */
+ * 876-895) for nx*ny*nz cells, velocity bc 'n', 'd' or 'z' ('z' takes the 'd'
+ * branches, as in the reference).  Owns a DST solver on the (nx-1, ny-1, nz-1)
+ * interior nodes ('n') or on all (nx+1, ny+1, nz+1) nodes ('d'). */
 WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
                                  whhd_sweep3d_t* out);
-/* Solve(qx, qy, qz, xv, yv, zv) (WaveletPotentials/cuParallelSweep3D.cu:939-965): qx
- * and qy integrate yv and -xv along z from zero at the bottom, qz = 0, then the
- * top planes of qx, qy are cleared and qz's top plane takes the x-integral of
- * the cleared qx.  zv is not read. */
+/* Solve(qx, qy, qz, xv, yv, zv) (WaveletPotentials/cuParallelSweep3D.cu:926-965): qx
+ * and qy integrate yv and -xv along z from their bottom planes, qz = 0.  bc 'n':
+ * the bottom planes are zero, then the top planes of qx, qy are cleared and
+ * qz's top plane takes the x-integral of the cleared qx; zv is not read.
+ * bc 'd': qx's bottom plane is zero and qy's integrates zv along x
+ * (SweepingQy_x); no top step. */
 WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
                                 const float* xv, const float* yv, const float* zv, void* stream);
-/* Project(qx, qy, qz) for bc 'n' = ProjectZero (WaveletPotentials/cuParallelSweep3D.cu:
- * 1054-1064, 1316-1326): q -= grad phi with (-L_h) phi = -div q at the interior
- * nodes, phi = 0 on the boundary nodes. */
+/* Project(qx, qy, qz) (WaveletPotentials/cuParallelSweep3D.cu:1316-1326).  bc 'n':
+ * ProjectZero (:1054-1064), q -= grad phi with (-L_h) phi = -div q at the
+ * interior nodes, phi = 0 on the boundary nodes.  bc 'd': ProjectD (:1066-1079),
+ * phi on every node from the wall-clamped divergence (cuGetB_q_d_3d), zero one
+ * node outside, and q -= grad phi over every edge. */
 WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz, void* stream);
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h);
 

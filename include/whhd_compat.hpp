@@ -19,6 +19,7 @@
 //   CudaMatAdd/MatMul/ArraySum/AbsSum/NormSum/FindMaxValue
 //                                WaveletPotentials/cudaMath.cuh:127-139, 202
 //   CuDst3D, CuParallelSweep3D   WaveletPotentials/cuDst3D.cuh:6-13, cuParallelSweep3D.cuh:9-23
+//   CuDst                        WaveletPotentials/cuDst.cuh
 // Differences, all deliberate:
 //   - DTYPE is float (the reference builds FP64 or FP32 by macro); arrays are
 //     device memory, y fastest, as in the reference.
@@ -373,7 +374,14 @@ public:
     void Solve(DTYPE* dst, DTYPE* f) { check(whhd_dst3d_solve(h_, dst, f, stream_), "Solve"); }
 };
 
-// bc 'n' only; the DST solve is direct, so tol is not used.
+class CuDst : public Owned<whhd_dst3d_t, whhd_dst3d_destroy> {
+public:
+    CuDst() = delete;
+    CuDst(int2 dim, DTYPE2 dx) { check(whhd_dst2d_create(dim.x, dim.y, dx.x, dx.y, &h_), "CuDst"); }
+    void Solve(DTYPE* dst, DTYPE* f) { check(whhd_dst3d_solve(h_, dst, f, stream_), "Solve"); }
+};
+
+// bc 'n', 'd' or 'z'; the DST solve is direct, so tol is not used.
 class CuParallelSweep3D : public Owned<whhd_sweep3d_t, whhd_sweep3d_destroy> {
 public:
     CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) {

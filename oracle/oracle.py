@@ -69,6 +69,9 @@ def _setup(L):
     L.orc_divergence3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d]
     L.orc_sweep3d.argtypes = [_D] * 5 + [i, i, i, d, d, d]
     L.orc_dst3d_solve.argtypes = [_D, _D, i, i, i, d, d, d]
+    L.orc_dst2d_solve.argtypes = [_D, _D, i, i, d, d]
+    L.orc_sweep3d_d.argtypes = [_D] * 6 + [i, i, i, d, d, d]
+    L.orc_project_d.argtypes = [_D] * 3 + [i, i, i, d, d, d]
     L.orc_project_zero.argtypes = [_D] * 3 + [i, i, i, d, d, d]
     L.orc_conv3.argtypes = [_D, _D, i, _D, i, i, i, i, i, _D, _D, i]
     L.orc_conv3_bwd_data.argtypes = [_D, _D, i, i, i, i, _D, i]
@@ -342,12 +345,40 @@ def sweep3d(xv, yv, n, dx):
     return qx, qy, qz
 
 
+def sweep3d_d(xv, yv, zv, n, dx):
+    """CuParallelSweep3D::Solve for velocity bc 'd' ('z'): returns (qx, qy, qz)."""
+    nx, ny, nz = n
+    v = [np.ascontiguousarray(a, dtype=np.float64) for a in (xv, yv, zv)]
+    qx, qy, qz = field(nx, ny + 1, nz + 1), field(nx + 1, ny, nz + 1), field(nx + 1, ny + 1, nz)
+    _check(lib().orc_sweep3d_d(_p(qx), _p(qy), _p(qz), *map(_p, v), nx, ny, nz, *map(float, dx)),
+           "sweep3d_d")
+    return qx, qy, qz
+
+
+def project_d(qx, qy, qz, n, dx):
+    """CuParallelSweep3D::ProjectD; returns new arrays."""
+    nx, ny, nz = n
+    q = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (qx, qy, qz)]
+    _check(lib().orc_project_d(_p(q[0]), _p(q[1]), _p(q[2]), nx, ny, nz, *map(float, dx)),
+           "project_d")
+    return tuple(q)
+
+
 def dst3d_solve(b, dx):
     """(-L_h) phi = b with zero values one node outside; b is (n3, n1, n2)."""
     b = np.ascontiguousarray(b, dtype=np.float64)
     n3, n1, n2 = b.shape
     phi = np.empty_like(b)
     _check(lib().orc_dst3d_solve(_p(phi), _p(b), n1, n2, n3, *map(float, dx)), "dst3d_solve")
+    return phi
+
+
+def dst2d_solve(b, dx):
+    """(-L_h) phi = b on an (n1, n2) node grid, zero one node outside."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n1, n2 = b.shape
+    phi = np.empty_like(b)
+    _check(lib().orc_dst2d_solve(_p(phi), _p(b), n1, n2, *map(float, dx)), "dst2d_solve")
     return phi
 
 

@@ -103,6 +103,94 @@ int orc_dst3d_solve(double* phi, const double* b, int n1, int n2, int n3, double
     return ORC_OK;
 }
 
+/* CuParallelSweep3D::Solve for velocity bc 'd' (and 'z', which takes the same
+ * branches): WaveletPotentials/cuParallelSweep3D.cu:926-965.  qz = 0, the
+ * bottom plane of qx is 0 and that of qy integrates zv along x
+ * (SweepingQy_x, :68-85); then the z sweeps from those planes; no top gauge. */
+int orc_sweep3d_d(double* qx, double* qy, double* qz, const double* xv, const double* yv,
+                  const double* zv, int nx, int ny, int nz, double dx, double dy, double dz) {
+    if (!qx || !qy || !qz || !xv || !yv || !zv || nx < 1 || ny < 1 || nz < 1) return ORC_INVALID;
+    (void)dy;
+    memset(qz, 0, sizeof(double) * (size_t)(nx + 1) * (ny + 1) * nz);
+    for (int y = 0; y < ny; ++y) {
+        qy[I3(y, 0, 0, nx + 1, ny)] = 0.0;
+        for (int x = 1; x <= nx; ++x)
+            qy[I3(y, x, 0, nx + 1, ny)] = qy[I3(y, x - 1, 0, nx + 1, ny)] + zv[I3(y, x - 1, 0, nx, ny)] * dx;
+    }
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y <= ny; ++y) {
+            qx[I3(y, x, 0, nx, ny + 1)] = 0.0;
+            for (int z = 1; z <= nz; ++z)
+                qx[I3(y, x, z, nx, ny + 1)] =
+                    qx[I3(y, x, z - 1, nx, ny + 1)] + yv[I3(y, x, z - 1, nx, ny + 1)] * dz;
+        }
+    for (int x = 0; x <= nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 1; z <= nz; ++z)
+                qy[I3(y, x, z, nx + 1, ny)] =
+                    qy[I3(y, x, z - 1, nx + 1, ny)] - xv[I3(y, x, z - 1, nx + 1, ny)] * dz;
+    return ORC_OK;
+}
+
+/* CuParallelSweep3D::ProjectD (WaveletPotentials/cuParallelSweep3D.cu:1066-1079):
+ * b = -div q at every node of the potential grid with the edge index clamped at
+ * the walls (cuGetB_q_d_3d, cuConvergence.cu:161-200), a DST-I solve on those
+ * (nx+1, ny+1, nz+1) nodes, and q -= grad phi over every edge
+ * (cuGradientMinus3D_d_Q, cuGradient.cu:2408-2446). */
+int orc_project_d(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
+                  double dy, double dz) {
+    if (!qx || !qy || !qz || nx < 1 || ny < 1 || nz < 1) return ORC_INVALID;
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    double* phi = (double*)malloc(sizeof(double) * (size_t)px * py * pz);
+    if (!phi) return ORC_INVALID;
+    for (int k = 0; k < pz; ++k)
+        for (int j = 0; j < px; ++j)
+            for (int i = 0; i < py; ++i) {
+                const int jl = j > 0 ? j - 1 : 0, jr = j < nx - 1 ? j : nx - 1;
+                const int ib = i > 0 ? i - 1 : 0, it = i < ny - 1 ? i : ny - 1;
+                const int kf = k > 0 ? k - 1 : 0, kb = k < nz - 1 ? k : nz - 1;
+                const double d = (qx[I3(i, jr, k, nx, ny + 1)] - qx[I3(i, jl, k, nx, ny + 1)]) / dx +
+                                 (qy[I3(it, j, k, nx + 1, ny)] - qy[I3(ib, j, k, nx + 1, ny)]) / dy +
+                                 (qz[I3(i, j, kb, nx + 1, ny + 1)] - qz[I3(i, j, kf, nx + 1, ny + 1)]) / dz;
+                phi[I3(i, j, k, px, py)] = -d;
+            }
+    orc_dst3d_solve(phi, phi, px, py, pz, dx, dy, dz);
+    for (int z = 0; z < pz; ++z)
+        for (int x = 0; x < px; ++x)
+            for (int y = 0; y < py; ++y) {
+                const double p = phi[I3(y, x, z, px, py)];
+                if (x < nx) qx[I3(y, x, z, nx, ny + 1)] -= (phi[I3(y, x + 1, z, px, py)] - p) / dx;
+                if (y < ny) qy[I3(y, x, z, nx + 1, ny)] -= (phi[I3(y + 1, x, z, px, py)] - p) / dy;
+                if (z < nz) qz[I3(y, x, z, nx + 1, ny + 1)] -= (phi[I3(y, x, z + 1, px, py)] - p) / dz;
+            }
+    free(phi);
+    return ORC_OK;
+}
+
+/* CuDst::Solve (WaveletPotentials/cuDst.cu:207-311): the 2D DST-I Poisson solve,
+ * (-L_h) phi = b for the 5-point Laplacian on an (n1, n2) node grid with zero
+ * values one node outside; each axis uses its own spacing (the reference
+ * scales both by dx.x*dx.y, the same for square cells). */
+int orc_dst2d_solve(double* phi, const double* b, int n1, int n2, double dx, double dy) {
+    if (!phi || !b || n1 < 1 || n2 < 1) return ORC_INVALID;
+    const size_t N = (size_t)n1 * n2;
+    double* tmp = (double*)malloc(sizeof(double) * (n1 > n2 ? n1 : n2));
+    if (!tmp) return ORC_INVALID;
+    memmove(phi, b, N * sizeof(double));
+    dst_axis(phi, n1, n2, 1, 0, tmp);
+    dst_axis(phi, n1, n2, 1, 1, tmp);
+    for (int x = 0; x < n1; ++x)
+        for (int y = 0; y < n2; ++y) {
+            const double lx = (2.0 - 2.0 * cos(M_PI * (x + 1) / (n1 + 1))) / (dx * dx);
+            const double ly = (2.0 - 2.0 * cos(M_PI * (y + 1) / (n2 + 1))) / (dy * dy);
+            phi[y + (size_t)x * n2] /= (lx + ly);
+        }
+    dst_axis(phi, n1, n2, 1, 1, tmp);
+    dst_axis(phi, n1, n2, 1, 0, tmp);
+    free(tmp);
+    return ORC_OK;
+}
+
 /* b = -div q at the interior nodes (1..nx-1, 1..ny-1, 1..nz-1) of the potential grid */
 static void node_div(double* b, const double* qx, const double* qy, const double* qz, int nx,
                      int ny, int nz, double dx, double dy, double dz) {

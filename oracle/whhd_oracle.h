@@ -136,6 +136,14 @@ int orc_sweep3d(double* qx, double* qy, double* qz, const double* xv, const doub
 /* (-L_h) phi = b on an (n1, n2, n3) node grid, zero one node outside (CuDst3D::Solve). */
 int orc_dst3d_solve(double* phi, const double* b, int n1, int n2, int n3, double dx, double dy,
                     double dz);
+/* CuDst::Solve: 2D DST-I Poisson solve on an (n1, n2) node grid, phi may equal b. */
+int orc_dst2d_solve(double* phi, const double* b, int n1, int n2, double dx, double dy);
+/* CuParallelSweep3D::Solve for velocity bc 'd' ('z' takes the same branches). */
+int orc_sweep3d_d(double* qx, double* qy, double* qz, const double* xv, const double* yv,
+                  const double* zv, int nx, int ny, int nz, double dx, double dy, double dz);
+/* CuParallelSweep3D::ProjectD: the projection on every node of the potential grid. */
+int orc_project_d(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
+                  double dy, double dz);
 /* CuParallelSweep3D::ProjectZero: q -= grad phi with (-L_h) phi = -div q at interior nodes. */
 int orc_project_zero(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
                      double dy, double dz);

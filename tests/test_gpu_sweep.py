@@ -65,3 +65,52 @@ def test_dst_solve(whhd, orc, shape, h):
     t = cuda(b)
     d.Solve(t, t)
     assert np.abs(host(t) - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape,h", [((15, 15), (0.1, 0.1)), ((127, 33), (0.01, 0.03)), ((1, 7), (0.5, 0.2))])
+def test_dst2d_solve(whhd, orc, shape, h):
+    """CuDst (2D) against the oracle, out of place and in place."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    b = rng.standard_normal(shape).astype(np.float32).astype(np.float64)
+    d = whhd.CuDst(shape, h)
+    phi = torch.empty(shape, device="cuda")
+    d.Solve(phi, cuda(b))
+    ref = orc.dst2d_solve(b, h)
+    assert np.abs(host(phi) - ref).max() <= 1e-5 * np.abs(ref).max()
+    t = cuda(b)
+    d.Solve(t, t)
+    assert np.abs(host(t) - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("bc", ["d", "z"])
+def test_sweep_and_project_velocity_bc_d(whhd, orc, bc):
+    """Velocity bc 'd' ('z' takes the same branches): the swept potential against
+    the oracle, then ProjectD against the oracle and its interior node
+    divergence."""
+    n = (24, 20, 28)
+    h = (0.04, 0.05, 0.035)
+    nx, ny, nz = n
+    rng = np.random.default_rng(21)
+    q0 = [rng.standard_normal(s) for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny),
+                                           (nz, nx + 1, ny + 1))]
+    u = [a.astype(np.float32).astype(np.float64) for a in I.curl3d(*q0, h)]
+    sw = whhd.CuParallelSweep3D(n, h, bc)
+    q = [torch.full(s, np.nan, device="cuda") for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny),
+                                                        (nz, nx + 1, ny + 1))]
+    sw.Solve(*q, *(cuda(a) for a in u))
+    ref = orc.sweep3d_d(*u, n, h)
+    for g, r in zip(q, ref):
+        g = host(g)
+        assert np.isfinite(g).all()
+        assert np.abs(g - r).max() <= 1e-5 * max(np.abs(r).max(), 1e-30)
+    sw.Project(*q)
+    pref = orc.project_d(*ref, n, h)
+    scale = max(np.abs(r).max() for r in pref)
+    got = [host(g) for g in q]
+    for g, r in zip(got, pref):
+        assert np.abs(g - r).max() <= 1e-4 * scale
+    qx, qy, qz = got
+    div = ((qx[1:-1, 1:, 1:-1] - qx[1:-1, :-1, 1:-1]) / h[0]
+           + (qy[1:-1, 1:-1, 1:] - qy[1:-1, 1:-1, :-1]) / h[1]
+           + (qz[1:, 1:-1, 1:-1] - qz[:-1, 1:-1, 1:-1]) / h[2])
+    assert np.abs(div).max() <= 1e-4 * scale / min(h)

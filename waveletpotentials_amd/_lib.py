@@ -91,6 +91,7 @@ SIGNATURES = {
     "whhd_dst3d_create": (_I, [_I, _I, _I, _D, _D, _D, ctypes.POINTER(_P)]),
     "whhd_dst3d_solve": (_I, [_P, _P, _P, _P]),
     "whhd_dst3d_destroy": (None, [_P]),
+    "whhd_dst2d_create": (_I, [_I, _I, _D, _D, ctypes.POINTER(_P)]),
     "whhd_sweep3d_create": (_I, [_I, _I, _I, _D, _D, _D, _C, ctypes.POINTER(_P)]),
     "whhd_sweep3d_solve": (_I, [_P] * 7 + [_P]),
     "whhd_sweep3d_project": (_I, [_P] * 4 + [_P]),

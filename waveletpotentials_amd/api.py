@@ -28,7 +28,7 @@ from ._lib import WhhdError, check, lib
 __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "string_from_type",
            "CuSolidLevelSet3D", "CuFastSweeping", "CuMultigridFrac3D", "CuGradient",
            "CuConvergence", "CudaMatAdd", "CudaMatMul", "CudaArraySum", "CudaArrayNormSum",
-           "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
+           "CudaArrayAbsSum", "CudaFindMaxValue", "CuDst3D", "CuDst", "CuParallelSweep3D", "CuWHHDForward", "CuGetInnerProduct", "CuDivLocalProject3D", "CuDivLocalProject",
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "CuBatchRecover3D",
            "UNet3D", "WriteDataTxt", "WriteDataETxt", "ReadsDataTxt", "WriteDataTxt_sparse",
            "WriteVectorTxt_sparse", "WriteDataBinary", "ReadDataBinary", "WritePly",
@@ -730,9 +730,27 @@ class CuDst3D(_Handle):
                                      _ptr(f, numel=n, name="f"), _stream()), "CuDst3D.Solve")
 
 
+class CuDst(_Handle):
+    """CuDst(int2 dim, DTYPE2 dx) (WaveletPotentials/cuDst.cuh): 2D DST-I Poisson solver
+    on a dim[0] x dim[1] node grid (arrays (dim[0], dim[1]), y fastest)."""
+    _destroy = "whhd_dst3d_destroy"
+
+    def __init__(self, dim, dx):
+        super().__init__()
+        self.dim = tuple(int(v) for v in dim)
+        check(lib().whhd_dst2d_create(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)),
+              "CuDst")
+
+    def Solve(self, dst, f):
+        """Solve(dst, f): (-L_h) dst = f (WaveletPotentials/cuDst.cu:207-311)."""
+        n = self.dim[0] * self.dim[1]
+        check(lib().whhd_dst3d_solve(self._h, _ptr(dst, numel=n, name="dst"),
+                                     _ptr(f, numel=n, name="f"), _stream()), "CuDst.Solve")
+
+
 class CuParallelSweep3D(_Handle):
     """CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cuh):
-    the sweep + DST comparator (bc 'n')."""
+    the sweep + DST comparator, velocity bc 'n', 'd' or 'z' ('z' as 'd')."""
     _destroy = "whhd_sweep3d_destroy"
 
     def __init__(self, dim, dx, bc="n"):

@@ -1,7 +1,9 @@
 // Sweep + DST comparator: a vector potential by integrating the velocity along
 // z, made divergence free by a DST-I Poisson solve.
 //   CuParallelSweep3D::Solve   WaveletPotentials/cuParallelSweep3D.cu:9-66, 939-965
-//   CuParallelSweep3D::Project WaveletPotentials/cuParallelSweep3D.cu:1054-1064, 1316-1326
+//   CuParallelSweep3D::Project WaveletPotentials/cuParallelSweep3D.cu:1054-1079, 1316-1326
+//   (velocity bc 'n': ProjectZero on the interior nodes; 'd' and 'z': ProjectD
+//   on every node of the potential grid)
 //   CuDst3D::Solve             WaveletPotentials/cuDst3D.cu:183-341
 // The reference sweeps one column per thread with cuFFT for the DST.  Here the
 // sweep is the same column walk (y-fastest, so each z step of a wavefront is a
@@ -20,10 +22,12 @@ namespace {
 
 // One thread per node column (x <= nx, y <= ny): the qx column (x < nx) and qy
 // column (y < ny) are running sums along z (SweepingQx / SweepingQy), the qz
-// column is cleared.
+// column is cleared.  qy starts from zero, or with qy_base from the bottom
+// plane already in qy (velocity bc 'd').
 __global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
                              float* __restrict__ qz, const float* __restrict__ xv,
-                             const float* __restrict__ yv, int nx, int ny, int nz, float dz) {
+                             const float* __restrict__ yv, int nx, int ny, int nz, float dz,
+                             int qy_base) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     if (y > ny || x > nx) return;
@@ -41,8 +45,8 @@ __global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
     if (y < ny) {  // qy (nx+1, ny, nz+1) from xv (nx+1, ny, nz)
         const int64_t s = (int64_t)(nx + 1) * ny;
         const int64_t b = y + (int64_t)x * ny;
-        float acc = 0.f;
-        qy[b] = 0.f;
+        float acc = qy_base ? qy[b] : 0.f;
+        qy[b] = acc;
 #pragma unroll 8
         for (int z = 1; z <= nz; ++z) {
             acc -= xv[b + (z - 1) * s] * dz;
@@ -76,6 +80,60 @@ __global__ void top_gauge_kernel(float* __restrict__ qx, float* __restrict__ qy,
     }
     if (y < ny)
         for (int x = 0; x <= nx; ++x) qy[tqy + y + (int64_t)x * ny] = 0.f;
+}
+
+// Velocity bc 'd' bottom plane (SweepingQy_x, WaveletPotentials/cuParallelSweep3D.cu:
+// 68-85): per y, qy(x, y, 0) = sum_{x' < x} zv(x', y, 0) dx.
+__global__ void bottom_qy_kernel(float* __restrict__ qy, const float* __restrict__ zv, int nx,
+                                 int ny, float dx) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ny) return;
+    float acc = 0.f;
+    qy[y] = 0.f;
+    for (int x = 1; x <= nx; ++x) {
+        acc += zv[y + (int64_t)(x - 1) * ny] * dx;
+        qy[y + (int64_t)x * ny] = acc;
+    }
+}
+
+// Velocity bc 'd' divergence (cuGetB_q_d_3d, WaveletPotentials/cuConvergence.cu:
+// 161-200): b = -div q at every node (nx+1, ny+1, nz+1); at a wall the edge
+// index is clamped, so the missing edge repeats the wall edge and that axis
+// contributes nothing.
+__global__ void node_div_d_kernel(float* __restrict__ b, const float* __restrict__ qx,
+                                  const float* __restrict__ qy, const float* __restrict__ qz,
+                                  int nx, int ny, int nz, float ihx, float ihy, float ihz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;  // y
+    const int j = blockIdx.y * 4 + threadIdx.y;   // x
+    const int k = blockIdx.z;                     // z
+    if (i > ny || j > nx) return;
+    const int64_t NX = nx, NY = ny;
+    const int jl = max(j - 1, 0), jr = min(j, nx - 1);
+    const int ib = max(i - 1, 0), it = min(i, ny - 1);
+    const int kf = max(k - 1, 0), kb = min(k, nz - 1);
+    auto QX = [&](int x) { return qx[i + x * (NY + 1) + k * NX * (NY + 1)]; };
+    auto QY = [&](int yy) { return qy[yy + j * NY + k * (NX + 1) * NY]; };
+    auto QZ = [&](int z) { return qz[i + j * (NY + 1) + z * (NX + 1) * (NY + 1)]; };
+    const float d = (QX(jr) - QX(jl)) * ihx + (QY(it) - QY(ib)) * ihy + (QZ(kb) - QZ(kf)) * ihz;
+    b[i + j * (NY + 1) + k * (NX + 1) * (NY + 1)] = -d;
+}
+
+// q -= grad phi over every edge between two nodes (cuGradientMinus3D_d_Q,
+// WaveletPotentials/cuGradient.cu:2408-2446); phi is (nx+1, ny+1, nz+1).
+__global__ void grad_minus_d_kernel(float* __restrict__ qx, float* __restrict__ qy,
+                                    float* __restrict__ qz, const float* __restrict__ phi, int nx,
+                                    int ny, int nz, float ihx, float ihy, float ihz) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y > ny || x > nx) return;
+    const int64_t NX = nx, NY = ny;
+    const int64_t sp = (NX + 1) * (NY + 1);
+    const int64_t ip = y + x * (NY + 1) + z * sp;
+    const float p = phi[ip];
+    if (x < nx) qx[y + x * (NY + 1) + z * NX * (NY + 1)] -= (phi[ip + NY + 1] - p) * ihx;
+    if (y < ny) qy[y + x * NY + z * (NX + 1) * NY] -= (phi[ip + 1] - p) * ihy;
+    if (z < nz) qz[ip] -= (phi[ip + sp] - p) * ihz;
 }
 
 // b = -div q at the interior nodes (GetB_3d_q_zeros); b is (nx-1, ny-1, nz-1).
@@ -203,6 +261,7 @@ hipError_t done(const char* name) {
 
 struct whhd_dst3d_s {
     int n1, n2, n3;
+    bool planar = false;  // 2D solver (CuDst): no z passes, no z eigenvalue
     float* S[3] = {nullptr, nullptr, nullptr};
     float* lam[3] = {nullptr, nullptr, nullptr};
     float* tmp = nullptr;
@@ -210,6 +269,7 @@ struct whhd_dst3d_s {
 
 struct whhd_sweep3d_s {
     int nx, ny, nz;
+    char bc;  // 'n', or 'd' ('z' behaves as 'd')
     double dx, dy, dz;
     whhd_dst3d_s* dst = nullptr;
     float* b = nullptr;
@@ -290,6 +350,17 @@ int dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz, whhd_d
 
 int dst3d_solve(const whhd_dst3d_s* h, float* phi, const float* b, hipStream_t s) {
     if (!h || !phi || !b) return fail(WHHD_INVALID_ARG, "dst3d_solve: null argument");
+    if (h->planar) {
+        // b -> tmp (x) -> phi (y), scale, phi -> tmp (y) -> phi (x)
+        TRY(dst_pass(h, h->tmp, b, 0, s));
+        TRY(dst_pass(h, phi, h->tmp, 1, s));
+        hipLaunchKernelGGL(eigen_scale_kernel, grid_xyz(h->n1, h->n2, 1), dim3(64, 4), 0, s, phi,
+                           h->lam[0], h->lam[1], h->lam[2], h->n1, h->n2, 1);
+        hipError_t e = done("eigen_scale_kernel");
+        if (e != hipSuccess) return hip_fail(e, "eigen scale kernel");
+        TRY(dst_pass(h, h->tmp, phi, 1, s));
+        return dst_pass(h, phi, h->tmp, 0, s);
+    }
     // b -> tmp (x) -> phi (y) -> tmp (z), scale, tmp -> phi (z) -> tmp (y) -> phi (x)
     TRY(dst_pass(h, h->tmp, b, 0, s));
     TRY(dst_pass(h, phi, h->tmp, 1, s));
@@ -318,22 +389,40 @@ WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* 
 
 WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h) { whhd::dst_release(h); }
 
+WHHD_API int whhd_dst2d_create(int n1, int n2, double dx, double dy, whhd_dst3d_t* out) {
+    int st = whhd::dst3d_create(n1, n2, 1, dx, dy, 1.0, out);
+    if (st != WHHD_OK) return st;
+    (*out)->planar = true;
+    const float zero = 0.f;  // no z eigenvalue
+    hipError_t e = hipMemcpy((*out)->lam[2], &zero, sizeof zero, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        whhd::dst_release(*out);
+        *out = nullptr;
+        return whhd::hip_fail(e, "dst2d_create: eigenvalue upload");
+    }
+    return WHHD_OK;
+}
+
 WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
                                  whhd_sweep3d_t* out) {
     using namespace whhd;
     if (!out) return fail(WHHD_INVALID_ARG, "sweep3d_create: null handle pointer");
     *out = nullptr;
-    if (bc != 'n')
-        return fail(WHHD_INVALID_ARG, "sweep3d_create: only velocity bc 'n' is implemented");
+    if (bc != 'n' && bc != 'd' && bc != 'z')
+        return fail(WHHD_INVALID_ARG, "sweep3d_create: velocity bc must be 'n', 'd' or 'z'");
     if (nx < 2 || ny < 2 || nz < 2 || nz > 65534 ||
-        (int64_t)(nx + 1) * (ny + 1) * (nz + 1) >= (1LL << 31))
+        (int64_t)(nx + 2) * (ny + 2) * (nz + 2) >= (1LL << 31))
         return fail(WHHD_INVALID_ARG, "sweep3d_create: extents out of range");
+    if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "sweep3d_create: spacing <= 0");
     auto* h = new whhd_sweep3d_s;
     h->nx = nx; h->ny = ny; h->nz = nz;
+    h->bc = bc == 'n' ? 'n' : 'd';
     h->dx = dx; h->dy = dy; h->dz = dz;
-    int st = dst3d_create(nx - 1, ny - 1, nz - 1, dx, dy, dz, &h->dst);
+    // the potential grid's interior nodes ('n'), or all of its nodes ('d')
+    const int e = h->bc == 'n' ? -1 : 1;
+    int st = dst3d_create(nx + e, ny + e, nz + e, dx, dy, dz, &h->dst);
     if (st == WHHD_OK &&
-        hipMalloc(&h->b, (size_t)(nx - 1) * (ny - 1) * (nz - 1) * 4) != hipSuccess)
+        hipMalloc(&h->b, (size_t)(nx + e) * (ny + e) * (nz + e) * 4) != hipSuccess)
         st = fail(WHHD_OOM, "sweep3d_create: scratch allocation failed");
     if (st != WHHD_OK) {
         dst_release(h->dst);
@@ -347,14 +436,19 @@ WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, d
 WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
                                 const float* xv, const float* yv, const float* zv, void* stream) {
     using namespace whhd;
-    (void)zv;  // the z sweep integrates xv and yv only (SweepingQx/Qy)
     if (!h || !qx || !qy || !qz || !xv || !yv)
         return fail(WHHD_INVALID_ARG, "sweep3d_solve: null argument");
+    const bool d = h->bc == 'd';
+    if (d && !zv) return fail(WHHD_INVALID_ARG, "sweep3d_solve: velocity bc 'd' needs zv");
     hipStream_t s = (hipStream_t)stream;
+    if (d)  // SetQxQyBoundary 'd': qy's bottom plane integrates zv along x
+        hipLaunchKernelGGL(bottom_qy_kernel, dim3((h->ny + 255) / 256), dim3(256), 0, s, qy, zv,
+                           h->nx, h->ny, (float)h->dx);
     hipLaunchKernelGGL(sweep_kernel, grid_xyz(h->nx + 1, h->ny + 1, 1), dim3(64, 4), 0, s, qx, qy,
-                       qz, xv, yv, h->nx, h->ny, h->nz, (float)h->dz);
-    hipLaunchKernelGGL(top_gauge_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx, qy, qz,
-                       h->nx, h->ny, h->nz, (float)h->dx, (float)h->dz);
+                       qz, xv, yv, h->nx, h->ny, h->nz, (float)h->dz, d ? 1 : 0);
+    if (!d)
+        hipLaunchKernelGGL(top_gauge_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx, qy,
+                           qz, h->nx, h->ny, h->nz, (float)h->dx, (float)h->dz);
     hipError_t e = done("sweep_kernels");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "sweep kernels");
 }
@@ -365,6 +459,19 @@ WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float*
     if (!h || !qx || !qy || !qz) return fail(WHHD_INVALID_ARG, "sweep3d_project: null argument");
     hipStream_t s = (hipStream_t)stream;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const float ihx = (float)(1.0 / h->dx), ihy = (float)(1.0 / h->dy), ihz = (float)(1.0 / h->dz);
+    if (h->bc == 'd') {  // ProjectD, WaveletPotentials/cuParallelSweep3D.cu:1066-1079
+        hipLaunchKernelGGL(node_div_d_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s,
+                           h->b, qx, qy, qz, nx, ny, nz, ihx, ihy, ihz);
+        hipError_t e = done("node_div_d_kernel");
+        if (e != hipSuccess) return hip_fail(e, "node divergence kernel");
+        int st = dst3d_solve(h->dst, h->b, h->b, s);
+        if (st != WHHD_OK) return st;
+        hipLaunchKernelGGL(grad_minus_d_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s,
+                           qx, qy, qz, h->b, nx, ny, nz, ihx, ihy, ihz);
+        e = done("grad_minus_d_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "gradient kernel");
+    }
     hipLaunchKernelGGL(node_div_kernel, grid_xyz(nx - 1, ny - 1, nz - 1), dim3(64, 4), 0, s, h->b,
                        qx, qy, qz, nx, ny, nz, (float)(1.0 / h->dx), (float)(1.0 / h->dy),
                        (float)(1.0 / h->dz));
