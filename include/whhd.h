@@ -378,6 +378,24 @@ WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* q
 WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz, void* stream);
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h);
 
+/* ---- verification helpers on host arrays ------------------------------- */
+/* SetSliceZero(val, dim, axis) (WaveletPotentials/StreamInitial.cpp:5-43): zero the
+ * first and last slices of an (nx, ny, nz) array normal to axis 'x', 'y' or 'z'. */
+WHHD_API int whhd_set_slice_zero(float* val, int nx, int ny, int nz, char axis);
+/* StreamInitial::GradientStream 3D (WaveletPotentials/StreamInitial.cpp:87-143): face
+ * velocities = discrete curl of the edge potential on nx*ny*nz cells. */
+WHHD_API int whhd_gradient_stream3d(float* xv, float* yv, float* zv, const float* qx,
+                                    const float* qy, const float* qz, int nx, int ny, int nz,
+                                    double dx, double dy, double dz);
+/* StreamInitial::GradientStream 2D (WaveletPotentials/StreamInitial.cpp:145-175):
+ * xv = dqz/dy, yv = -dqz/dx. */
+WHHD_API int whhd_gradient_stream2d(float* xv, float* yv, const float* qz, int nx, int ny,
+                                    double dx, double dy);
+/* CalculateErrorL2 (WaveletPotentials/Utils.cpp:184-207, 296-331) over `count`
+ * (base, check, n) arrays, FP64 sums; eps 1e-12 in the denominator when count > 1. */
+WHHD_API int whhd_error_l2(const float* const* base, const float* const* check,
+                           const long long* n, int count, double* out);
+
 /* ---- field I/O on host arrays ------------------------------------------ */
 /* The reference's dumps (WaveletPotentials/Utils.cpp:812-990); arrays are host
  * memory, y fastest.  Text: one row of ny values per (x, z) in memory order,

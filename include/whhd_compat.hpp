@@ -398,4 +398,46 @@ public:
     void Project(DTYPE* qx, DTYPE* qy, DTYPE* qz, DTYPE tol = 1e-6f) { ProjectZero(qx, qy, qz, tol); }
 };
 
+// Host verification helpers of the reference's tests (host arrays).
+inline void SetSliceZero(DTYPE* val, int3 dim, char ns) {
+    check(whhd_set_slice_zero(val, dim.x, dim.y, dim.z, ns), "SetSliceZero");
+}
+
+class StreamInitial {
+public:
+    static StreamInitial* GetInstance() {
+        static StreamInitial inst;
+        return &inst;
+    }
+    void GradientStream(DTYPE* xv, DTYPE* yv, DTYPE* zv, DTYPE* qx, DTYPE* qy, DTYPE* qz,
+                        int3 dim, DTYPE3 dx) {
+        check(whhd_gradient_stream3d(xv, yv, zv, qx, qy, qz, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z),
+              "GradientStream");
+    }
+    void GradientStream(DTYPE* xv, DTYPE* yv, DTYPE* qz, int2 dim, DTYPE2 dx) {
+        check(whhd_gradient_stream2d(xv, yv, qz, dim.x, dim.y, dx.x, dx.y), "GradientStream");
+    }
+};
+
+inline DTYPE CalculateErrorL2(const DTYPE* value1, const DTYPE* value2, int3 size3) {
+    const float* b[1] = {value1};
+    const float* c[1] = {value2};
+    const long long n[1] = {(long long)size3.x * size3.y * size3.z};
+    double out = 0.0;
+    check(whhd_error_l2(b, c, n, 1, &out), "CalculateErrorL2");
+    return (DTYPE)out;
+}
+inline DTYPE CalculateErrorL2(const DTYPE* base1, const DTYPE* check1, int3 size1,
+                              const DTYPE* base2, const DTYPE* check2, int3 size2,
+                              const DTYPE* base3, const DTYPE* check3, int3 size3) {
+    const float* b[3] = {base1, base2, base3};
+    const float* c[3] = {check1, check2, check3};
+    const long long n[3] = {(long long)size1.x * size1.y * size1.z,
+                            (long long)size2.x * size2.y * size2.z,
+                            (long long)size3.x * size3.y * size3.z};
+    double out = 0.0;
+    check(whhd_error_l2(b, c, n, 3, &out), "CalculateErrorL2");
+    return (DTYPE)out;
+}
+
 }  // namespace whhd_compat

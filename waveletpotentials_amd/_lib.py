@@ -24,6 +24,11 @@ SIGNATURES = {
     "whhd_write_binary": (_I, [_S, _P, ctypes.c_longlong]),
     "whhd_read_binary": (_I, [_S, _P, ctypes.c_longlong]),
     "whhd_write_ply": (_I, [_S, _P, _I, _I, _I, _D, _D, _D, _D, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    "whhd_set_slice_zero": (_I, [_P, _I, _I, _I, ctypes.c_char]),
+    "whhd_gradient_stream3d": (_I, [_P] * 6 + [_I, _I, _I, _D, _D, _D]),
+    "whhd_gradient_stream2d": (_I, [_P] * 3 + [_I, _I, _D, _D]),
+    "whhd_error_l2": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P),
+                           ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(_D)]),
     "whhd_status_string": (_S, [_I]),
     "whhd_last_error": (_S, []),
     "whhd_version": (_S, []),

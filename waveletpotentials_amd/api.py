@@ -32,6 +32,7 @@ __all__ = ["WaveletType", "proj_bc", "proj_WaveletType", "type_from_string", "st
            "CuWaveletPotentialRecover3D", "CuWaveletPotentialRecover2D", "CuBatchRecover3D",
            "UNet3D", "WriteDataTxt", "WriteDataETxt", "ReadsDataTxt", "WriteDataTxt_sparse",
            "WriteVectorTxt_sparse", "WriteDataBinary", "ReadDataBinary", "WritePly",
+           "SetSliceZero", "StreamInitial", "CalculateErrorL2",
            "WhhdError",
            "mac_shapes3d", "mac_shapes2d"]
 
@@ -775,6 +776,100 @@ class CuParallelSweep3D(_Handle):
 
     def Project(self, qx, qy, qz):
         check(lib().whhd_sweep3d_project(self._h, *self._q(qx, qy, qz), _stream()), "Project")
+
+
+# ---- verification helpers on host arrays (WaveletPotentials/StreamInitial.cpp,
+# WaveletPotentials/Utils.cpp:184-331) ------------------------------------------
+def _f32c(a, name):
+    import numpy as np
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]):
+        raise TypeError(f"{name}: expected a C-contiguous float32 numpy array")
+    return a
+
+
+def SetSliceZero(val, dim, ns):
+    """SetSliceZero(val, dim, ns) (StreamInitial.cpp:5-43): zero the first and last
+    slices normal to axis ns of the (dim.x, dim.y, dim.z) float32 array `val`
+    (shape (nz, nx, ny)), in place."""
+    a = _f32c(val, "val")
+    nx, ny, nz = (int(v) for v in dim)
+    if a.size != nx * ny * nz:
+        raise ValueError("SetSliceZero: array size does not match dim")
+    check(lib().whhd_set_slice_zero(a.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, _b(ns)),
+          "SetSliceZero")
+
+
+class StreamInitial:
+    """StreamInitial::GetInstance()->GradientStream (StreamInitial.cpp:87-175): face
+    velocities from an edge potential (3D) or stream function (2D), on float32
+    numpy arrays written in place."""
+    _inst = None
+
+    @classmethod
+    def GetInstance(cls):
+        if cls._inst is None:
+            cls._inst = cls()
+        return cls._inst
+
+    def GradientStream(self, *args):
+        """(xv, yv, zv, qx, qy, qz, dim, dx) or (xv, yv, qz, dim, dx)."""
+        if len(args) == 8:
+            *arrs, dim, dx = args
+            names = ("xv", "yv", "zv", "qx", "qy", "qz")
+            nx, ny, nz = (int(v) for v in dim)
+            sizes = ((nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1),
+                     nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1), (nx + 1) * (ny + 1) * nz)
+            p = []
+            for a, nm, sz in zip(arrs, names, sizes):
+                a = _f32c(a, nm)
+                if a.size != sz:
+                    raise ValueError(f"GradientStream: {nm} has {a.size} elements, expected {sz}")
+                p.append(a.ctypes.data_as(ctypes.c_void_p))
+            check(lib().whhd_gradient_stream3d(*p, nx, ny, nz, *(float(v) for v in dx)),
+                  "GradientStream")
+        elif len(args) == 5:
+            xv, yv, qz, dim, dx = args
+            nx, ny = (int(v) for v in dim)
+            p = []
+            for a, nm, sz in zip((xv, yv, qz), ("xv", "yv", "qz"),
+                                 ((nx + 1) * ny, nx * (ny + 1), (nx + 1) * (ny + 1))):
+                a = _f32c(a, nm)
+                if a.size != sz:
+                    raise ValueError(f"GradientStream: {nm} has {a.size} elements, expected {sz}")
+                p.append(a.ctypes.data_as(ctypes.c_void_p))
+            check(lib().whhd_gradient_stream2d(*p, nx, ny, *(float(v) for v in dx)),
+                  "GradientStream")
+        else:
+            raise TypeError("GradientStream takes (xv, yv, zv, qx, qy, qz, dim, dx) or "
+                            "(xv, yv, qz, dim, dx)")
+
+
+def CalculateErrorL2(*args):
+    """CalculateErrorL2(base, check, size) or (base1, check1, size1, base2, check2,
+    size2, base3, check3, size3) (Utils.cpp:184-207, 296-331): the relative L2
+    error of `check` against `base` over float32 arrays."""
+    if len(args) not in (3, 9):
+        raise TypeError("CalculateErrorL2 takes 3 or 9 arguments")
+    k = len(args) // 3
+    B = (ctypes.c_void_p * k)()
+    C = (ctypes.c_void_p * k)()
+    N = (ctypes.c_longlong * k)()
+    keep = []
+    for i in range(k):
+        b, c, size = args[3 * i: 3 * i + 3]
+        n = 1
+        for v in size:
+            n *= int(v)
+        b, c = _f32c(b, "base"), _f32c(c, "check")
+        if b.size != n or c.size != n:
+            raise ValueError("CalculateErrorL2: array size does not match size")
+        keep += [b, c]
+        B[i] = b.ctypes.data
+        C[i] = c.ctypes.data
+        N[i] = n
+    out = ctypes.c_double(0.0)
+    check(lib().whhd_error_l2(B, C, N, k, ctypes.byref(out)), "CalculateErrorL2")
+    return out.value
 
 
 # ---- field I/O on host arrays (WaveletPotentials/Utils.cpp:812-990) ------------
