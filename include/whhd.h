@@ -377,6 +377,17 @@ WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* q
  * node outside, and q -= grad phi over every edge. */
 WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz, void* stream);
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h);
+/* CuParallelSweep3D(int2 dim, DTYPE2 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cu:
+ * 897-916) on nx*ny cells; release with whhd_sweep3d_destroy.  Only
+ * whhd_sweep2d_solve applies to it. */
+WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
+                                 whhd_sweep3d_t* out);
+/* Solve(qz, xv, yv) (WaveletPotentials/cuParallelSweep3D.cu:982-995): qz (nx+1, ny+1)
+ * integrates -yv along x from its x = 0 column: zero for bc 'n', the integral of
+ * xv along y for 'd' and 'z' (from qz(0, 0) = 0, which the reference leaves as
+ * the array held it). */
+WHHD_API int whhd_sweep2d_solve(whhd_sweep3d_t h, float* qz, const float* xv, const float* yv,
+                                void* stream);
 
 /* ---- verification helpers on host arrays ------------------------------- */
 /* SetSliceZero(val, dim, axis) (WaveletPotentials/StreamInitial.cpp:5-43): zero the

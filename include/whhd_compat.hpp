@@ -388,6 +388,12 @@ public:
         check(whhd_sweep3d_create(dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, bc, &h_),
               "CuParallelSweep3D");
     }
+    CuParallelSweep3D(int2 dim, DTYPE2 dx, char bc) {
+        check(whhd_sweep2d_create(dim.x, dim.y, dx.x, dx.y, bc, &h_), "CuParallelSweep3D");
+    }
+    void Solve(DTYPE* qz, DTYPE* xv, DTYPE* yv) {
+        check(whhd_sweep2d_solve(h_, qz, xv, yv, stream_), "Solve");
+    }
     void Solve(DTYPE* qx, DTYPE* qy, DTYPE* qz, DTYPE* xv, DTYPE* yv, DTYPE* zv) {
         check(whhd_sweep3d_solve(h_, qx, qy, qz, xv, yv, zv, stream_), "Solve");
     }

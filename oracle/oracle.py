@@ -72,6 +72,7 @@ def _setup(L):
     L.orc_dst2d_solve.argtypes = [_D, _D, i, i, d, d]
     L.orc_sweep3d_d.argtypes = [_D] * 6 + [i, i, i, d, d, d]
     L.orc_project_d.argtypes = [_D] * 3 + [i, i, i, d, d, d]
+    L.orc_sweep2d.argtypes = [_D] * 3 + [i, i, d, d, c]
     L.orc_project_zero.argtypes = [_D] * 3 + [i, i, i, d, d, d]
     L.orc_conv3.argtypes = [_D, _D, i, _D, i, i, i, i, i, _D, _D, i]
     L.orc_conv3_bwd_data.argtypes = [_D, _D, i, i, i, i, _D, i]
@@ -353,6 +354,17 @@ def sweep3d_d(xv, yv, zv, n, dx):
     _check(lib().orc_sweep3d_d(_p(qx), _p(qy), _p(qz), *map(_p, v), nx, ny, nz, *map(float, dx)),
            "sweep3d_d")
     return qx, qy, qz
+
+
+def sweep2d(xv, yv, n, dx, bc="n"):
+    """CuParallelSweep3D::Solve(qz, xv, yv): the 2D sweep; xv (nx+1, ny), yv (nx, ny+1)."""
+    nx, ny = n
+    qz = np.zeros((nx + 1, ny + 1))
+    xv = np.ascontiguousarray(xv, dtype=np.float64)
+    yv = np.ascontiguousarray(yv, dtype=np.float64)
+    _check(lib().orc_sweep2d(_p(qz), _p(xv), _p(yv), nx, ny, float(dx[0]), float(dx[1]), _b(bc)),
+           "sweep2d")
+    return qz
 
 
 def project_d(qx, qy, qz, n, dx):

@@ -167,6 +167,22 @@ int orc_project_d(double* qx, double* qy, double* qz, int nx, int ny, int nz, do
     return ORC_OK;
 }
 
+/* CuParallelSweep3D::Solve(qz, xv, yv), the 2D sweep
+ * (WaveletPotentials/cuParallelSweep3D.cu:982-995, kernels :87-142): qz (nx+1,
+ * ny+1) integrates -yv along x from its x = 0 column, which is 0 for bc 'n'
+ * (SweepingQz_x_0) and the integral of xv along y for 'd' / 'z' (SweepingQz_y,
+ * starting from qz(0, 0) = 0). */
+int orc_sweep2d(double* qz, const double* xv, const double* yv, int nx, int ny, double dx,
+                double dy, char bc) {
+    if (!qz || !xv || !yv || nx < 1 || ny < 1) return ORC_INVALID;
+    qz[0] = 0.0;
+    for (int y = 1; y <= ny; ++y) qz[y] = bc == 'n' ? 0.0 : qz[y - 1] + xv[y - 1] * dy;
+    for (int y = 0; y <= ny; ++y)
+        for (int x = 1; x <= nx; ++x)
+            qz[y + (size_t)x * (ny + 1)] = qz[y + (size_t)(x - 1) * (ny + 1)] - yv[y + (size_t)(x - 1) * (ny + 1)] * dx;
+    return ORC_OK;
+}
+
 /* CuDst::Solve (WaveletPotentials/cuDst.cu:207-311): the 2D DST-I Poisson solve,
  * (-L_h) phi = b for the 5-point Laplacian on an (n1, n2) node grid with zero
  * values one node outside; each axis uses its own spacing (the reference

@@ -141,6 +141,9 @@ int orc_dst2d_solve(double* phi, const double* b, int n1, int n2, double dx, dou
 /* CuParallelSweep3D::Solve for velocity bc 'd' ('z' takes the same branches). */
 int orc_sweep3d_d(double* qx, double* qy, double* qz, const double* xv, const double* yv,
                   const double* zv, int nx, int ny, int nz, double dx, double dy, double dz);
+/* CuParallelSweep3D::Solve(qz, xv, yv): the 2D sweep, bc 'n' or 'd' ('z' as 'd'). */
+int orc_sweep2d(double* qz, const double* xv, const double* yv, int nx, int ny, double dx,
+                double dy, char bc);
 /* CuParallelSweep3D::ProjectD: the projection on every node of the potential grid. */
 int orc_project_d(double* qx, double* qy, double* qz, int nx, int ny, int nz, double dx,
                   double dy, double dz);

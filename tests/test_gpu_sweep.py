@@ -114,3 +114,21 @@ def test_sweep_and_project_velocity_bc_d(whhd, orc, bc):
            + (qy[1:-1, 1:-1, 1:] - qy[1:-1, 1:-1, :-1]) / h[1]
            + (qz[1:, 1:-1, 1:-1] - qz[:-1, 1:-1, 1:-1]) / h[2])
     assert np.abs(div).max() <= 1e-4 * scale / min(h)
+
+
+@pytest.mark.parametrize("bc", ["n", "d", "z"])
+def test_sweep2d(whhd, orc, bc):
+    """The 2D sweep (CuParallelSweep3D(int2, ...).Solve(qz, xv, yv)) against the oracle."""
+    n = (64, 48)
+    nx, ny = n
+    dx = (1 / 64, 1 / 48)
+    rng = np.random.default_rng(30)
+    psi = rng.standard_normal((nx + 1, ny + 1))
+    u = [a[0].astype(np.float32).astype(np.float64) for a in orc.curl2d(psi, n, dx)]
+    sw = whhd.CuParallelSweep3D(n, dx, bc)
+    qz = torch.full((nx + 1, ny + 1), np.nan, device="cuda")
+    sw.Solve(qz, cuda(u[0]), cuda(u[1]))
+    ref = orc.sweep2d(u[0], u[1], n, dx, bc)
+    g = host(qz)
+    assert np.isfinite(g).all()
+    assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()

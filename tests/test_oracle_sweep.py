@@ -115,3 +115,23 @@ def test_project_d_keeps_curl_and_clears_interior_divergence(orc):
     assert np.abs(div).max() < 1e-9 * max(np.abs(a).max() for a in q) / min(h)
     for a, b in zip(orc.curl3d(*q, n, h), orc.curl3d(qx, qy, qz, n, h)):
         assert np.abs(a - b).max() < 1e-9 * np.abs(a).max()
+
+
+def test_sweep2d(orc):
+    """2D sweep: bc 'n' reproduces a divergence-free velocity whose flux through
+    the x = 0 wall is zero; bc 'd' (integrating xv along that wall first)
+    reproduces one with any wall flux, where 'n' then fails."""
+    n = (16, 12)
+    dx = (0.05, 0.08)
+    rng = np.random.default_rng(12)
+    psi = rng.standard_normal((n[0] + 1, n[1] + 1))
+    u = orc.curl2d(psi, n, dx)
+    psi0 = psi.copy()
+    psi0[0, :] = 0.0  # no flux through x = 0
+    u0 = orc.curl2d(psi0, n, dx)
+    for bc, vel, exact in (("n", u0, True), ("d", u0, True), ("d", u, True), ("n", u, False)):
+        q = orc.sweep2d(vel[0][0], vel[1][0], n, dx, bc)
+        c = orc.curl2d(q, n, dx)
+        err = max(np.abs(a - b).max() for a, b in zip(c, vel))
+        scale = max(np.abs(b).max() for b in vel)
+        assert (err < 1e-10 * scale) == exact, (bc, err)

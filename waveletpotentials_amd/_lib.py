@@ -25,6 +25,8 @@ SIGNATURES = {
     "whhd_read_binary": (_I, [_S, _P, ctypes.c_longlong]),
     "whhd_write_ply": (_I, [_S, _P, _I, _I, _I, _D, _D, _D, _D, _I, ctypes.POINTER(ctypes.c_longlong)]),
     "whhd_set_slice_zero": (_I, [_P, _I, _I, _I, ctypes.c_char]),
+    "whhd_sweep2d_create": (_I, [_I, _I, _D, _D, ctypes.c_char, ctypes.POINTER(_P)]),
+    "whhd_sweep2d_solve": (_I, [_P, _P, _P, _P, _P]),
     "whhd_gradient_stream3d": (_I, [_P] * 6 + [_I, _I, _I, _D, _D, _D]),
     "whhd_gradient_stream2d": (_I, [_P] * 3 + [_I, _I, _D, _D]),
     "whhd_error_l2": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P),

@@ -757,6 +757,10 @@ class CuParallelSweep3D(_Handle):
     def __init__(self, dim, dx, bc="n"):
         super().__init__()
         self.dim = tuple(int(v) for v in dim)
+        if len(self.dim) == 2:  # CuParallelSweep3D(int2, DTYPE2, char): the 2D sweep
+            check(lib().whhd_sweep2d_create(*self.dim, *(float(v) for v in dx), _b(bc),
+                                            ctypes.byref(self._h)), "CuParallelSweep3D")
+            return
         check(lib().whhd_sweep3d_create(*self.dim, *(float(v) for v in dx), _b(bc),
                                         ctypes.byref(self._h)), "CuParallelSweep3D")
         s = mac_shapes3d(*self.dim)
@@ -767,7 +771,17 @@ class CuParallelSweep3D(_Handle):
         return [_ptr(qx, numel=s["qx"], name="qx"), _ptr(qy, numel=s["qy"], name="qy"),
                 _ptr(qz, numel=s["qz"], name="qz")]
 
-    def Solve(self, qx, qy, qz, xv, yv, zv):
+    def Solve(self, *args):
+        """(qx, qy, qz, xv, yv, zv) for a 3D handle, (qz, xv, yv) for a 2D one."""
+        if len(self.dim) == 2:
+            qz, xv, yv = args
+            nx, ny = self.dim
+            check(lib().whhd_sweep2d_solve(self._h, _ptr(qz, numel=(nx + 1) * (ny + 1), name="qz"),
+                                           _ptr(xv, numel=(nx + 1) * ny, name="xv"),
+                                           _ptr(yv, numel=nx * (ny + 1), name="yv"), _stream()),
+                  "Solve")
+            return
+        qx, qy, qz, xv, yv, zv = args
         s = self._size
         check(lib().whhd_sweep3d_solve(self._h, *self._q(qx, qy, qz),
                                        _ptr(xv, numel=s["xv"], name="xv"),

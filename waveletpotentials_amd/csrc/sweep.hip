@@ -136,6 +136,26 @@ __global__ void grad_minus_d_kernel(float* __restrict__ qx, float* __restrict__ 
     if (z < nz) qz[ip] -= (phi[ip + sp] - p) * ihz;
 }
 
+// 2D sweep, CuParallelSweep3D::Solve(qz, xv, yv) (WaveletPotentials/cuParallelSweep3D.cu:
+// 982-995): one thread per y walks x, qz(x, y) = qz(x-1, y) - yv(x-1, y) dx
+// (SweepingQz_x).  bc 'n' starts from qz(0, y) = 0 (SweepingQz_x_0); bc 'd'
+// starts from the integral of xv along y at x = 0 (SweepingQz_y), summed in the
+// same order here, with qz(0, 0) = 0 where the reference keeps the array's value.
+__global__ void sweep2d_kernel(float* __restrict__ qz, const float* __restrict__ xv,
+                               const float* __restrict__ yv, int nx, int ny, float dx, float dy,
+                               int dbc) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y > ny) return;
+    float acc = 0.f;
+    if (dbc)
+        for (int k = 1; k <= y; ++k) acc = acc + xv[k - 1] * dy;
+    qz[y] = acc;
+    for (int x = 1; x <= nx; ++x) {
+        acc = acc - yv[y + (int64_t)(x - 1) * (ny + 1)] * dx;
+        qz[y + (int64_t)x * (ny + 1)] = acc;
+    }
+}
+
 // b = -div q at the interior nodes (GetB_3d_q_zeros); b is (nx-1, ny-1, nz-1).
 __global__ void node_div_kernel(float* __restrict__ b, const float* __restrict__ qx,
                                 const float* __restrict__ qy, const float* __restrict__ qz, int nx,
@@ -438,6 +458,7 @@ WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* q
     using namespace whhd;
     if (!h || !qx || !qy || !qz || !xv || !yv)
         return fail(WHHD_INVALID_ARG, "sweep3d_solve: null argument");
+    if (h->nz == 0) return fail(WHHD_INVALID_ARG, "sweep3d_solve: the handle is a 2D sweep");
     const bool d = h->bc == 'd';
     if (d && !zv) return fail(WHHD_INVALID_ARG, "sweep3d_solve: velocity bc 'd' needs zv");
     hipStream_t s = (hipStream_t)stream;
@@ -457,6 +478,7 @@ WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float*
                                   void* stream) {
     using namespace whhd;
     if (!h || !qx || !qy || !qz) return fail(WHHD_INVALID_ARG, "sweep3d_project: null argument");
+    if (h->nz == 0) return fail(WHHD_INVALID_ARG, "sweep3d_project: the handle is a 2D sweep");
     hipStream_t s = (hipStream_t)stream;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     const float ihx = (float)(1.0 / h->dx), ihy = (float)(1.0 / h->dy), ihz = (float)(1.0 / h->dz);
@@ -484,6 +506,36 @@ WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float*
                        (float)(1.0 / h->dz));
     e = done("grad_minus_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "gradient kernel");
+}
+
+WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
+                                 whhd_sweep3d_t* out) {
+    using namespace whhd;
+    if (!out) return fail(WHHD_INVALID_ARG, "sweep2d_create: null handle pointer");
+    *out = nullptr;
+    if (bc != 'n' && bc != 'd' && bc != 'z')
+        return fail(WHHD_INVALID_ARG, "sweep2d_create: velocity bc must be 'n', 'd' or 'z'");
+    if (nx < 1 || ny < 1 || (int64_t)(nx + 1) * (ny + 1) >= (1LL << 31))
+        return fail(WHHD_INVALID_ARG, "sweep2d_create: extents out of range");
+    if (!(dx > 0 && dy > 0)) return fail(WHHD_INVALID_ARG, "sweep2d_create: spacing <= 0");
+    auto* h = new whhd_sweep3d_s;
+    h->nx = nx; h->ny = ny; h->nz = 0;  // nz = 0 marks a 2D handle
+    h->bc = bc == 'n' ? 'n' : 'd';
+    h->dx = dx; h->dy = dy; h->dz = 1.0;
+    *out = h;
+    return WHHD_OK;
+}
+
+WHHD_API int whhd_sweep2d_solve(whhd_sweep3d_t h, float* qz, const float* xv, const float* yv,
+                                void* stream) {
+    using namespace whhd;
+    if (!h || !qz || !xv || !yv) return fail(WHHD_INVALID_ARG, "sweep2d_solve: null argument");
+    if (h->nz != 0) return fail(WHHD_INVALID_ARG, "sweep2d_solve: the handle is a 3D sweep");
+    hipLaunchKernelGGL(sweep2d_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, qz, xv, yv, h->nx, h->ny, (float)h->dx, (float)h->dy,
+                       h->bc == 'd' ? 1 : 0);
+    hipError_t e = done("sweep2d_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "2D sweep kernel");
 }
 
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h) {
