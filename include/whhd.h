@@ -349,6 +349,13 @@ WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, dou
  * dx.x*dx.y, the same for cubic cells).  phi may equal b. */
 WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream);
 WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h);
+/* FP64 forms of the DST solver (the reference built with USE_DOUBLE): a handle
+ * made by a *_f64 create is solved only by whhd_dst3d_solve_f64, and the reverse
+ * (WHHD_INVALID_ARG otherwise). */
+WHHD_API int whhd_dst3d_create_f64(int n1, int n2, int n3, double dx, double dy, double dz,
+                                   whhd_dst3d_t* out);
+WHHD_API int whhd_dst2d_create_f64(int n1, int n2, double dx, double dy, whhd_dst3d_t* out);
+WHHD_API int whhd_dst3d_solve_f64(whhd_dst3d_t h, double* phi, const double* b, void* stream);
 /* CuDst(int2 dim, DTYPE2 dx) (WaveletPotentials/cuDst.cu:4-36): the 2D solver on an
  * (n1, n2) node grid; solve with whhd_dst3d_solve (CuDst::Solve,
  * WaveletPotentials/cuDst.cu:207-311: (-L_h) phi = b for the 5-point Laplacian,
@@ -388,6 +395,19 @@ WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
  * the array held it). */
 WHHD_API int whhd_sweep2d_solve(whhd_sweep3d_t h, float* qz, const float* xv, const float* yv,
                                 void* stream);
+/* FP64 forms of the sweep comparator (USE_DOUBLE): same semantics on double
+ * arrays; a handle made by a *_f64 create takes only the *_f64 solves. */
+WHHD_API int whhd_sweep3d_create_f64(int nx, int ny, int nz, double dx, double dy, double dz,
+                                     char bc, whhd_sweep3d_t* out);
+WHHD_API int whhd_sweep3d_solve_f64(whhd_sweep3d_t h, double* qx, double* qy, double* qz,
+                                    const double* xv, const double* yv, const double* zv,
+                                    void* stream);
+WHHD_API int whhd_sweep3d_project_f64(whhd_sweep3d_t h, double* qx, double* qy, double* qz,
+                                      void* stream);
+WHHD_API int whhd_sweep2d_create_f64(int nx, int ny, double dx, double dy, char bc,
+                                     whhd_sweep3d_t* out);
+WHHD_API int whhd_sweep2d_solve_f64(whhd_sweep3d_t h, double* qz, const double* xv,
+                                    const double* yv, void* stream);
 
 /* ---- verification helpers on host arrays ------------------------------- */
 /* SetSliceZero(val, dim, axis) (WaveletPotentials/StreamInitial.cpp:5-43): zero the

@@ -132,3 +132,56 @@ def test_sweep2d(whhd, orc, bc):
     g = host(qz)
     assert np.isfinite(g).all()
     assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def cuda64(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("bc", ["n", "d"])
+def test_fp64_comparator_matches_oracle(whhd, orc, bc):
+    """The FP64 comparator (the reference's USE_DOUBLE build) against the FP64
+    oracle at 1e-12: sweep, projection, and the 3D and 2D DST solves."""
+    n = (24, 20, 28)
+    nx, ny, nz = n
+    h = (0.04, 0.05, 0.035)
+    rng = np.random.default_rng(40)
+    if bc == "n":
+        q0 = I.spectral_potential3d(n, 4, I.seed_for(4, 41), tuple(a * b for a, b in zip(n, h)))
+    else:
+        q0 = [rng.standard_normal(s) for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny),
+                                               (nz, nx + 1, ny + 1))]
+    u = I.curl3d(*q0, h)
+    sw = whhd.CuParallelSweep3D(n, h, bc, dtype=torch.float64)
+    q = [torch.full(s, np.nan, device="cuda", dtype=torch.float64)
+         for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny), (nz, nx + 1, ny + 1))]
+    sw.Solve(*q, *(cuda64(a) for a in u))
+    ref = orc.sweep3d(u[0], u[1], n, h) if bc == "n" else orc.sweep3d_d(*u, n, h)
+    for g, r in zip(q, ref):
+        assert np.abs(host(g) - r).max() <= 1e-12 * np.abs(r).max()
+    sw.Project(*q)
+    pref = orc.project_zero(*ref, n, h) if bc == "n" else orc.project_d(*ref, n, h)
+    scale = max(np.abs(r).max() for r in pref)
+    for g, r in zip(q, pref):
+        assert np.abs(host(g) - r).max() <= 1e-11 * scale
+    with pytest.raises(TypeError):
+        sw.Solve(*[a.float() for a in q], *(cuda(a) for a in u))
+    b = rng.standard_normal((7, 9, 5))
+    d3 = whhd.CuDst3D((9, 5, 7), (0.1, 0.2, 0.15), dtype=torch.float64)
+    phi = torch.empty((7, 9, 5), device="cuda", dtype=torch.float64)
+    d3.Solve(phi, cuda64(b))
+    r3 = orc.dst3d_solve(b, (0.1, 0.2, 0.15))
+    assert np.abs(host(phi) - r3).max() <= 1e-12 * np.abs(r3).max()
+    b2 = rng.standard_normal((33, 17))
+    d2 = whhd.CuDst((33, 17), (0.03, 0.07), dtype=torch.float64)
+    phi2 = torch.empty((33, 17), device="cuda", dtype=torch.float64)
+    d2.Solve(phi2, cuda64(b2))
+    r2 = orc.dst2d_solve(b2, (0.03, 0.07))
+    assert np.abs(host(phi2) - r2).max() <= 1e-12 * np.abs(r2).max()
+    s2 = whhd.CuParallelSweep3D((32, 24), (1 / 32, 1 / 24), bc, dtype=torch.float64)
+    psi = rng.standard_normal((33, 25))
+    v2 = [a[0] for a in orc.curl2d(psi, (32, 24), (1 / 32, 1 / 24))]
+    qz = torch.full((33, 25), np.nan, device="cuda", dtype=torch.float64)
+    s2.Solve(qz, cuda64(v2[0]), cuda64(v2[1]))
+    rz = orc.sweep2d(v2[0], v2[1], (32, 24), (1 / 32, 1 / 24), bc)
+    assert np.abs(host(qz) - rz).max() <= 1e-12 * np.abs(rz).max()

@@ -98,6 +98,14 @@ SIGNATURES = {
     "whhd_dst3d_create": (_I, [_I, _I, _I, _D, _D, _D, ctypes.POINTER(_P)]),
     "whhd_dst3d_solve": (_I, [_P, _P, _P, _P]),
     "whhd_dst3d_destroy": (None, [_P]),
+    "whhd_dst3d_create_f64": (_I, [_I, _I, _I, _D, _D, _D, ctypes.POINTER(_P)]),
+    "whhd_dst2d_create_f64": (_I, [_I, _I, _D, _D, ctypes.POINTER(_P)]),
+    "whhd_dst3d_solve_f64": (_I, [_P, _P, _P, _P]),
+    "whhd_sweep3d_create_f64": (_I, [_I, _I, _I, _D, _D, _D, ctypes.c_char, ctypes.POINTER(_P)]),
+    "whhd_sweep3d_solve_f64": (_I, [_P] * 8),
+    "whhd_sweep3d_project_f64": (_I, [_P] * 5),
+    "whhd_sweep2d_create_f64": (_I, [_I, _I, _D, _D, ctypes.c_char, ctypes.POINTER(_P)]),
+    "whhd_sweep2d_solve_f64": (_I, [_P] * 5),
     "whhd_dst2d_create": (_I, [_I, _I, _D, _D, ctypes.POINTER(_P)]),
     "whhd_sweep3d_create": (_I, [_I, _I, _I, _D, _D, _D, _C, ctypes.POINTER(_P)]),
     "whhd_sweep3d_solve": (_I, [_P] * 7 + [_P]),

@@ -715,81 +715,92 @@ def CudaFindMaxValue(src, size):
 
 
 class CuDst3D(_Handle):
-    """CuDst3D(int3 dim, DTYPE3 dx) (WaveletPotentials/cuDst3D.cuh): DST-I Poisson solver."""
+    """CuDst3D(int3 dim, DTYPE3 dx) (WaveletPotentials/cuDst3D.cuh): DST-I Poisson solver.
+    dtype torch.float64 builds the FP64 solver (the reference's USE_DOUBLE)."""
     _destroy = "whhd_dst3d_destroy"
 
-    def __init__(self, dim, dx):
+    def __init__(self, dim, dx, dtype=torch.float32):
         super().__init__()
         self.dim = tuple(int(v) for v in dim)
-        check(lib().whhd_dst3d_create(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)),
+        self.dtype = dtype
+        f = "whhd_dst3d_create_f64" if dtype == torch.float64 else "whhd_dst3d_create"
+        check(getattr(lib(), f)(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)),
               "CuDst3D")
 
     def Solve(self, dst, f):
         """Solve(dst, f): (-L_h) dst = f (WaveletPotentials/cuDst3D.cu:265-341)."""
-        n = self.dim[0] * self.dim[1] * self.dim[2]
-        check(lib().whhd_dst3d_solve(self._h, _ptr(dst, numel=n, name="dst"),
-                                     _ptr(f, numel=n, name="f"), _stream()), "CuDst3D.Solve")
+        n = 1
+        for v in self.dim:
+            n *= v
+        fn = "whhd_dst3d_solve_f64" if self.dtype == torch.float64 else "whhd_dst3d_solve"
+        check(getattr(lib(), fn)(self._h, _ptr(dst, self.dtype, numel=n, name="dst"),
+                                 _ptr(f, self.dtype, numel=n, name="f"), _stream()),
+              type(self).__name__ + ".Solve")
 
 
-class CuDst(_Handle):
+class CuDst(CuDst3D):
     """CuDst(int2 dim, DTYPE2 dx) (WaveletPotentials/cuDst.cuh): 2D DST-I Poisson solver
     on a dim[0] x dim[1] node grid (arrays (dim[0], dim[1]), y fastest)."""
-    _destroy = "whhd_dst3d_destroy"
 
-    def __init__(self, dim, dx):
-        super().__init__()
+    def __init__(self, dim, dx, dtype=torch.float32):
+        _Handle.__init__(self)
         self.dim = tuple(int(v) for v in dim)
-        check(lib().whhd_dst2d_create(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)),
-              "CuDst")
-
-    def Solve(self, dst, f):
-        """Solve(dst, f): (-L_h) dst = f (WaveletPotentials/cuDst.cu:207-311)."""
-        n = self.dim[0] * self.dim[1]
-        check(lib().whhd_dst3d_solve(self._h, _ptr(dst, numel=n, name="dst"),
-                                     _ptr(f, numel=n, name="f"), _stream()), "CuDst.Solve")
+        self.dtype = dtype
+        f = "whhd_dst2d_create_f64" if dtype == torch.float64 else "whhd_dst2d_create"
+        check(getattr(lib(), f)(*self.dim, *(float(v) for v in dx), ctypes.byref(self._h)), "CuDst")
 
 
 class CuParallelSweep3D(_Handle):
     """CuParallelSweep3D(int3 dim, DTYPE3 dx, char bc) (WaveletPotentials/cuParallelSweep3D.cuh):
-    the sweep + DST comparator, velocity bc 'n', 'd' or 'z' ('z' as 'd')."""
+    the sweep + DST comparator, velocity bc 'n', 'd' or 'z' ('z' as 'd'); a 2-element
+    dim makes the 2D sweep.  dtype torch.float64 builds the FP64 comparator."""
     _destroy = "whhd_sweep3d_destroy"
 
-    def __init__(self, dim, dx, bc="n"):
+    def __init__(self, dim, dx, bc="n", dtype=torch.float32):
         super().__init__()
         self.dim = tuple(int(v) for v in dim)
+        self.dtype = dtype
+        sfx = "_f64" if dtype == torch.float64 else ""
         if len(self.dim) == 2:  # CuParallelSweep3D(int2, DTYPE2, char): the 2D sweep
-            check(lib().whhd_sweep2d_create(*self.dim, *(float(v) for v in dx), _b(bc),
-                                            ctypes.byref(self._h)), "CuParallelSweep3D")
+            check(getattr(lib(), "whhd_sweep2d_create" + sfx)(
+                *self.dim, *(float(v) for v in dx), _b(bc), ctypes.byref(self._h)),
+                "CuParallelSweep3D")
             return
-        check(lib().whhd_sweep3d_create(*self.dim, *(float(v) for v in dx), _b(bc),
-                                        ctypes.byref(self._h)), "CuParallelSweep3D")
+        check(getattr(lib(), "whhd_sweep3d_create" + sfx)(
+            *self.dim, *(float(v) for v in dx), _b(bc), ctypes.byref(self._h)), "CuParallelSweep3D")
         s = mac_shapes3d(*self.dim)
         self._size = {k: int(torch.Size(v).numel()) for k, v in s.items()}
 
+    def _sfx(self):
+        return "_f64" if self.dtype == torch.float64 else ""
+
     def _q(self, qx, qy, qz):
         s = self._size
-        return [_ptr(qx, numel=s["qx"], name="qx"), _ptr(qy, numel=s["qy"], name="qy"),
-                _ptr(qz, numel=s["qz"], name="qz")]
+        d = self.dtype
+        return [_ptr(qx, d, numel=s["qx"], name="qx"), _ptr(qy, d, numel=s["qy"], name="qy"),
+                _ptr(qz, d, numel=s["qz"], name="qz")]
 
     def Solve(self, *args):
         """(qx, qy, qz, xv, yv, zv) for a 3D handle, (qz, xv, yv) for a 2D one."""
+        d = self.dtype
         if len(self.dim) == 2:
             qz, xv, yv = args
             nx, ny = self.dim
-            check(lib().whhd_sweep2d_solve(self._h, _ptr(qz, numel=(nx + 1) * (ny + 1), name="qz"),
-                                           _ptr(xv, numel=(nx + 1) * ny, name="xv"),
-                                           _ptr(yv, numel=nx * (ny + 1), name="yv"), _stream()),
-                  "Solve")
+            check(getattr(lib(), "whhd_sweep2d_solve" + self._sfx())(
+                self._h, _ptr(qz, d, numel=(nx + 1) * (ny + 1), name="qz"),
+                _ptr(xv, d, numel=(nx + 1) * ny, name="xv"),
+                _ptr(yv, d, numel=nx * (ny + 1), name="yv"), _stream()), "Solve")
             return
         qx, qy, qz, xv, yv, zv = args
         s = self._size
-        check(lib().whhd_sweep3d_solve(self._h, *self._q(qx, qy, qz),
-                                       _ptr(xv, numel=s["xv"], name="xv"),
-                                       _ptr(yv, numel=s["yv"], name="yv"),
-                                       _ptr(zv, numel=s["zv"], name="zv"), _stream()), "Solve")
+        check(getattr(lib(), "whhd_sweep3d_solve" + self._sfx())(
+            self._h, *self._q(qx, qy, qz), _ptr(xv, d, numel=s["xv"], name="xv"),
+            _ptr(yv, d, numel=s["yv"], name="yv"), _ptr(zv, d, numel=s["zv"], name="zv"),
+            _stream()), "Solve")
 
     def Project(self, qx, qy, qz):
-        check(lib().whhd_sweep3d_project(self._h, *self._q(qx, qy, qz), _stream()), "Project")
+        check(getattr(lib(), "whhd_sweep3d_project" + self._sfx())(
+            self._h, *self._q(qx, qy, qz), _stream()), "Project")
 
 
 # ---- verification helpers on host arrays (WaveletPotentials/StreamInitial.cpp,

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "whhd_internal.h"
@@ -24,9 +25,10 @@ namespace {
 // column (y < ny) are running sums along z (SweepingQx / SweepingQy), the qz
 // column is cleared.  qy starts from zero, or with qy_base from the bottom
 // plane already in qy (velocity bc 'd').
-__global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
-                             float* __restrict__ qz, const float* __restrict__ xv,
-                             const float* __restrict__ yv, int nx, int ny, int nz, float dz,
+template <class T>
+__global__ void sweep_kernel(T* __restrict__ qx, T* __restrict__ qy,
+                             T* __restrict__ qz, const T* __restrict__ xv,
+                             const T* __restrict__ yv, int nx, int ny, int nz, T dz,
                              int qy_base) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
@@ -34,8 +36,8 @@ __global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
     if (x < nx) {  // qx (nx, ny+1, nz+1) from yv (nx, ny+1, nz)
         const int64_t s = (int64_t)nx * (ny + 1);
         const int64_t b = y + (int64_t)x * (ny + 1);
-        float acc = 0.f;
-        qx[b] = 0.f;
+        T acc = T(0);
+        qx[b] = T(0);
 #pragma unroll 8
         for (int z = 1; z <= nz; ++z) {
             acc += yv[b + (z - 1) * s] * dz;
@@ -45,7 +47,7 @@ __global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
     if (y < ny) {  // qy (nx+1, ny, nz+1) from xv (nx+1, ny, nz)
         const int64_t s = (int64_t)(nx + 1) * ny;
         const int64_t b = y + (int64_t)x * ny;
-        float acc = qy_base ? qy[b] : 0.f;
+        T acc = qy_base ? qy[b] : T(0);
         qy[b] = acc;
 #pragma unroll 8
         for (int z = 1; z <= nz; ++z) {
@@ -56,40 +58,42 @@ __global__ void sweep_kernel(float* __restrict__ qx, float* __restrict__ qy,
     {  // qz (nx+1, ny+1, nz) = 0
         const int64_t s = (int64_t)(nx + 1) * (ny + 1);
         const int64_t b = y + (int64_t)x * (ny + 1);
-        for (int z = 0; z < nz; ++z) qz[b + z * s] = 0.f;
+        for (int z = 0; z < nz; ++z) qz[b + z * s] = T(0);
     }
 }
 
 // Top gauge for velocity bc 'n' (cuSweepingQ_x and the clears that follow it):
 // per y, qp(x) = sum_{x' < x} qx(x', y, top) * dx; the top planes of qx and qy
 // become zero and the top plane of qz takes -qp / dz.
-__global__ void top_gauge_kernel(float* __restrict__ qx, float* __restrict__ qy,
-                                 float* __restrict__ qz, int nx, int ny, int nz, float dx,
-                                 float dz) {
+template <class T>
+__global__ void top_gauge_kernel(T* __restrict__ qx, T* __restrict__ qy,
+                                 T* __restrict__ qz, int nx, int ny, int nz, T dx,
+                                 T dz) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y > ny) return;
     const int64_t tqx = (int64_t)nz * nx * (ny + 1);        // top plane of qx
     const int64_t tqy = (int64_t)nz * (nx + 1) * ny;        // top plane of qy
     const int64_t tqz = (int64_t)(nz - 1) * (nx + 1) * (ny + 1);  // top plane of qz
-    float qp = 0.f;
+    T qp = T(0);
     for (int x = 1; x <= nx; ++x) {
-        float* e = qx + tqx + y + (int64_t)(x - 1) * (ny + 1);
+        T* e = qx + tqx + y + (int64_t)(x - 1) * (ny + 1);
         qp += *e * dx;
-        *e = 0.f;
+        *e = T(0);
         qz[tqz + y + (int64_t)x * (ny + 1)] -= qp / dz;
     }
     if (y < ny)
-        for (int x = 0; x <= nx; ++x) qy[tqy + y + (int64_t)x * ny] = 0.f;
+        for (int x = 0; x <= nx; ++x) qy[tqy + y + (int64_t)x * ny] = T(0);
 }
 
 // Velocity bc 'd' bottom plane (SweepingQy_x, WaveletPotentials/cuParallelSweep3D.cu:
 // 68-85): per y, qy(x, y, 0) = sum_{x' < x} zv(x', y, 0) dx.
-__global__ void bottom_qy_kernel(float* __restrict__ qy, const float* __restrict__ zv, int nx,
-                                 int ny, float dx) {
+template <class T>
+__global__ void bottom_qy_kernel(T* __restrict__ qy, const T* __restrict__ zv, int nx,
+                                 int ny, T dx) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= ny) return;
-    float acc = 0.f;
-    qy[y] = 0.f;
+    T acc = T(0);
+    qy[y] = T(0);
     for (int x = 1; x <= nx; ++x) {
         acc += zv[y + (int64_t)(x - 1) * ny] * dx;
         qy[y + (int64_t)x * ny] = acc;
@@ -100,9 +104,10 @@ __global__ void bottom_qy_kernel(float* __restrict__ qy, const float* __restrict
 // 161-200): b = -div q at every node (nx+1, ny+1, nz+1); at a wall the edge
 // index is clamped, so the missing edge repeats the wall edge and that axis
 // contributes nothing.
-__global__ void node_div_d_kernel(float* __restrict__ b, const float* __restrict__ qx,
-                                  const float* __restrict__ qy, const float* __restrict__ qz,
-                                  int nx, int ny, int nz, float ihx, float ihy, float ihz) {
+template <class T>
+__global__ void node_div_d_kernel(T* __restrict__ b, const T* __restrict__ qx,
+                                  const T* __restrict__ qy, const T* __restrict__ qz,
+                                  int nx, int ny, int nz, T ihx, T ihy, T ihz) {
     const int i = blockIdx.x * 64 + threadIdx.x;  // y
     const int j = blockIdx.y * 4 + threadIdx.y;   // x
     const int k = blockIdx.z;                     // z
@@ -114,15 +119,16 @@ __global__ void node_div_d_kernel(float* __restrict__ b, const float* __restrict
     auto QX = [&](int x) { return qx[i + x * (NY + 1) + k * NX * (NY + 1)]; };
     auto QY = [&](int yy) { return qy[yy + j * NY + k * (NX + 1) * NY]; };
     auto QZ = [&](int z) { return qz[i + j * (NY + 1) + z * (NX + 1) * (NY + 1)]; };
-    const float d = (QX(jr) - QX(jl)) * ihx + (QY(it) - QY(ib)) * ihy + (QZ(kb) - QZ(kf)) * ihz;
+    const T d = (QX(jr) - QX(jl)) * ihx + (QY(it) - QY(ib)) * ihy + (QZ(kb) - QZ(kf)) * ihz;
     b[i + j * (NY + 1) + k * (NX + 1) * (NY + 1)] = -d;
 }
 
 // q -= grad phi over every edge between two nodes (cuGradientMinus3D_d_Q,
 // WaveletPotentials/cuGradient.cu:2408-2446); phi is (nx+1, ny+1, nz+1).
-__global__ void grad_minus_d_kernel(float* __restrict__ qx, float* __restrict__ qy,
-                                    float* __restrict__ qz, const float* __restrict__ phi, int nx,
-                                    int ny, int nz, float ihx, float ihy, float ihz) {
+template <class T>
+__global__ void grad_minus_d_kernel(T* __restrict__ qx, T* __restrict__ qy,
+                                    T* __restrict__ qz, const T* __restrict__ phi, int nx,
+                                    int ny, int nz, T ihx, T ihy, T ihz) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
@@ -130,7 +136,7 @@ __global__ void grad_minus_d_kernel(float* __restrict__ qx, float* __restrict__ 
     const int64_t NX = nx, NY = ny;
     const int64_t sp = (NX + 1) * (NY + 1);
     const int64_t ip = y + x * (NY + 1) + z * sp;
-    const float p = phi[ip];
+    const T p = phi[ip];
     if (x < nx) qx[y + x * (NY + 1) + z * NX * (NY + 1)] -= (phi[ip + NY + 1] - p) * ihx;
     if (y < ny) qy[y + x * NY + z * (NX + 1) * NY] -= (phi[ip + 1] - p) * ihy;
     if (z < nz) qz[ip] -= (phi[ip + sp] - p) * ihz;
@@ -141,12 +147,13 @@ __global__ void grad_minus_d_kernel(float* __restrict__ qx, float* __restrict__ 
 // (SweepingQz_x).  bc 'n' starts from qz(0, y) = 0 (SweepingQz_x_0); bc 'd'
 // starts from the integral of xv along y at x = 0 (SweepingQz_y), summed in the
 // same order here, with qz(0, 0) = 0 where the reference keeps the array's value.
-__global__ void sweep2d_kernel(float* __restrict__ qz, const float* __restrict__ xv,
-                               const float* __restrict__ yv, int nx, int ny, float dx, float dy,
+template <class T>
+__global__ void sweep2d_kernel(T* __restrict__ qz, const T* __restrict__ xv,
+                               const T* __restrict__ yv, int nx, int ny, T dx, T dy,
                                int dbc) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y > ny) return;
-    float acc = 0.f;
+    T acc = T(0);
     if (dbc)
         for (int k = 1; k <= y; ++k) acc = acc + xv[k - 1] * dy;
     qz[y] = acc;
@@ -157,9 +164,10 @@ __global__ void sweep2d_kernel(float* __restrict__ qz, const float* __restrict__
 }
 
 // b = -div q at the interior nodes (GetB_3d_q_zeros); b is (nx-1, ny-1, nz-1).
-__global__ void node_div_kernel(float* __restrict__ b, const float* __restrict__ qx,
-                                const float* __restrict__ qy, const float* __restrict__ qz, int nx,
-                                int ny, int nz, float ihx, float ihy, float ihz) {
+template <class T>
+__global__ void node_div_kernel(T* __restrict__ b, const T* __restrict__ qx,
+                                const T* __restrict__ qy, const T* __restrict__ qz, int nx,
+                                int ny, int nz, T ihx, T ihy, T ihz) {
     const int i = blockIdx.x * 64 + threadIdx.x;  // y - 1
     const int j = blockIdx.y * 4 + threadIdx.y;   // x - 1
     const int k = blockIdx.z;                     // z - 1
@@ -168,7 +176,7 @@ __global__ void node_div_kernel(float* __restrict__ b, const float* __restrict__
     const int64_t iqx = (i + 1) + (int64_t)j * (ny + 1) + (int64_t)(k + 1) * nx * (ny + 1);
     const int64_t iqy = i + (int64_t)(j + 1) * ny + (int64_t)(k + 1) * (nx + 1) * ny;
     const int64_t iqz = (i + 1) + (int64_t)(j + 1) * (ny + 1) + (int64_t)k * (nx + 1) * (ny + 1);
-    const float d = (qx[iqx + (ny + 1)] - qx[iqx]) * ihx + (qy[iqy + 1] - qy[iqy]) * ihy +
+    const T d = (qx[iqx + (ny + 1)] - qx[iqx]) * ihx + (qy[iqy + 1] - qy[iqy]) * ihy +
                     (qz[iqz + (int64_t)(nx + 1) * (ny + 1)] - qz[iqz]) * ihz;
     b[i + (int64_t)j * py + (int64_t)k * px * py] = -d;
 }
@@ -176,20 +184,22 @@ __global__ void node_div_kernel(float* __restrict__ b, const float* __restrict__
 // q -= grad phi with phi = 0 on the boundary nodes (GradientMinus3D_zero_Q):
 // one thread per node (x <= nx, y <= ny, z <= nz), owning the qx, qy, qz edges
 // that start at it.
-__device__ __forceinline__ float phi_at(const float* __restrict__ phi, int x, int y, int z, int nx,
+template <class T>
+__device__ __forceinline__ T phi_at(const T* __restrict__ phi, int x, int y, int z, int nx,
                                         int ny, int nz) {
-    if (x < 1 || y < 1 || z < 1 || x > nx - 1 || y > ny - 1 || z > nz - 1) return 0.f;
+    if (x < 1 || y < 1 || z < 1 || x > nx - 1 || y > ny - 1 || z > nz - 1) return T(0);
     return phi[(y - 1) + (int64_t)(x - 1) * (ny - 1) + (int64_t)(z - 1) * (nx - 1) * (ny - 1)];
 }
 
-__global__ void grad_minus_kernel(float* __restrict__ qx, float* __restrict__ qy,
-                                  float* __restrict__ qz, const float* __restrict__ phi, int nx,
-                                  int ny, int nz, float ihx, float ihy, float ihz) {
+template <class T>
+__global__ void grad_minus_kernel(T* __restrict__ qx, T* __restrict__ qy,
+                                  T* __restrict__ qz, const T* __restrict__ phi, int nx,
+                                  int ny, int nz, T ihx, T ihy, T ihz) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
     if (y > ny || x > nx) return;
-    const float p = phi_at(phi, x, y, z, nx, ny, nz);
+    const T p = phi_at(phi, x, y, z, nx, ny, nz);
     if (x < nx)
         qx[y + (int64_t)x * (ny + 1) + (int64_t)z * nx * (ny + 1)] -=
             (phi_at(phi, x + 1, y, z, nx, ny, nz) - p) * ihx;
@@ -206,21 +216,22 @@ __global__ void grad_minus_kernel(float* __restrict__ qx, float* __restrict__ qy
 // a 64 (i) x 64 (k) tile, each thread 4 x 4, with 16-wide j slices staged in LDS.
 constexpr int kTI = 64, kTK = 64, kTJ = 16;
 
-__global__ void __launch_bounds__(256) dst_pass_kernel(float* __restrict__ out,
-                                                       const float* __restrict__ in,
-                                                       const float* __restrict__ S, int n, int m,
+template <class T>
+__global__ void __launch_bounds__(256) dst_pass_kernel(T* __restrict__ out,
+                                                       const T* __restrict__ in,
+                                                       const T* __restrict__ S, int n, int m,
                                                        int64_t si, int64_t sj, int64_t sb) {
-    __shared__ float As[kTJ][kTI + 1];
-    __shared__ float Bs[kTJ][kTK + 1];
+    __shared__ T As[kTJ][kTI + 1];
+    __shared__ T Bs[kTJ][kTK + 1];
     const int i0 = blockIdx.x * kTI, k0 = blockIdx.y * kTK;
     const int64_t boff = (int64_t)blockIdx.z * sb;
     const int t = threadIdx.x;
     const int ti = (t & 15) * 4, tk = (t >> 4) * 4;
-    float acc[4][4];
+    T acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = 0.f;
+        for (int c = 0; c < 4; ++c) acc[r][c] = T(0);
     const bool jfast = (sj == 1);
     for (int j0 = 0; j0 < n; j0 += kTJ) {
 #pragma unroll
@@ -229,15 +240,15 @@ __global__ void __launch_bounds__(256) dst_pass_kernel(float* __restrict__ out,
             const int ii = jfast ? idx / kTJ : idx % kTI;
             const int jj = jfast ? idx % kTJ : idx / kTI;
             const int gi = i0 + ii, gj = j0 + jj;
-            As[jj][ii] = (gi < m && gj < n) ? in[boff + gi * si + gj * sj] : 0.f;
+            As[jj][ii] = (gi < m && gj < n) ? in[boff + gi * si + gj * sj] : T(0);
             const int kk = idx / kTJ, jb = idx % kTJ;
             const int gk = k0 + kk, gjb = j0 + jb;
-            Bs[jb][kk] = (gk < n && gjb < n) ? S[(int64_t)gk * n + gjb] : 0.f;
+            Bs[jb][kk] = (gk < n && gjb < n) ? S[(int64_t)gk * n + gjb] : T(0);
         }
         __syncthreads();
 #pragma unroll
         for (int jj = 0; jj < kTJ; ++jj) {
-            float a[4], b[4];
+            T a[4], b[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = As[jj][ti + r];
 #pragma unroll
@@ -259,8 +270,9 @@ __global__ void __launch_bounds__(256) dst_pass_kernel(float* __restrict__ out,
 }
 
 // phi_hat /= lambda_x + lambda_y + lambda_z (cuDstProj_dx)
-__global__ void eigen_scale_kernel(float* __restrict__ a, const float* __restrict__ lx,
-                                   const float* __restrict__ ly, const float* __restrict__ lz,
+template <class T>
+__global__ void eigen_scale_kernel(T* __restrict__ a, const T* __restrict__ lx,
+                                   const T* __restrict__ ly, const T* __restrict__ lz,
                                    int n1, int n2, int n3) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
@@ -279,20 +291,24 @@ hipError_t done(const char* name) {
 }  // namespace
 }  // namespace whhd
 
+// Handles hold their tables and scratch in FP32 or, for the *_f64 entry
+// points (the reference built with USE_DOUBLE), in FP64.
 struct whhd_dst3d_s {
     int n1, n2, n3;
     bool planar = false;  // 2D solver (CuDst): no z passes, no z eigenvalue
-    float* S[3] = {nullptr, nullptr, nullptr};
-    float* lam[3] = {nullptr, nullptr, nullptr};
-    float* tmp = nullptr;
+    bool fp64 = false;
+    void* S[3] = {nullptr, nullptr, nullptr};
+    void* lam[3] = {nullptr, nullptr, nullptr};
+    void* tmp = nullptr;
 };
 
 struct whhd_sweep3d_s {
-    int nx, ny, nz;
-    char bc;  // 'n', or 'd' ('z' behaves as 'd')
+    int nx, ny, nz;       // nz = 0: a 2D sweep
+    char bc;              // 'n', or 'd' ('z' behaves as 'd')
+    bool fp64 = false;
     double dx, dy, dz;
     whhd_dst3d_s* dst = nullptr;
-    float* b = nullptr;
+    void* b = nullptr;
 };
 
 namespace whhd {
@@ -308,8 +324,12 @@ void dst_release(whhd_dst3d_s* h) {
     delete h;
 }
 
+template <class T>
+constexpr bool is64() { return sizeof(T) == sizeof(double); }
+
 // one DST-I pass along axis a (0: x, 1: y, 2: z) of an (n1, n2, n3) y-fastest array
-int dst_pass(const whhd_dst3d_s* h, float* out, const float* in, int a, hipStream_t s) {
+template <class T>
+int dst_pass(const whhd_dst3d_s* h, T* out, const T* in, int a, hipStream_t s) {
     const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
     int n, m;
     int64_t si, sj, sb;
@@ -322,7 +342,8 @@ int dst_pass(const whhd_dst3d_s* h, float* out, const float* in, int a, hipStrea
         n = n3; m = n1 * n2; si = 1; sj = (int64_t)n1 * n2; sb = 0; nb = 1;
     }
     dim3 grid((unsigned)((m + kTI - 1) / kTI), (unsigned)((n + kTK - 1) / kTK), nb);
-    hipLaunchKernelGGL(dst_pass_kernel, grid, dim3(256), 0, s, out, in, h->S[a], n, m, si, sj, sb);
+    hipLaunchKernelGGL(dst_pass_kernel<T>, grid, dim3(256), 0, s, out, in, (const T*)h->S[a], n, m,
+                       si, sj, sb);
     hipError_t e = done("dst_pass_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "dst pass kernel");
 }
@@ -331,7 +352,9 @@ int dst_pass(const whhd_dst3d_s* h, float* out, const float* in, int a, hipStrea
 
 }  // namespace
 
-int dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz, whhd_dst3d_s** out) {
+template <class T>
+int dst3d_create_t(int n1, int n2, int n3, double dx, double dy, double dz, bool planar,
+                   whhd_dst3d_s** out) {
     if (!out) return fail(WHHD_INVALID_ARG, "dst3d_create: null handle pointer");
     *out = nullptr;
     if (n1 < 1 || n2 < 1 || n3 < 1 || n3 > 65535 || (int64_t)n1 * n2 * n3 >= (1LL << 31))
@@ -339,28 +362,32 @@ int dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz, whhd_d
     if (!(dx > 0 && dy > 0 && dz > 0)) return fail(WHHD_INVALID_ARG, "dst3d_create: spacing <= 0");
     auto* h = new whhd_dst3d_s;
     h->n1 = n1; h->n2 = n2; h->n3 = n3;
+    h->planar = planar;
+    h->fp64 = is64<T>();
     const int ns[3] = {n1, n2, n3};
     const double hs[3] = {dx, dy, dz};
     for (int a = 0; a < 3; ++a) {
         const int n = ns[a];
         // orthonormal DST-I, S(k, j) = sqrt(2/(n+1)) sin(pi (k+1)(j+1) / (n+1)), and the
-        // eigenvalues (2 - 2 cos(pi (k+1)/(n+1))) / h^2 of the 3-point -d2/dx2
-        std::vector<float> S((size_t)n * n), lam(n);
+        // eigenvalues (2 - 2 cos(pi (k+1)/(n+1))) / h^2 of the 3-point -d2/dx2 (none
+        // on z for the 2D solver)
+        std::vector<T> S((size_t)n * n), lam(n);
         const double sc = std::sqrt(2.0 / (n + 1));
         for (int k = 0; k < n; ++k) {
             for (int j = 0; j < n; ++j)
-                S[(size_t)k * n + j] = (float)(sc * std::sin(M_PI * (double)((int64_t)(k + 1) * (j + 1) % (2 * (n + 1))) / (n + 1)));
-            lam[k] = (float)((2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1))) / (hs[a] * hs[a]));
+                S[(size_t)k * n + j] = (T)(sc * std::sin(M_PI * (double)((int64_t)(k + 1) * (j + 1) % (2 * (n + 1))) / (n + 1)));
+            lam[k] = (planar && a == 2) ? T(0)
+                                        : (T)((2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1))) / (hs[a] * hs[a]));
         }
-        if (hipMalloc(&h->S[a], S.size() * 4) != hipSuccess ||
-            hipMalloc(&h->lam[a], lam.size() * 4) != hipSuccess ||
-            hipMemcpy(h->S[a], S.data(), S.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->lam[a], lam.data(), lam.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMalloc(&h->S[a], S.size() * sizeof(T)) != hipSuccess ||
+            hipMalloc(&h->lam[a], lam.size() * sizeof(T)) != hipSuccess ||
+            hipMemcpy(h->S[a], S.data(), S.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->lam[a], lam.data(), lam.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
             dst_release(h);
             return fail(WHHD_OOM, "dst3d_create: table allocation failed");
         }
     }
-    if (hipMalloc(&h->tmp, (size_t)n1 * n2 * n3 * 4) != hipSuccess) {
+    if (hipMalloc(&h->tmp, (size_t)n1 * n2 * n3 * sizeof(T)) != hipSuccess) {
         dst_release(h);
         return fail(WHHD_OOM, "dst3d_create: scratch allocation failed");
     }
@@ -368,64 +395,43 @@ int dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz, whhd_d
     return WHHD_OK;
 }
 
-int dst3d_solve(const whhd_dst3d_s* h, float* phi, const float* b, hipStream_t s) {
+template <class T>
+int dst3d_solve_t(const whhd_dst3d_s* h, T* phi, const T* b, hipStream_t s) {
     if (!h || !phi || !b) return fail(WHHD_INVALID_ARG, "dst3d_solve: null argument");
+    if (h->fp64 != is64<T>())
+        return fail(WHHD_INVALID_ARG, is64<T>() ? "dst solve (FP64): the handle was created for FP32"
+                                                : "dst solve (FP32): the handle was created for FP64");
+    T* tmp = (T*)h->tmp;
+    const T* const* lam = (const T* const*)h->lam;
     if (h->planar) {
         // b -> tmp (x) -> phi (y), scale, phi -> tmp (y) -> phi (x)
-        TRY(dst_pass(h, h->tmp, b, 0, s));
-        TRY(dst_pass(h, phi, h->tmp, 1, s));
-        hipLaunchKernelGGL(eigen_scale_kernel, grid_xyz(h->n1, h->n2, 1), dim3(64, 4), 0, s, phi,
-                           h->lam[0], h->lam[1], h->lam[2], h->n1, h->n2, 1);
+        TRY(dst_pass<T>(h, tmp, b, 0, s));
+        TRY(dst_pass<T>(h, phi, tmp, 1, s));
+        hipLaunchKernelGGL(eigen_scale_kernel<T>, grid_xyz(h->n1, h->n2, 1), dim3(64, 4), 0, s, phi,
+                           lam[0], lam[1], lam[2], h->n1, h->n2, 1);
         hipError_t e = done("eigen_scale_kernel");
         if (e != hipSuccess) return hip_fail(e, "eigen scale kernel");
-        TRY(dst_pass(h, h->tmp, phi, 1, s));
-        return dst_pass(h, phi, h->tmp, 0, s);
+        TRY(dst_pass<T>(h, tmp, phi, 1, s));
+        return dst_pass<T>(h, phi, tmp, 0, s);
     }
     // b -> tmp (x) -> phi (y) -> tmp (z), scale, tmp -> phi (z) -> tmp (y) -> phi (x)
-    TRY(dst_pass(h, h->tmp, b, 0, s));
-    TRY(dst_pass(h, phi, h->tmp, 1, s));
-    TRY(dst_pass(h, h->tmp, phi, 2, s));
-    hipLaunchKernelGGL(eigen_scale_kernel, grid_xyz(h->n1, h->n2, h->n3), dim3(64, 4), 0, s, h->tmp,
-                       h->lam[0], h->lam[1], h->lam[2], h->n1, h->n2, h->n3);
+    TRY(dst_pass<T>(h, tmp, b, 0, s));
+    TRY(dst_pass<T>(h, phi, tmp, 1, s));
+    TRY(dst_pass<T>(h, tmp, phi, 2, s));
+    hipLaunchKernelGGL(eigen_scale_kernel<T>, grid_xyz(h->n1, h->n2, h->n3), dim3(64, 4), 0, s, tmp,
+                       lam[0], lam[1], lam[2], h->n1, h->n2, h->n3);
     hipError_t e = done("eigen_scale_kernel");
     if (e != hipSuccess) return hip_fail(e, "eigen scale kernel");
-    TRY(dst_pass(h, phi, h->tmp, 2, s));
-    TRY(dst_pass(h, h->tmp, phi, 1, s));
-    return dst_pass(h, phi, h->tmp, 0, s);
+    TRY(dst_pass<T>(h, phi, tmp, 2, s));
+    TRY(dst_pass<T>(h, tmp, phi, 1, s));
+    return dst_pass<T>(h, phi, tmp, 0, s);
 }
 
-}  // namespace whhd
+namespace {
 
-extern "C" {
-
-WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz,
-                               whhd_dst3d_t* out) {
-    return whhd::dst3d_create(n1, n2, n3, dx, dy, dz, out);
-}
-
-WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream) {
-    return whhd::dst3d_solve(h, phi, b, (hipStream_t)stream);
-}
-
-WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h) { whhd::dst_release(h); }
-
-WHHD_API int whhd_dst2d_create(int n1, int n2, double dx, double dy, whhd_dst3d_t* out) {
-    int st = whhd::dst3d_create(n1, n2, 1, dx, dy, 1.0, out);
-    if (st != WHHD_OK) return st;
-    (*out)->planar = true;
-    const float zero = 0.f;  // no z eigenvalue
-    hipError_t e = hipMemcpy((*out)->lam[2], &zero, sizeof zero, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        whhd::dst_release(*out);
-        *out = nullptr;
-        return whhd::hip_fail(e, "dst2d_create: eigenvalue upload");
-    }
-    return WHHD_OK;
-}
-
-WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
-                                 whhd_sweep3d_t* out) {
-    using namespace whhd;
+template <class T>
+int sweep3d_create_t(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                     whhd_sweep3d_t* out) {
     if (!out) return fail(WHHD_INVALID_ARG, "sweep3d_create: null handle pointer");
     *out = nullptr;
     if (bc != 'n' && bc != 'd' && bc != 'z')
@@ -437,12 +443,13 @@ WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, d
     auto* h = new whhd_sweep3d_s;
     h->nx = nx; h->ny = ny; h->nz = nz;
     h->bc = bc == 'n' ? 'n' : 'd';
+    h->fp64 = is64<T>();
     h->dx = dx; h->dy = dy; h->dz = dz;
     // the potential grid's interior nodes ('n'), or all of its nodes ('d')
     const int e = h->bc == 'n' ? -1 : 1;
-    int st = dst3d_create(nx + e, ny + e, nz + e, dx, dy, dz, &h->dst);
+    int st = dst3d_create_t<T>(nx + e, ny + e, nz + e, dx, dy, dz, false, &h->dst);
     if (st == WHHD_OK &&
-        hipMalloc(&h->b, (size_t)(nx + e) * (ny + e) * (nz + e) * 4) != hipSuccess)
+        hipMalloc(&h->b, (size_t)(nx + e) * (ny + e) * (nz + e) * sizeof(T)) != hipSuccess)
         st = fail(WHHD_OOM, "sweep3d_create: scratch allocation failed");
     if (st != WHHD_OK) {
         dst_release(h->dst);
@@ -453,64 +460,68 @@ WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, d
     return WHHD_OK;
 }
 
-WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
-                                const float* xv, const float* yv, const float* zv, void* stream) {
-    using namespace whhd;
+template <class T>
+int check_sweep(const whhd_sweep3d_s* h, bool planar, const char* what) {
+    if (planar != (h->nz == 0))
+        return fail(WHHD_INVALID_ARG, std::string(what) + (planar ? ": the handle is a 3D sweep"
+                                                                  : ": the handle is a 2D sweep"));
+    if (h->fp64 != is64<T>())
+        return fail(WHHD_INVALID_ARG, std::string(what) + (is64<T>() ? ": the handle was created for FP32"
+                                                                     : ": the handle was created for FP64"));
+    return WHHD_OK;
+}
+
+template <class T>
+int sweep3d_solve_t(whhd_sweep3d_t h, T* qx, T* qy, T* qz, const T* xv, const T* yv, const T* zv,
+                    hipStream_t s) {
     if (!h || !qx || !qy || !qz || !xv || !yv)
         return fail(WHHD_INVALID_ARG, "sweep3d_solve: null argument");
-    if (h->nz == 0) return fail(WHHD_INVALID_ARG, "sweep3d_solve: the handle is a 2D sweep");
+    TRY(check_sweep<T>(h, false, "sweep3d_solve"));
     const bool d = h->bc == 'd';
     if (d && !zv) return fail(WHHD_INVALID_ARG, "sweep3d_solve: velocity bc 'd' needs zv");
-    hipStream_t s = (hipStream_t)stream;
     if (d)  // SetQxQyBoundary 'd': qy's bottom plane integrates zv along x
-        hipLaunchKernelGGL(bottom_qy_kernel, dim3((h->ny + 255) / 256), dim3(256), 0, s, qy, zv,
-                           h->nx, h->ny, (float)h->dx);
-    hipLaunchKernelGGL(sweep_kernel, grid_xyz(h->nx + 1, h->ny + 1, 1), dim3(64, 4), 0, s, qx, qy,
-                       qz, xv, yv, h->nx, h->ny, h->nz, (float)h->dz, d ? 1 : 0);
+        hipLaunchKernelGGL(bottom_qy_kernel<T>, dim3((h->ny + 255) / 256), dim3(256), 0, s, qy, zv,
+                           h->nx, h->ny, (T)h->dx);
+    hipLaunchKernelGGL(sweep_kernel<T>, grid_xyz(h->nx + 1, h->ny + 1, 1), dim3(64, 4), 0, s, qx, qy,
+                       qz, xv, yv, h->nx, h->ny, h->nz, (T)h->dz, d ? 1 : 0);
     if (!d)
-        hipLaunchKernelGGL(top_gauge_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx, qy,
-                           qz, h->nx, h->ny, h->nz, (float)h->dx, (float)h->dz);
+        hipLaunchKernelGGL(top_gauge_kernel<T>, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx,
+                           qy, qz, h->nx, h->ny, h->nz, (T)h->dx, (T)h->dz);
     hipError_t e = done("sweep_kernels");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "sweep kernels");
 }
 
-WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
-                                  void* stream) {
-    using namespace whhd;
+template <class T>
+int sweep3d_project_t(whhd_sweep3d_t h, T* qx, T* qy, T* qz, hipStream_t s) {
     if (!h || !qx || !qy || !qz) return fail(WHHD_INVALID_ARG, "sweep3d_project: null argument");
-    if (h->nz == 0) return fail(WHHD_INVALID_ARG, "sweep3d_project: the handle is a 2D sweep");
-    hipStream_t s = (hipStream_t)stream;
+    TRY(check_sweep<T>(h, false, "sweep3d_project"));
     const int nx = h->nx, ny = h->ny, nz = h->nz;
-    const float ihx = (float)(1.0 / h->dx), ihy = (float)(1.0 / h->dy), ihz = (float)(1.0 / h->dz);
+    const T ihx = (T)(1.0 / h->dx), ihy = (T)(1.0 / h->dy), ihz = (T)(1.0 / h->dz);
+    T* b = (T*)h->b;
     if (h->bc == 'd') {  // ProjectD, WaveletPotentials/cuParallelSweep3D.cu:1066-1079
-        hipLaunchKernelGGL(node_div_d_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s,
-                           h->b, qx, qy, qz, nx, ny, nz, ihx, ihy, ihz);
+        hipLaunchKernelGGL(node_div_d_kernel<T>, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s,
+                           b, qx, qy, qz, nx, ny, nz, ihx, ihy, ihz);
         hipError_t e = done("node_div_d_kernel");
         if (e != hipSuccess) return hip_fail(e, "node divergence kernel");
-        int st = dst3d_solve(h->dst, h->b, h->b, s);
-        if (st != WHHD_OK) return st;
-        hipLaunchKernelGGL(grad_minus_d_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s,
-                           qx, qy, qz, h->b, nx, ny, nz, ihx, ihy, ihz);
+        TRY(dst3d_solve_t<T>(h->dst, b, b, s));
+        hipLaunchKernelGGL(grad_minus_d_kernel<T>, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0,
+                           s, qx, qy, qz, b, nx, ny, nz, ihx, ihy, ihz);
         e = done("grad_minus_d_kernel");
         return e == hipSuccess ? WHHD_OK : hip_fail(e, "gradient kernel");
     }
-    hipLaunchKernelGGL(node_div_kernel, grid_xyz(nx - 1, ny - 1, nz - 1), dim3(64, 4), 0, s, h->b,
-                       qx, qy, qz, nx, ny, nz, (float)(1.0 / h->dx), (float)(1.0 / h->dy),
-                       (float)(1.0 / h->dz));
+    hipLaunchKernelGGL(node_div_kernel<T>, grid_xyz(nx - 1, ny - 1, nz - 1), dim3(64, 4), 0, s, b,
+                       qx, qy, qz, nx, ny, nz, ihx, ihy, ihz);
     hipError_t e = done("node_div_kernel");
     if (e != hipSuccess) return hip_fail(e, "node divergence kernel");
-    int st = dst3d_solve(h->dst, h->b, h->b, s);
-    if (st != WHHD_OK) return st;
-    hipLaunchKernelGGL(grad_minus_kernel, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s, qx,
-                       qy, qz, h->b, nx, ny, nz, (float)(1.0 / h->dx), (float)(1.0 / h->dy),
-                       (float)(1.0 / h->dz));
+    TRY(dst3d_solve_t<T>(h->dst, b, b, s));
+    hipLaunchKernelGGL(grad_minus_kernel<T>, grid_xyz(nx + 1, ny + 1, nz + 1), dim3(64, 4), 0, s, qx,
+                       qy, qz, b, nx, ny, nz, ihx, ihy, ihz);
     e = done("grad_minus_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "gradient kernel");
 }
 
-WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
-                                 whhd_sweep3d_t* out) {
-    using namespace whhd;
+template <class T>
+int sweep2d_create_t(int nx, int ny, double dx, double dy, char bc, whhd_sweep3d_t* out) {
     if (!out) return fail(WHHD_INVALID_ARG, "sweep2d_create: null handle pointer");
     *out = nullptr;
     if (bc != 'n' && bc != 'd' && bc != 'z')
@@ -519,25 +530,92 @@ WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
         return fail(WHHD_INVALID_ARG, "sweep2d_create: extents out of range");
     if (!(dx > 0 && dy > 0)) return fail(WHHD_INVALID_ARG, "sweep2d_create: spacing <= 0");
     auto* h = new whhd_sweep3d_s;
-    h->nx = nx; h->ny = ny; h->nz = 0;  // nz = 0 marks a 2D handle
+    h->nx = nx; h->ny = ny; h->nz = 0;
     h->bc = bc == 'n' ? 'n' : 'd';
+    h->fp64 = is64<T>();
     h->dx = dx; h->dy = dy; h->dz = 1.0;
     *out = h;
     return WHHD_OK;
 }
 
-WHHD_API int whhd_sweep2d_solve(whhd_sweep3d_t h, float* qz, const float* xv, const float* yv,
-                                void* stream) {
-    using namespace whhd;
+template <class T>
+int sweep2d_solve_t(whhd_sweep3d_t h, T* qz, const T* xv, const T* yv, hipStream_t s) {
     if (!h || !qz || !xv || !yv) return fail(WHHD_INVALID_ARG, "sweep2d_solve: null argument");
-    if (h->nz != 0) return fail(WHHD_INVALID_ARG, "sweep2d_solve: the handle is a 3D sweep");
-    hipLaunchKernelGGL(sweep2d_kernel, dim3((h->ny + 1 + 255) / 256), dim3(256), 0,
-                       (hipStream_t)stream, qz, xv, yv, h->nx, h->ny, (float)h->dx, (float)h->dy,
-                       h->bc == 'd' ? 1 : 0);
+    TRY(check_sweep<T>(h, true, "sweep2d_solve"));
+    hipLaunchKernelGGL(sweep2d_kernel<T>, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qz, xv, yv,
+                       h->nx, h->ny, (T)h->dx, (T)h->dy, h->bc == 'd' ? 1 : 0);
     hipError_t e = done("sweep2d_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "2D sweep kernel");
 }
 
+}  // namespace
+}  // namespace whhd
+
+extern "C" {
+
+WHHD_API int whhd_dst3d_create(int n1, int n2, int n3, double dx, double dy, double dz,
+                               whhd_dst3d_t* out) {
+    return whhd::dst3d_create_t<float>(n1, n2, n3, dx, dy, dz, false, out);
+}
+WHHD_API int whhd_dst3d_create_f64(int n1, int n2, int n3, double dx, double dy, double dz,
+                                   whhd_dst3d_t* out) {
+    return whhd::dst3d_create_t<double>(n1, n2, n3, dx, dy, dz, false, out);
+}
+WHHD_API int whhd_dst2d_create(int n1, int n2, double dx, double dy, whhd_dst3d_t* out) {
+    return whhd::dst3d_create_t<float>(n1, n2, 1, dx, dy, 1.0, true, out);
+}
+WHHD_API int whhd_dst2d_create_f64(int n1, int n2, double dx, double dy, whhd_dst3d_t* out) {
+    return whhd::dst3d_create_t<double>(n1, n2, 1, dx, dy, 1.0, true, out);
+}
+WHHD_API int whhd_dst3d_solve(whhd_dst3d_t h, float* phi, const float* b, void* stream) {
+    return whhd::dst3d_solve_t<float>(h, phi, b, (hipStream_t)stream);
+}
+WHHD_API int whhd_dst3d_solve_f64(whhd_dst3d_t h, double* phi, const double* b, void* stream) {
+    return whhd::dst3d_solve_t<double>(h, phi, b, (hipStream_t)stream);
+}
+WHHD_API void whhd_dst3d_destroy(whhd_dst3d_t h) { whhd::dst_release(h); }
+
+WHHD_API int whhd_sweep3d_create(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
+                                 whhd_sweep3d_t* out) {
+    return whhd::sweep3d_create_t<float>(nx, ny, nz, dx, dy, dz, bc, out);
+}
+WHHD_API int whhd_sweep3d_create_f64(int nx, int ny, int nz, double dx, double dy, double dz,
+                                     char bc, whhd_sweep3d_t* out) {
+    return whhd::sweep3d_create_t<double>(nx, ny, nz, dx, dy, dz, bc, out);
+}
+WHHD_API int whhd_sweep3d_solve(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
+                                const float* xv, const float* yv, const float* zv, void* stream) {
+    return whhd::sweep3d_solve_t<float>(h, qx, qy, qz, xv, yv, zv, (hipStream_t)stream);
+}
+WHHD_API int whhd_sweep3d_solve_f64(whhd_sweep3d_t h, double* qx, double* qy, double* qz,
+                                    const double* xv, const double* yv, const double* zv,
+                                    void* stream) {
+    return whhd::sweep3d_solve_t<double>(h, qx, qy, qz, xv, yv, zv, (hipStream_t)stream);
+}
+WHHD_API int whhd_sweep3d_project(whhd_sweep3d_t h, float* qx, float* qy, float* qz,
+                                  void* stream) {
+    return whhd::sweep3d_project_t<float>(h, qx, qy, qz, (hipStream_t)stream);
+}
+WHHD_API int whhd_sweep3d_project_f64(whhd_sweep3d_t h, double* qx, double* qy, double* qz,
+                                      void* stream) {
+    return whhd::sweep3d_project_t<double>(h, qx, qy, qz, (hipStream_t)stream);
+}
+WHHD_API int whhd_sweep2d_create(int nx, int ny, double dx, double dy, char bc,
+                                 whhd_sweep3d_t* out) {
+    return whhd::sweep2d_create_t<float>(nx, ny, dx, dy, bc, out);
+}
+WHHD_API int whhd_sweep2d_create_f64(int nx, int ny, double dx, double dy, char bc,
+                                     whhd_sweep3d_t* out) {
+    return whhd::sweep2d_create_t<double>(nx, ny, dx, dy, bc, out);
+}
+WHHD_API int whhd_sweep2d_solve(whhd_sweep3d_t h, float* qz, const float* xv, const float* yv,
+                                void* stream) {
+    return whhd::sweep2d_solve_t<float>(h, qz, xv, yv, (hipStream_t)stream);
+}
+WHHD_API int whhd_sweep2d_solve_f64(whhd_sweep3d_t h, double* qz, const double* xv,
+                                    const double* yv, void* stream) {
+    return whhd::sweep2d_solve_t<double>(h, qz, xv, yv, (hipStream_t)stream);
+}
 WHHD_API void whhd_sweep3d_destroy(whhd_sweep3d_t h) {
     if (!h) return;
     whhd::dst_release(h->dst);
