@@ -80,13 +80,14 @@ struct Proj3Args {
     float lenx, leny, lenz;       // domain length per axis
 };
 
-// One thread per (x, y) column walking ZB consecutive z cells; offsets in
-// index type I (32-bit whenever every array fits).  Grid row blockIdx.z
-// covers z block (blockIdx.z % zblocks) of problem (blockIdx.z / zblocks).
+// One thread per (x, y) column walking ZB consecutive z cells, y fastest
+// within the workgroup; offsets in index type I (32-bit whenever every array
+// fits).  Grid row blockIdx.z covers z block (blockIdx.z % zblocks) of
+// problem (blockIdx.z / zblocks).
 template <class I, int ZB>
 __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
-    const int y = blockIdx.x * 64 + threadIdx.x;
-    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y * blockDim.y + threadIdx.y;
     const int pb = blockIdx.z / p.zblocks;
     const int z0 = (blockIdx.z - pb * p.zblocks) * ZB;
     if (y >= p.ny || x >= p.nx) return;
@@ -485,8 +486,12 @@ int project3d_create(int nx, int ny, int nz, double dx, double dy, double dz, in
 
 template <int ZB>
 static void launch_project3d(const Proj3Args& a, int nb, bool wide, hipStream_t st) {
-    dim3 block(64, 4, 1);
-    dim3 grid((a.ny + 63) / 64, (a.nx + 3) / 4, (unsigned)(a.zblocks * nb));
+    // a workgroup spans up to 256 consecutive y of 256 / that many rows, so the
+    // potentials' rows (ny + 1 floats, not 128-byte aligned) are written by one
+    // workgroup each and few cache lines are shared between workgroups
+    const int yb = a.ny >= 256 ? 256 : (a.ny >= 128 ? 128 : 64);
+    dim3 block(yb, 256 / yb, 1);
+    dim3 grid((a.ny + yb - 1) / yb, (a.nx + block.y - 1) / block.y, (unsigned)(a.zblocks * nb));
     if (!wide) hipLaunchKernelGGL((project3d_kernel<int, ZB>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((project3d_kernel<int64_t, ZB>), grid, block, 0, st, a);
 }
