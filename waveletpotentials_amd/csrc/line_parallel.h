@@ -324,6 +324,10 @@ WHHD_DEV void fwd_level(const IO& io, float* in, float* out) {
     constexpr int NP = kBlock + 2 * H;
     constexpr int NB = (Wd + kBlock - 1) / kBlock;
     [[maybe_unused]] typename IO::Pend pend;
+    // contiguous lines store the details of two blocks at once (128 bytes from
+    // a 128-byte boundary when the line is aligned), not 64-byte halves written
+    // a block apart
+    [[maybe_unused]] float held[kBlock];
     if constexpr (TOP && IO::kTwoPhase) io.template fetch<0, fwd_front<H, N, 0>(0)>(pend);
     static_for<0, NB>([&](auto bi) {
         constexpr int b = decltype(bi)::value;
@@ -366,7 +370,25 @@ WHHD_DEV void fwd_level(const IO& io, float* in, float* out) {
                 if constexpr (k < Wd - EXT) run[k - b0] = D[j];
             }
         });
-        if constexpr (nd > 0) io.template store_run<Wd + b0, nd>(run);
+        // details of the next block (a block is held only when that one stores)
+        constexpr int nd_next = (Wd - EXT - b0 - kBlock) < kBlock ? (Wd - EXT - b0 - kBlock) : kBlock;
+        if constexpr (IO::kTwoPhase && nd > 0) {
+            if constexpr (b % 2 == 0 && nd == kBlock && b + 1 < NB && nd_next > 0) {
+#pragma unroll
+                for (int v = 0; v < kBlock; ++v) held[v] = run[v];
+            } else if constexpr (b % 2 == 1) {
+                float pair[2 * kBlock];
+#pragma unroll
+                for (int v = 0; v < kBlock; ++v) pair[v] = held[v];
+#pragma unroll
+                for (int v = 0; v < nd; ++v) pair[kBlock + v] = run[v];
+                io.template store_run<Wd + b0 - kBlock, kBlock + nd>(pair);
+            } else {
+                io.template store_run<Wd + b0, nd>(run);
+            }
+        } else if constexpr (nd > 0) {
+            io.template store_run<Wd + b0, nd>(run);
+        }
     });
 }
 
