@@ -598,8 +598,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
     src += (int64_t)blockIdx.y * g.batch_stride;
     dst += (int64_t)blockIdx.y * g.batch_stride;
+    // Workgroups go to the 8 XCDs round-robin (blockIdx.x % 8), each XCD with
+    // its own L2; give every XCD a contiguous run of tiles, so that the cache
+    // lines two neighbouring tiles share (rows of 2^L + 1 floats start
+    // unaligned) are fetched and written through one L2.
+    const unsigned nwg = gridDim.x, bxr = blockIdx.x % 8u, bxq = blockIdx.x / 8u;
+    const unsigned tile = bxr * (nwg / 8u) + (bxr < nwg % 8u ? bxr : nwg % 8u) + bxq;
     const int lane_ = threadIdx.x & 63;
-    const int64_t wave_first = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    const int64_t wave_first = ((int64_t)tile * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
     const int64_t li = wave_first + lane_;
     const bool valid = li < g.total_lines;  // lanes past the last line load line 0, never store
     const int64_t l = valid ? li : 0;
