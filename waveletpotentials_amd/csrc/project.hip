@@ -82,14 +82,22 @@ struct Proj3Args {
 
 // One thread per (x, y) column walking ZB consecutive z cells, y fastest
 // within the workgroup; offsets in index type I (32-bit whenever every array
-// fits).  Grid row blockIdx.z covers z block (blockIdx.z % zblocks) of
-// problem (blockIdx.z / zblocks).
+// fits).  Tile row tz covers z block (tz % zblocks) of problem
+// (tz / zblocks).
 template <class I, int ZB>
 __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    const int x = blockIdx.y * blockDim.y + threadIdx.y;
-    const int pb = blockIdx.z / p.zblocks;
-    const int z0 = (blockIdx.z - pb * p.zblocks) * ZB;
+    // workgroups go to the 8 XCDs round-robin in launch order; each XCD takes a
+    // contiguous run of tiles, so rows that share a cache line share an L2
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned nwg = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned r8 = lin % 8u;
+    const unsigned tile = r8 * (nwg / 8u) + (r8 < nwg % 8u ? r8 : nwg % 8u) + lin / 8u;
+    const unsigned tx = tile % gx, ty = (tile / gx) % gy, tz = tile / (gx * gy);
+    const int y = tx * blockDim.x + threadIdx.x;
+    const int x = ty * blockDim.y + threadIdx.y;
+    const int pb = tz / p.zblocks;
+    const int z0 = (tz - pb * p.zblocks) * ZB;
     if (y >= p.ny || x >= p.nx) return;
     if (pb) {
         p.xv += pb * p.bv[0]; p.yv += pb * p.bv[1]; p.zv += pb * p.bv[2];
