@@ -35,9 +35,9 @@ struct whhd_recover3d_s {
     char bc, bd;
     int wc, wd;
     int nb = 1;        // problems per solve, arrays back to back
-    int lp_waves = 2;  // wavefronts per workgroup of the line passes (2 measured
-                       // fastest at 256^3: 0.757 ms against 0.761 with 1 and 0.779 with 4)
-    int proj_zb = 0;   // z cells per projection thread (0: default)
+    int lp_waves = 1;  // wavefronts per workgroup of the line passes and z cells
+    int proj_zb = 8;   // per projection thread: fastest at 256^3 (0.713 ms against
+                       // 0.727 with 2 and 4, 0.753 with 4 and 8)
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
     float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
