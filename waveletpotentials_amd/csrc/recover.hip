@@ -38,6 +38,7 @@ struct whhd_recover3d_s {
     int lp_waves = 1;  // wavefronts per workgroup of the line passes and z cells
     int proj_zb = 8;   // per projection thread: fastest at 256^3 (0.713 ms against
                        // 0.727 with 2 and 4, 0.753 with 4 and 8)
+    int lp_waves_y = 2;  // the y (contiguous) passes: 0.708 ms against 0.710 with 1
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
     float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
@@ -104,9 +105,10 @@ static int three_pass(const float* in, float* scratch, float* out, int nx, int n
                       const whhd_recover3d_s* h, const char* arr) {
     StageTrace* tr = const_cast<StageTrace*>(&h->trace);
     const LineBatch b{h->nb, (int64_t)nx * ny * nz, h->lp_waves};
+    const LineBatch by{h->nb, (int64_t)nx * ny * nz, h->lp_waves_y};
     TRY(transform_lines<float>(out, in, nx, ny, nz, lv[0], 'x', px.bc, px.wt, fwd, s, nullptr, &b));
     mark(tr, s, fwd, arr, 'x');
-    TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s, nullptr, &b));
+    TRY(transform_lines<float>(scratch, out, nx, ny, nz, lv[1], 'y', py.bc, py.wt, fwd, s, nullptr, &by));
     mark(tr, s, fwd, arr, 'y');
     TRY(transform_lines<float>(out, scratch, nx, ny, nz, lv[2], 'z', pz.bc, pz.wt, fwd, s, nullptr, &b));
     mark(tr, s, fwd, arr, 'z');
@@ -221,6 +223,7 @@ int recover3d_set_tiles(whhd_recover3d_s* h, int lp_waves, int proj_zb) {
     if (proj_zb != 1 && proj_zb != 2 && proj_zb != 4 && proj_zb != 8)
         return fail(WHHD_INVALID_ARG, "projection z block must be 1, 2, 4 or 8");
     h->lp_waves = lp_waves;
+    h->lp_waves_y = lp_waves;
     h->proj_zb = proj_zb;
     return WHHD_OK;
 }
