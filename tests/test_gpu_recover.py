@@ -287,7 +287,7 @@ def test_stage_trace(whhd, kernel_mode):
     stages, solves = rec.TraceEnd()
     assert solves == 3
     assert [s["stage"] for s in stages] == (
-        [f"fwd {a} {x}" for a in ("xv", "yv", "zv") for x in "xyz"] + ["project"]
+        [f"fwd {a} {x}" for a in ("xv", "yv", "zv") for x in "yxz"] + ["project"]
         + [f"inv {a} {x}" for a in ("qx", "qy", "qz") for x in "xyz"])
     assert stages[9]["kernel"] == "project3d_kernel"
     want = "lp_kernel" if kernel_mode == 0 else "line_tile_kernel"
