@@ -288,7 +288,7 @@ def test_stage_trace(whhd, kernel_mode):
     assert solves == 3
     assert [s["stage"] for s in stages] == (
         [f"fwd {a} {x}" for a in ("xv", "yv", "zv") for x in "yxz"] + ["project"]
-        + [f"inv {a} {x}" for a in ("qx", "qy", "qz") for x in "xyz"])
+        + [f"inv {a} {x}" for a in ("qx", "qy", "qz") for x in "zyx"])
     assert stages[9]["kernel"] == "project3d_kernel"
     want = "lp_kernel" if kernel_mode == 0 else "line_tile_kernel"
     assert all(s["kernel"].startswith(want) for i, s in enumerate(stages) if i != 9)
