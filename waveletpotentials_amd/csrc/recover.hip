@@ -38,7 +38,7 @@ struct whhd_recover3d_s {
     int lp_waves = 1;  // wavefronts per workgroup of the line passes and z cells
     int proj_zb = 8;   // per projection thread: fastest at 256^3 (0.713 ms against
                        // 0.727 with 2 and 4, 0.753 with 4 and 8)
-    int lp_waves_y = 2;  // the y (contiguous) passes: 0.708 ms against 0.710 with 1
+    int lp_waves_y = 1;  // the y (contiguous) passes: 0.691 ms against 0.695 with 2
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
     float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
