@@ -124,6 +124,12 @@ typedef struct whhd_recover3d_s* whhd_recover3d_t;
 WHHD_API int whhd_recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz,
                                    char bc, int wavelet_curl, int wavelet_div,
                                    whhd_recover3d_t* out);
+/* Device scratch, in bytes, that whhd_recover3d_create allocates for these
+ * cell counts (the reference draws its scratch "tmp", "tmp1", "tmp2" from the
+ * CuMemoryManager singleton, WaveletPotentials/cuWaveletPotentialRecover3D.cu:
+ * 18-22): per component one velocity-sized array and one array of the larger
+ * of the velocity and potential sizes.  Host only; needs no device. */
+WHHD_API int whhd_recover3d_scratch_bytes(int nx, int ny, int nz, unsigned long long* bytes);
 /* Solve(qx, qy, qz, xv, yv, zv) (cuWaveletPotentialRecover3D.cu:16-61). */
 WHHD_API int whhd_recover3d_solve(whhd_recover3d_t h, float* qx, float* qy, float* qz,
                                   const float* xv, const float* yv, const float* zv,
@@ -204,11 +210,21 @@ WHHD_API void whhd_recover2d_destroy(whhd_recover2d_t h);
 WHHD_API int whhd_levelset_spheres(float* ls, int nx, int ny, int nz, double dx, double dy,
                                    double dz, const double* centers, const double* radii,
                                    int count, void* stream);
-/* CuSolidLevelSet3D::GetFrac (WaveletPotentials/cuSolidLevelSet3D.cu:348-353 with
+/* CuSolidLevelSet3D::GetFrac(ax, ay, az) (WaveletPotentials/cuSolidLevelSet3D.cu:348-353 with
  * cuComputeFrac :28-34 and cuGetAx/Ay/Az :36-90): wet face fractions
  * ax (nx+1, ny, nz), ay (nx, ny+1, nz), az (nx, ny, nz+1) of the cell level set ls. */
 WHHD_API int whhd_frac_from_ls(float* ax, float* ay, float* az, const float* ls, int nx, int ny,
                                int nz, double dx, double dy, double dz, void* stream);
+/* The static CuSolidLevelSet3D::GetFrac(ax, ay, az, ls, dim, dx)
+ * (WaveletPotentials/cuSolidLevelSet3D.cu:359-372 with cuGetAx/Ay/Az_divide
+ * :164-317 and the samplers of Interpolation.cuh:16-29, 80-157): a face is 1
+ * where the quadratic B-spline interpolant of ls at its centre exceeds the
+ * spacing along its normal, 0 where it is <= minus that spacing, and otherwise
+ * the share of its 128 x 128 midpoint sub-samples where the trilinear
+ * interpolant of ls is positive (a multiple of 1/16384).  Extents as
+ * whhd_frac_from_ls; needs at least 2 cells per axis. */
+WHHD_API int whhd_frac_divide(float* ax, float* ay, float* az, const float* ls, int nx, int ny,
+                              int nz, double dx, double dy, double dz, void* stream);
 /* CuFastSweeping::Solve(ls, dim, dx, n_iters) (WaveletPotentials/cuFastSweeping.cu:
  * 215-257): n_iters relaxations of the eikonal update on cells with ls > -1e-12
  * (solid cells stay fixed).  Each relaxation reads the previous iterate

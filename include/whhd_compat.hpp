@@ -26,8 +26,8 @@
 //   - Failures throw whhd_compat::Error (the reference aborts the process).
 //   - Work goes to the stream set with SetStream (default: the null stream);
 //     the reference always uses the null stream.
-//   - CuGetInnerProduct::GetIp returns a device array of 8*n doubles owned by
-//     the instance (the reference returns DTYPE* cached by string key).
+//   - CuGetInnerProduct's tables are computed in FP64 and stored as DTYPE
+//     (the reference accumulates them in float).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,6 +36,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "whhd.h"
 
@@ -71,6 +72,35 @@ inline void check(int status, const char* where) {
 inline char proj_bc(char bc) { return whhd_proj_bc(bc); }
 inline WaveletType proj_WaveletType(WaveletType t) {
     return static_cast<WaveletType>(whhd_proj_wavelet(t));
+}
+
+// Wavelet names of the Gram-table keys (WaveletPotentials/WaveletTypes.h:67-100,
+// WaveletTypes.cpp:4-67): "<bc>_cdf<p>.<q>"; an unknown name reads as CDF3.5,
+// as in the reference.
+inline const std::pair<WaveletType, const char*>* wavelet_names(int* count) {
+    static const std::pair<WaveletType, const char*> t[] = {
+        {WT_CDF3_5, "cdf3.5"},   {WT_CDF2_6, "cdf2.6"},   {WT_CDF4_4, "cdf4.4"},
+        {WT_CDF4_6, "cdf4.6"},   {WT_CDF3_7, "cdf3.7"},   {WT_CDF2_8, "cdf2.8"},
+        {WT_CDF3_9, "cdf3.9"},   {WT_CDF3_11, "cdf3.11"}, {WT_CDF2_10, "cdf2.10"},
+        {WT_CDF2_12, "cdf2.12"}, {WT_CDF4_8, "cdf4.8"},   {WT_CDF4_10, "cdf4.10"},
+        {WT_CDF2_0, "cdf2.0"},   {WT_CDF1_1, "cdf1.1"},   {WT_CDF1_7, "cdf1.7"}};
+    *count = (int)(sizeof(t) / sizeof(t[0]));
+    return t;
+}
+inline WaveletType type_from_string(const std::string& str) {
+    int n = 0;
+    const auto* t = wavelet_names(&n);
+    const std::string tail = str.size() > 2 ? str.substr(2) : std::string();
+    for (int i = 0; i < n; ++i)
+        if (tail == t[i].second) return t[i].first;
+    return WT_CDF3_5;
+}
+inline std::string string_from_type(WaveletType a, char bc) {
+    int n = 0;
+    const auto* t = wavelet_names(&n);
+    for (int i = 0; i < n; ++i)
+        if (t[i].first == a) return std::string(1, bc) + "_" + t[i].second;
+    throw Error(WHHD_UNSUPPORTED_WAVELET, "string_from_type");
 }
 
 // Handles are move-only owners of a C-ABI handle.
@@ -184,7 +214,12 @@ private:
     hipStream_t stream_ = nullptr;
 };
 
-// Gram tables, cached per (type, bc, n) (WaveletPotentials/cuGetInnerProduct.cu:92-126).
+// Gram tables, cached per key string + n (WaveletPotentials/cuGetInnerProduct.cu:
+// 85-126, cuGetInnerProduct.cuh:15-19).  Keys are "<bc>_cdf<p>.<q>" as built by
+// string_from_type.  GenerateIp returns the cached table when the key exists
+// and builds it otherwise; GetIp returns the cached table or nullptr.  A table
+// is 8 rows of n DTYPE (device memory owned by the instance): the diagonal,
+// the -2, -1, +1, +2 neighbours and the three parent entries.
 class CuGetInnerProduct {
 public:
     static CuGetInnerProduct* GetInstance() {
@@ -194,30 +229,45 @@ public:
     ~CuGetInnerProduct() {
         for (auto& kv : tables_) (void)hipFree(kv.second);
     }
-    const double* GetIp(WaveletType type, char bc, int n) {
-        const auto key = std::make_pair(std::string(1, bc) + std::to_string((int)type), n);
+    DTYPE* GetIp(std::string str, int dim) {
+        auto it = tables_.find(str + std::to_string(dim));
+        return it == tables_.end() ? nullptr : it->second;
+    }
+    DTYPE* GenerateIp(std::string str, int dim) {
+        const std::string key = str + std::to_string(dim);
         auto it = tables_.find(key);
         if (it != tables_.end()) return it->second;
-        return GenerateIp(type, bc, n);
-    }
-    const double* GenerateIp(WaveletType type, char bc, int n) {
-        double* d = nullptr;
-        if (hipMalloc(&d, sizeof(double) * 8 * (size_t)n) != hipSuccess)
-            throw Error(WHHD_OOM, "GenerateIp");
-        const int st = whhd_gram_table(d, type, bc, n, nullptr);
-        if (st != WHHD_OK) {
+        if (str.size() < 3) throw Error(WHHD_INVALID_ARG, "GenerateIp: key is not <bc>_cdf<p>.<q>");
+        const WaveletType type = type_from_string(str);
+        const size_t n = 8 * (size_t)dim;
+        double* d64 = nullptr;
+        if (hipMalloc(&d64, sizeof(double) * n) != hipSuccess) throw Error(WHHD_OOM, "GenerateIp");
+        int st = whhd_gram_table(d64, type, str[0], dim, nullptr);
+        std::vector<double> h64(n);
+        if (st == WHHD_OK &&
+            hipMemcpy(h64.data(), d64, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            st = WHHD_HIP_ERROR;
+        (void)hipFree(d64);
+        if (st != WHHD_OK) throw Error(st, "GenerateIp");
+        std::vector<DTYPE> h(h64.begin(), h64.end());
+        DTYPE* d = nullptr;
+        if (hipMalloc(&d, sizeof(DTYPE) * n) != hipSuccess) throw Error(WHHD_OOM, "GenerateIp");
+        if (hipMemcpy(d, h.data(), sizeof(DTYPE) * n, hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(d);
-            throw Error(st, "GenerateIp");
+            throw Error(WHHD_HIP_ERROR, "GenerateIp");
         }
-        const auto key = std::make_pair(std::string(1, bc) + std::to_string((int)type), n);
-        auto it = tables_.find(key);
-        if (it != tables_.end()) (void)hipFree(it->second);
         tables_[key] = d;
         return d;
     }
+    DTYPE* GetIp(WaveletType type, char bc, int dim) {
+        return GetIp(string_from_type(type, bc), dim);
+    }
+    DTYPE* GenerateIp(WaveletType type, char bc, int dim) {
+        return GenerateIp(string_from_type(type, bc), dim);
+    }
 
 private:
-    std::map<std::pair<std::string, int>, double*> tables_;
+    std::map<std::string, DTYPE*> tables_;
 };
 
 // ---- solid boundary, fractional multigrid and field operators -------------
@@ -234,14 +284,22 @@ public:
     CuSolidLevelSet3D& operator=(const CuSolidLevelSet3D&) = delete;
     ~CuSolidLevelSet3D() { (void)hipFree(ls_); }
     void InitLsSphere(DTYPE3 center, DTYPE radius) { InitLsSphere(ls_, center, radius, dim_, dx_); }
-    void GetFrac(DTYPE* ax, DTYPE* ay, DTYPE* az) { GetFrac(ax, ay, az, ls_, dim_, dx_); }
+    // the member overload: the two-point formula (cuGetAx/Ay/Az,
+    // WaveletPotentials/cuSolidLevelSet3D.cu:36-90, 348-353)
+    void GetFrac(DTYPE* ax, DTYPE* ay, DTYPE* az) {
+        check(whhd_frac_from_ls(ax, ay, az, ls_, dim_.x, dim_.y, dim_.z, dx_.x, dx_.y, dx_.z,
+                                nullptr),
+              "GetFrac");
+    }
     static void InitLsSphere(DTYPE* ls, DTYPE3 center, DTYPE radius, int3 dim, DTYPE3 dx) {
         const double c[3] = {center.x, center.y, center.z}, r = radius;
         check(whhd_levelset_spheres(ls, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, c, &r, 1, nullptr),
               "InitLsSphere");
     }
+    // the static overload: supersampled fractions (cuGetAx/Ay/Az_divide,
+    // WaveletPotentials/cuSolidLevelSet3D.cu:164-317, 359-372)
     static void GetFrac(DTYPE* ax, DTYPE* ay, DTYPE* az, DTYPE* ls, int3 dim, DTYPE3 dx) {
-        check(whhd_frac_from_ls(ax, ay, az, ls, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, nullptr),
+        check(whhd_frac_divide(ax, ay, az, ls, dim.x, dim.y, dim.z, dx.x, dx.y, dx.z, nullptr),
               "GetFrac");
     }
     DTYPE* ls_ = nullptr;

@@ -57,6 +57,11 @@ def _setup(L):
     L.orc_div3d.argtypes = [_D] * 4 + [i, i, i, d, d, d]
     L.orc_levelset_spheres.argtypes = [_D, i, i, i, d, d, d, _D, _D, i]
     L.orc_frac_from_ls.argtypes = [_D] * 4 + [i, i, i, d, d, d]
+    L.orc_frac_divide.argtypes = [_D] * 4 + [i, i, i, d, d, d]
+    L.orc_interp_linear3d.argtypes = [_D, d, d, d, i, i, i]
+    L.orc_interp_linear3d.restype = d
+    L.orc_interp_quadratic3d.argtypes = [_D, d, d, d, i, i, i]
+    L.orc_interp_quadratic3d.restype = d
     L.orc_eikonal_update.argtypes = [d] * 6
     L.orc_eikonal_update.restype = d
     L.orc_eikonal_sweep.argtypes = [_D, i, i, i, d, i]
@@ -253,6 +258,31 @@ def frac_from_ls(ls, dx):
     _check(lib().orc_frac_from_ls(_p(ax), _p(ay), _p(az), _p(ls), nx, ny, nz, *map(float, dx)),
            "frac_from_ls")
     return ax, ay, az
+
+
+def frac_divide(ls, dx):
+    """Supersampled face fractions (the reference's static GetFrac) of a cell
+    level set of shape (nz, nx, ny)."""
+    ls = np.ascontiguousarray(ls, dtype=np.float64)
+    nz, nx, ny = ls.shape
+    ax, ay, az = field(nx + 1, ny, nz), field(nx, ny + 1, nz), field(nx, ny, nz + 1)
+    _check(lib().orc_frac_divide(_p(ax), _p(ay), _p(az), _p(ls), nx, ny, nz, *map(float, dx)),
+           "frac_divide")
+    return ax, ay, az
+
+
+def interp_linear3d(v, g):
+    """InterpolateLinear3D of v (nz, nx, ny) at grid coordinates g = (x, y, z)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    nz, nx, ny = v.shape
+    return lib().orc_interp_linear3d(_p(v), *map(float, g), nx, ny, nz)
+
+
+def interp_quadratic3d(v, g):
+    """InterpolateQuadratic3D of v (nz, nx, ny) at grid coordinates g = (x, y, z)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    nz, nx, ny = v.shape
+    return lib().orc_interp_quadratic3d(_p(v), *map(float, g), nx, ny, nz)
 
 
 def eikonal_update(mx, my, mz, dx, dy, dz):

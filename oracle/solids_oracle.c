@@ -4,6 +4,12 @@
  * FP64 restatement of the reference's solid-geometry kernels:
  *   sphere level set   WaveletPotentials/cuSolidLevelSet3D.cu:11-26
  *   face fractions     WaveletPotentials/cuSolidLevelSet3D.cu:28-90 (cuComputeFrac, cuGetAx/Ay/Az)
+ *   supersampled face fractions
+ *                      WaveletPotentials/cuSolidLevelSet3D.cu:164-317, 359-372
+ *                      (cuGetAx/Ay/Az_divide, the static GetFrac), with the
+ *                      samplers of WaveletPotentials/Interpolation.cuh:16-29,
+ *                      80-157 (cuGetIxFx, InterpolateLinear3D,
+ *                      InterpolateQuadratic3D)
  *   eikonal relaxation WaveletPotentials/cuFastSweeping.cu:105-146, 215-257
  * The reference's sweep updates its array in place from many threads at
  * once; this restatement relaxes all cells from the previous iterate (Jacobi),
@@ -128,5 +134,127 @@ int orc_eikonal_sweep(double* ls, int nx, int ny, int nz, double h, int n_iters)
                 }
     }
     free(prev);
+    return ORC_OK;
+}
+
+/* ---- supersampled face fractions ------------------------------------------
+ * Grid coordinates are cell-centred: coordinate g of a point at world position
+ * p along an axis of spacing h is p / h - 1/2 (Interpolation callers,
+ * cuSolidLevelSet3D.cu:177, 189 with c_min = 0, dx_c = dx).  CLAMP_L is the
+ * reference's FP32-build value 1e-5 (Interpolation.cuh:11-13); the library
+ * computes in FP32. */
+#define ORC_CLAMP_L 1e-5
+
+/* cuGetIxFx, Interpolation.cuh:16-29 */
+static void ixfx(int* ix, double* fx, double x, int n) {
+    if (x < ORC_CLAMP_L) x = ORC_CLAMP_L;
+    *ix = (int)x;
+    if (*ix >= n - 1) {
+        *ix = n - 2;
+        *fx = 1.0 - ORC_CLAMP_L;
+    } else {
+        *fx = x - *ix;
+    }
+}
+
+/* InterpolateLinear3D, Interpolation.cuh:80-117 (same summation order) */
+double orc_interp_linear3d(const double* v, double x, double y, double z, int nx, int ny, int nz) {
+    int ix, iy, iz;
+    double fx, fy, fz;
+    ixfx(&ix, &fx, x, nx);
+    ixfx(&iy, &fy, y, ny);
+    ixfx(&iz, &fz, z, nz);
+    const double wx[2] = {1 - fx, fx}, wy[2] = {1 - fy, fy}, wz[2] = {1 - fz, fz};
+    const size_t slice = (size_t)nx * ny;
+    size_t i = (size_t)iy + (size_t)ix * ny + (size_t)iz * slice;
+    double r = v[i] * wx[0] * wy[0] * wz[0];
+    r += v[i + 1] * wx[0] * wy[1] * wz[0];
+    r += v[i + ny] * wx[1] * wy[0] * wz[0];
+    r += v[i + ny + 1] * wx[1] * wy[1] * wz[0];
+    i += slice;
+    r += v[i] * wx[0] * wy[0] * wz[1];
+    r += v[i + 1] * wx[0] * wy[1] * wz[1];
+    r += v[i + ny] * wx[1] * wy[0] * wz[1];
+    r += v[i + ny + 1] * wx[1] * wy[1] * wz[1];
+    return r;
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* InterpolateQuadratic3D, Interpolation.cuh:119-157: quadratic B-spline
+ * weights on the three nodes from floor(g - 1/2), indices clamped to the grid */
+double orc_interp_quadratic3d(const double* v, double x, double y, double z, int nx, int ny,
+                              int nz) {
+    const int ix = (int)floor(x - 0.5), iy = (int)floor(y - 0.5), iz = (int)floor(z - 0.5);
+    const double fx = x - ix, fy = y - iy, fz = z - iz;
+#define SQ(a) ((a) * (a))
+    const double wx[3] = {0.5 * SQ(1.5 - fx), 0.75 - SQ(fx - 1.0), 0.5 * SQ(fx - 0.5)};
+    const double wy[3] = {0.5 * SQ(1.5 - fy), 0.75 - SQ(fy - 1.0), 0.5 * SQ(fy - 0.5)};
+    const double wz[3] = {0.5 * SQ(1.5 - fz), 0.75 - SQ(fz - 1.0), 0.5 * SQ(fz - 0.5)};
+#undef SQ
+    double r = 0.0;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                const int X = clampi(ix + j, 0, nx - 1), Y = clampi(iy + i, 0, ny - 1),
+                          Z = clampi(iz + k, 0, nz - 1);
+                r += wx[j] * wy[i] * wz[k] * v[IDX(Y, X, Z, nx, ny)];
+            }
+    return r;
+}
+
+/* One face of cuGetA{x,y,z}_divide, cuSolidLevelSet3D.cu:164-317.  The face is
+ * normal to `axis`; g0[] holds the grid coordinates of its lower corner in the
+ * two in-plane axes and of its plane along `axis`.  Quadratic band test at
+ * the face centre against +-h (h: the normal axis's spacing), then the share
+ * of 128 x 128 midpoint sub-samples where the linear interpolant is > 0. */
+#define ORC_SUB 128
+static double divide_face(const double* ls, int nx, int ny, int nz, int axis, const double g0[3],
+                          double h) {
+    double c[3] = {g0[0], g0[1], g0[2]};
+    for (int a = 0; a < 3; ++a)
+        if (a != axis) c[a] += 0.5;
+    const double lc = orc_interp_quadratic3d(ls, c[0], c[1], c[2], nx, ny, nz);
+    if (lc > h) return 1.0;
+    if (!(lc > -h)) return 0.0;
+    const int p = axis == 0 ? 1 : 0, q = axis == 2 ? 1 : 2; /* in-plane axes */
+    long pos = 0;
+    for (int k = 0; k < ORC_SUB; ++k)
+        for (int i = 0; i < ORC_SUB; ++i) {
+            double g[3] = {g0[0], g0[1], g0[2]};
+            g[p] += (i + 0.5) / ORC_SUB;
+            g[q] += (k + 0.5) / ORC_SUB;
+            if (orc_interp_linear3d(ls, g[0], g[1], g[2], nx, ny, nz) > 0.0) ++pos;
+        }
+    double r = (double)pos / (ORC_SUB * ORC_SUB);
+    return r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
+}
+
+int orc_frac_divide(double* ax, double* ay, double* az, const double* ls, int nx, int ny, int nz,
+                    double dx, double dy, double dz) {
+    if (!ax || !ay || !az || !ls || nx < 2 || ny < 2 || nz < 2) return ORC_INVALID;
+    /* face (x, y, z) of ax lies in the plane at world x = x*dx; its lower
+     * corner has grid coordinates (x - 1/2, y - 1/2, z - 1/2) */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x <= nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                const double g0[3] = {x - 0.5, y - 0.5, z - 0.5};
+                ax[IDX(y, x, z, nx + 1, ny)] = divide_face(ls, nx, ny, nz, 0, g0, dx);
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y <= ny; ++y) {
+                const double g0[3] = {x - 0.5, y - 0.5, z - 0.5};
+                ay[IDX(y, x, z, nx, ny + 1)] = divide_face(ls, nx, ny, nz, 1, g0, dy);
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z <= nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                const double g0[3] = {x - 0.5, y - 0.5, z - 0.5};
+                az[IDX(y, x, z, nx, ny)] = divide_face(ls, nx, ny, nz, 2, g0, dz);
+            }
     return ORC_OK;
 }

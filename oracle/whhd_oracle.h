@@ -104,6 +104,17 @@ int orc_frac_from_ls(double* ax, double* ay, double* az, const double* ls, int n
 double orc_eikonal_update(double mx, double my, double mz, double dx, double dy, double dz);
 /* n_iters Jacobi relaxations of fast_sweep_kernel_pos (in place on ls). */
 int orc_eikonal_sweep(double* ls, int nx, int ny, int nz, double h, int n_iters);
+/* cuGetAx/Ay/Az_divide (the static GetFrac): quadratic band test, then the
+ * share of 128 x 128 sub-face samples of the linearly interpolated level set
+ * that are > 0.  Extents as orc_frac_from_ls; needs nx, ny, nz >= 2. */
+int orc_frac_divide(double* ax, double* ay, double* az, const double* ls, int nx, int ny, int nz,
+                    double dx, double dy, double dz);
+/* InterpolateLinear3D / InterpolateQuadratic3D at cell-centred grid
+ * coordinates (x, y, z) of a (nx, ny, nz) y-fastest array. */
+double orc_interp_linear3d(const double* v, double x, double y, double z, int nx, int ny, int nz);
+double orc_interp_quadratic3d(const double* v, double x, double y, double z, int nx, int ny,
+                              int nz);
+
 /* ---- fractional multigrid (mg_oracle.c) ---- */
 /* Level extents of CuMultigridFrac3D's hierarchy (3 ints per level); returns the count. */
 int orc_mgfrac3d_levels(int nx, int ny, int nz, int* dims_out);

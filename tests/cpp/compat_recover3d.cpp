@@ -4,7 +4,13 @@
 //
 //   compat_recover3d N DIR   reads DIR/{xv,yv,zv}.f32 (raw FP32, y fastest),
 //                            writes DIR/{qx,qy,qz}.f32 and DIR/solve1d.f32
-//                            (forward CDF3.7 'n' transform of xv along z)
+//                            (forward CDF3.7 'n' transform of xv along z),
+//                            and the Gram tables DIR/gram_d.f32 ("d_cdf4.6",
+//                            N+1) and DIR/gram_n.f32 ("n_cdf3.7", N) after
+//                            checking the cache semantics of
+//                            CuGetInnerProduct (GetIp is nullptr before
+//                            GenerateIp; a second GenerateIp of a key returns
+//                            the first pointer)
 //   compat_recover3d solids N DIR cx cy cz r
 //                            the solid-obstacle steps of the reference's test
 //                            driver on DIR/{xv,yv,zv}.f32 (sphere at c, radius
@@ -12,7 +18,9 @@
 //                            V-cycles, gradient correction; writes DIR/ls.f32,
 //                            DIR/f.f32, DIR/p.f32 and prints the residual and
 //                            max |f|; then the sweep + DST comparator on the
-//                            input velocity, writing DIR/{sx,sy,sz}.f32
+//                            input velocity, writing DIR/{sx,sy,sz}.f32; and
+//                            the static GetFrac overload (supersampled
+//                            fractions) of the level set, DIR/{dax,day,daz}.f32
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -85,6 +93,15 @@ static int solids(int argc, char** argv) {
         sls.InitLsSphere(c, r);
         sls.GetFrac(ax, ay, az);
         ls = to_host(sls.ls_, nc);
+        // the static overload: supersampled fractions of the same level set
+        float *dax, *day, *daz;
+        for (float** a : {&dax, &day, &daz})
+            if (hipMalloc(a, nv * 4)) return 3;
+        CuSolidLevelSet3D::GetFrac(dax, day, daz, sls.ls_, dim, dx);
+        write_f32(dir + "/dax.f32", to_host(dax, nv));
+        write_f32(dir + "/day.f32", to_host(day, nv));
+        write_f32(dir + "/daz.f32", to_host(daz, nv));
+        for (float* a : {dax, day, daz}) (void)hipFree(a);
         CuConvergence::GetInstance()->GetB_3d_frac_vel(f, xv, yv, zv, ax, ay, az, dim, dx, 'n');
         fmax0 = CudaFindMaxValue(f, (int)nc);
         CuMultigridFrac3D mg(ax, ay, az, dim, dx, 'n');
@@ -148,6 +165,21 @@ int main(int argc, char** argv) {
         while ((2 << lz) <= N) ++lz;
         CuWHHDForward::GetInstance()->Solve1D(t, xv, make_int3(N + 1, N, N), lz, 'z', 'n',
                                               WT_CDF3_7, true);
+        // Gram-table cache (WaveletPotentials/cuGetInnerProduct.cu:85-126)
+        CuGetInnerProduct* ip = CuGetInnerProduct::GetInstance();
+        if (ip->GetIp("d_cdf4.6", N + 1) != nullptr || ip->GetIp(WT_CDF4_6, 'd', N + 1) != nullptr) {
+            std::fprintf(stderr, "GetIp returned a table that was never generated\n");
+            return 5;
+        }
+        DTYPE* gd = ip->GenerateIp("d_cdf4.6", N + 1);
+        DTYPE* gd2 = ip->GenerateIp(WT_CDF4_6, 'd', N + 1);
+        DTYPE* gn = ip->GenerateIp(WT_CDF3_7, 'n', N);
+        DTYPE* gn2 = ip->GenerateIp("n_cdf3.7", N);
+        if (gd != gd2 || gn != gn2 || ip->GetIp("d_cdf4.6", N + 1) != gd ||
+            ip->GetIp(WT_CDF3_7, 'n', N) != gn || gd == gn) {
+            std::fprintf(stderr, "GenerateIp did not return the cached table\n");
+            return 5;
+        }
         bool threw = false;
         try {
             CuWaveletPotentialRecover3D bad(make_int3(N + 3, N, N), make_float3(h, h, h), 'n',
@@ -168,6 +200,11 @@ int main(int argc, char** argv) {
     write_f32(dir + "/qy.f32", to_host(qy, nq));
     write_f32(dir + "/qz.f32", to_host(qz, nq));
     write_f32(dir + "/solve1d.f32", to_host(t, nv));
+    {
+        CuGetInnerProduct* ip = CuGetInnerProduct::GetInstance();
+        write_f32(dir + "/gram_d.f32", to_host(ip->GetIp("d_cdf4.6", N + 1), 8 * (size_t)(N + 1)));
+        write_f32(dir + "/gram_n.f32", to_host(ip->GetIp("n_cdf3.7", N), 8 * (size_t)N));
+    }
     for (float* p : {xv, yv, zv, qx, qy, qz, t}) (void)hipFree(p);
     std::printf("ok\n");
     return 0;

@@ -76,3 +76,24 @@ def test_trace_calls_reject_null_handle(L):
     assert L.whhd_recover3d_trace_begin(None, 4) == 1
     assert L.whhd_recover3d_trace_end(None, None, 0, None, None) == 1
     assert L.whhd_recover3d_trace_label(None, 0) is None
+
+
+@pytest.mark.parametrize("n", [(16, 8, 32), (128, 32, 64), (16, 32, 8), (64, 256, 256), (256, 256, 256),
+                               (4, 64, 4)])
+def test_recover3d_scratch_holds_every_pass(L, n):
+    """The handle's ping-pong scratch of component c receives the middle pass of
+    velocity c's chain (MAC extents with +1 on axis c) and of potential c's
+    chain (+1 on the other two axes), so on non-cubic grids it must hold the
+    larger of the two; the transformed velocity needs its own array."""
+    nx, ny, nz = n
+    vel = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
+    pot = [nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1), (nx + 1) * (ny + 1) * nz]
+    want = 4 * sum(v + max(v, p) for v, p in zip(vel, pot))
+    got = ctypes.c_ulonglong()
+    assert L.whhd_recover3d_scratch_bytes(nx, ny, nz, ctypes.byref(got)) == 0
+    assert got.value == want
+    # the rule sized by the potential alone (the old sizing) is short wherever a
+    # velocity array is the larger one
+    short = 4 * sum(v + p for v, p in zip(vel, pot))
+    assert (got.value > short) == any(v > p for v, p in zip(vel, pot))
+    assert L.whhd_recover3d_scratch_bytes(0, 4, 4, ctypes.byref(got)) == 1

@@ -37,6 +37,14 @@ def test_compat_classes_match_oracle(orc, tmp_path):
     t_ref, _ = orc.transform(xv, 4, "z", "n", "cdf3.7", True)
     t_got = np.fromfile(tmp_path / "solve1d.f32", dtype=np.float32).astype(np.float64)
     assert orc.rel_l2(t_ref, t_got.reshape(t_ref.shape)) < 1e-5
+    # CuGetInnerProduct: the program checked the cache (nullptr before
+    # generation, one pointer per key); both tables stay readable afterwards
+    # and equal the oracle's Gram tables rounded to FP32
+    for fname, wt, bc, m in (("gram_d", "cdf4.6", "d", N + 1), ("gram_n", "cdf3.7", "n", N)):
+        g_ref = np.asarray(orc.gram_table(wt, bc, m)).reshape(-1)
+        g_got = np.fromfile(tmp_path / f"{fname}.f32", dtype=np.float32).astype(np.float64)
+        assert g_got.shape == g_ref.shape
+        assert np.abs(g_got - g_ref).max() <= 1e-6 * np.abs(g_ref).max()
 
 
 def test_compat_solid_classes_match_oracle(orc, tmp_path):
@@ -77,3 +85,10 @@ def test_compat_solid_classes_match_oracle(orc, tmp_path):
     q_ref = orc.project_zero(*orc.sweep3d(xv, yv, n, h), n, h)
     q_got = [load(k, q.shape) for k, q in zip(("sx", "sy", "sz"), q_ref)]
     assert orc.rel_l2(list(q_ref), q_got) < 1e-5
+    # the static GetFrac overload runs the supersampled fractions, not the
+    # two-point formula of the member overload
+    d_ref = orc.frac_divide(load("ls", ls_ref.shape), h)
+    for k, a in zip(("dax", "day", "daz"), d_ref):
+        got = load(k, a.shape)
+        assert np.abs(got - a).max() <= 4.0 / 16384, k
+    assert max(np.abs(a - b).max() for a, b in zip(d_ref, fr)) > 1e-2

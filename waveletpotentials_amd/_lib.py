@@ -48,6 +48,7 @@ SIGNATURES = {
     "whhd_project2d_run": (_I, [_P, _P, _P, _P, _P]),
     "whhd_project2d_destroy": (None, [_P]),
     "whhd_recover3d_create": (_I, [_I, _I, _I, _D, _D, _D, _C, _I, _I, ctypes.POINTER(_P)]),
+    "whhd_recover3d_scratch_bytes": (_I, [_I, _I, _I, ctypes.POINTER(ctypes.c_ulonglong)]),
     "whhd_recover3d_solve": (_I, [_P] * 7 + [_P]),
     "whhd_recover3d_solve_proj": (_I, [_P] * 10 + [_P]),
     "whhd_recover3d_cut": (_I, [_P] * 7 + [_I, _P]),
@@ -70,6 +71,7 @@ SIGNATURES = {
     "whhd_recover2d_destroy": (None, [_P]),
     "whhd_levelset_spheres": (_I, [_P, _I, _I, _I, _D, _D, _D, _P, _P, _I, _P]),
     "whhd_frac_from_ls": (_I, [_P, _P, _P, _P, _I, _I, _I, _D, _D, _D, _P]),
+    "whhd_frac_divide": (_I, [_P, _P, _P, _P, _I, _I, _I, _D, _D, _D, _P]),
     "whhd_eikonal_sweep": (_I, [_P, _P, _I, _I, _I, _D, _I, _P]),
     "whhd_mgfrac3d_create": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, _D, _C, ctypes.POINTER(_P)]),
     "whhd_mgfrac3d_levels": (_I, [_P, _P, _I]),

@@ -444,6 +444,18 @@ class CuSolidLevelSet3D:
                                       _ptr(self.ls, name="ls"), *self.dim, *self.dx, _stream()),
               "GetFrac")
 
+    @staticmethod
+    def GetFracDivide(ax, ay, az, ls, dim, dx):
+        """The static GetFrac(ax, ay, az, ls, dim, dx)
+        (WaveletPotentials/cuSolidLevelSet3D.cu:359-372): supersampled fractions
+        from the cuGetA{x,y,z}_divide kernels (:164-317)."""
+        nx, ny, nz = (int(v) for v in dim)
+        check(lib().whhd_frac_divide(_ptr(ax, numel=(nx + 1) * ny * nz, name="ax"),
+                                     _ptr(ay, numel=nx * (ny + 1) * nz, name="ay"),
+                                     _ptr(az, numel=nx * ny * (nz + 1), name="az"),
+                                     _ptr(ls, numel=nx * ny * nz, name="ls"), nx, ny, nz,
+                                     *(float(v) for v in dx), _stream()), "GetFrac (static)")
+
 
 class CuFastSweeping:
     """CuFastSweeping (WaveletPotentials/cuFastSweeping.cuh): eikonal relaxation."""

@@ -5,10 +5,11 @@
 // Per component, the velocity's own axis uses the divergence wavelet with
 // bc_div = proj_bc(bc) and the other axes the curl wavelet with bc; the
 // potential's own axis uses the curl wavelet and the other axes the
-// divergence wavelet.  The handle owns its scratch (one velocity-sized and one
-// potential-sized array per component, allocated at create: passes run out of
-// place) and enqueues everything on the caller's stream without
-// synchronizing, so a solve can be captured in a HIP graph.
+// divergence wavelet.  The handle owns its scratch (per component one
+// velocity-sized array and one array of the larger of the velocity and the
+// potential sizes, allocated at create: passes run out of place) and enqueues
+// everything on the caller's stream without synchronizing, so a solve can be
+// captured in a HIP graph.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -41,7 +42,11 @@ struct whhd_recover3d_s {
     int lp_waves_y = 1;  // the y (contiguous) passes: 0.691 ms against 0.695 with 2
     whhd_project3d_s* proj = nullptr;
     float* tmp[3] = {nullptr, nullptr, nullptr};  // transformed velocity (velocity-sized)
-    float* sq[3] = {nullptr, nullptr, nullptr};   // potential-sized ping-pong scratch
+    // ping-pong scratch of component c: the potential's coefficients and the
+    // middle pass of both of component c's three-pass chains, so it holds
+    // max(velocity size, potential size) elements per problem
+    float* sq[3] = {nullptr, nullptr, nullptr};
+    size_t sq_elems[3] = {0, 0, 0};  // elements per problem of sq[c]
     whhd::StageTrace trace;
 };
 
@@ -103,9 +108,11 @@ static void mark(StageTrace* tr, hipStream_t s, bool fwd, const char* arr, char 
     std::snprintf(lab, sizeof(lab), "%s %s %c", fwd ? "fwd" : "inv", arr, axis);
     tr->mark(s, lab);
 }
-static int three_pass(const float* in, float* scratch, float* out, int nx, int ny, int nz,
-                      const int lv[3], Pass px, Pass py, Pass pz, bool fwd, hipStream_t s,
-                      const whhd_recover3d_s* h, const char* arr) {
+static int three_pass(const float* in, float* scratch, size_t scratch_elems, float* out, int nx,
+                      int ny, int nz, const int lv[3], Pass px, Pass py, Pass pz, bool fwd,
+                      hipStream_t s, const whhd_recover3d_s* h, const char* arr) {
+    if ((size_t)nx * ny * nz > scratch_elems)
+        return fail(WHHD_INVALID_ARG, "recover3d: scratch smaller than the array a pass writes");
     StageTrace* tr = const_cast<StageTrace*>(&h->trace);
     const LineBatch b{h->nb, (int64_t)nx * ny * nz, h->lp_waves};
     const LineBatch by{h->nb, (int64_t)nx * ny * nz, h->lp_waves_y};
@@ -129,29 +136,42 @@ static int three_pass(const float* in, float* scratch, float* out, int nx, int n
 
 // Velocity component c (0: xv, 1: yv, 2: zv): its own axis uses the divergence
 // wavelet, the others the curl wavelet.
-static int vel_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
-                    bool fwd, hipStream_t s) {
+static int vel_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, size_t cap,
+                    float* out, bool fwd, hipStream_t s) {
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X + 1, Y, Z, lv, D, C, C, fwd, s, h, "xv");
-    case 1: return three_pass(in, scratch, out, X, Y + 1, Z, lv, C, D, C, fwd, s, h, "yv");
-    default: return three_pass(in, scratch, out, X, Y, Z + 1, lv, C, C, D, fwd, s, h, "zv");
+    case 0: return three_pass(in, scratch, cap, out, X + 1, Y, Z, lv, D, C, C, fwd, s, h, "xv");
+    case 1: return three_pass(in, scratch, cap, out, X, Y + 1, Z, lv, C, D, C, fwd, s, h, "yv");
+    default: return three_pass(in, scratch, cap, out, X, Y, Z + 1, lv, C, C, D, fwd, s, h, "zv");
     }
 }
 
 // Potential component c (0: qx, 1: qy, 2: qz): its own axis uses the curl
 // wavelet, the others the divergence wavelet.
-static int pot_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, float* out,
-                    bool fwd, hipStream_t s) {
+static int pot_pass(whhd_recover3d_s* h, int c, const float* in, float* scratch, size_t cap,
+                    float* out, bool fwd, hipStream_t s) {
     const Pass C{h->bc, h->wc}, D{h->bd, h->wd};
     const int lv[3] = {h->lx, h->ly, h->lz};
     const int X = h->nx, Y = h->ny, Z = h->nz;
     switch (c) {
-    case 0: return three_pass(in, scratch, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s, h, "qx");
-    case 1: return three_pass(in, scratch, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s, h, "qy");
-    default: return three_pass(in, scratch, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s, h, "qz");
+    case 0: return three_pass(in, scratch, cap, out, X, Y + 1, Z + 1, lv, C, D, D, fwd, s, h, "qx");
+    case 1: return three_pass(in, scratch, cap, out, X + 1, Y, Z + 1, lv, D, C, D, fwd, s, h, "qy");
+    default: return three_pass(in, scratch, cap, out, X + 1, Y + 1, Z, lv, D, D, C, fwd, s, h, "qz");
+    }
+}
+
+// Scratch of a recovery handle, elements per problem: tmp[c] holds velocity
+// component c; sq[c] holds potential component c's coefficients and the
+// middle pass of velocity c's and potential c's chains.
+void recover3d_scratch_elems(int nx, int ny, int nz, size_t tmp[3], size_t sq[3]) {
+    const size_t X = nx, Y = ny, Z = nz;
+    const size_t vel[3] = {(X + 1) * Y * Z, X * (Y + 1) * Z, X * Y * (Z + 1)};
+    const size_t pot[3] = {X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z};
+    for (int c = 0; c < 3; ++c) {
+        tmp[c] = vel[c];
+        sq[c] = vel[c] > pot[c] ? vel[c] : pot[c];
     }
 }
 
@@ -183,9 +203,9 @@ int recover3d_create(int nx, int ny, int nz, double dx, double dy, double dz, ch
         delete h;
         return st;
     }
-    const size_t sz[6] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz,
-                          (size_t)nx * ny * (nz + 1), (size_t)nx * (ny + 1) * (nz + 1),
-                          (size_t)(nx + 1) * ny * (nz + 1), (size_t)(nx + 1) * (ny + 1) * nz};
+    size_t sz[6];
+    recover3d_scratch_elems(nx, ny, nz, sz, sz + 3);
+    for (int c = 0; c < 3; ++c) h->sq_elems[c] = sz[3 + c];
     float** slot[6] = {&h->tmp[0], &h->tmp[1], &h->tmp[2], &h->sq[0], &h->sq[1], &h->sq[2]};
     for (int i = 0; i < 6; ++i)
         if (hipMalloc(slot[i], sz[i] * nb * sizeof(float)) != hipSuccess) {
@@ -211,7 +231,8 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
     float* q[3] = {qx, qy, qz};
     float* vo[3] = {xo, yo, zo};
     h->trace.solve_start(s);
-    for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, v[c], S[c], t[c], true, s));
+    const size_t* cap = h->sq_elems;
+    for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, v[c], S[c], cap[c], t[c], true, s));
     // q coefficients land in the scratch; with v_out, s x q overwrites the
     // transformed velocity in place at the projected coefficients
     if (vout)
@@ -221,9 +242,9 @@ int recover3d_solve(whhd_recover3d_s* h, float* qx, float* qy, float* qz, float*
         TRY(project3d_run(h->proj, S[0], S[1], S[2], nullptr, nullptr, nullptr, t[0], t[1], t[2], s,
                           h->nb, h->proj_zb));
     h->trace.mark(s, "project");
-    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], q[c], false, s));
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], cap[c], q[c], false, s));
     if (vout)
-        for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, t[c], S[c], vo[c], false, s));
+        for (int c = 0; c < 3; ++c) TRY(vel_pass(h, c, t[c], S[c], cap[c], vo[c], false, s));
     h->trace.solve_end();
     return WHHD_OK;
 }
@@ -249,7 +270,11 @@ int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const fl
     const float* q[3] = {qx, qy, qz};
     float* o[3] = {ox, oy, oz};
     h->trace.solve_start(s);
-    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, q[c], o[c], S[c], true, s));
+    // the forward passes ping-pong through the caller's output (potential-sized)
+    const size_t pot_e[3] = {(size_t)h->nx * (h->ny + 1) * (h->nz + 1),
+                             (size_t)(h->nx + 1) * h->ny * (h->nz + 1),
+                             (size_t)(h->nx + 1) * (h->ny + 1) * h->nz};
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, q[c], o[c], pot_e[c], S[c], true, s));
     dim3 block(64, 4, 1);
     dim3 grid((h->ny + 1 + 63) / 64, (h->nx + 1 + 3) / 4, h->nz + 1);
     hipLaunchKernelGGL(cut_kernel, grid, block, 0, s, S[0], S[1], S[2], h->nx, h->ny, h->nz, h->lx,
@@ -258,7 +283,7 @@ int recover3d_cut(whhd_recover3d_s* h, float* ox, float* oy, float* oz, const fl
     if (e != hipSuccess) return hip_fail(e, "cut kernel");
     set_last_kernel("cut_kernel");
     h->trace.mark(s, "cut");
-    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], o[c], false, s));
+    for (int c = 0; c < 3; ++c) TRY(pot_pass(h, c, S[c], S[c], h->sq_elems[c], o[c], false, s));
     h->trace.solve_end();
     return WHHD_OK;
 }
@@ -339,6 +364,15 @@ WHHD_API int whhd_recover3d_cut(whhd_recover3d_t h, float* qx_out, float* qy_out
                                 const float* qx, const float* qy, const float* qz, int level,
                                 void* stream) {
     return whhd::recover3d_cut(h, qx_out, qy_out, qz_out, qx, qy, qz, level, (hipStream_t)stream);
+}
+
+WHHD_API int whhd_recover3d_scratch_bytes(int nx, int ny, int nz, unsigned long long* bytes) {
+    if (!bytes || nx < 1 || ny < 1 || nz < 1)
+        return whhd::fail(WHHD_INVALID_ARG, "recover3d_scratch_bytes: bad argument");
+    size_t t[3], q[3];
+    whhd::recover3d_scratch_elems(nx, ny, nz, t, q);
+    *bytes = (unsigned long long)(t[0] + t[1] + t[2] + q[0] + q[1] + q[2]) * sizeof(float);
+    return WHHD_OK;
 }
 
 WHHD_API int whhd_recover3d_trace_begin(whhd_recover3d_t h, int max_solves) {
