@@ -202,6 +202,33 @@ static void restrict_faces(double* dst, const double* src, i3 fine, int normal) 
     free(t2);
 }
 
+/* SolveP1RN with spacings: x, then y, then z restriction of a cell field of
+ * extents d (cuRestrict3D.cu:939-970). */
+static void restrict_cells(double* dst, const double* src, i3 d, const double* hx,
+                           const double* hy, const double* hz) {
+    const size_t n = (size_t)d.x * d.y * d.z;
+    double* t1 = (double*)malloc(n * sizeof(double));
+    double* t2 = (double*)malloc(n * sizeof(double));
+    i3 s = d;
+    restrict_axis_w(t1, src, s, 0, hx);
+    s.x = half_up(d.x);
+    restrict_axis_w(t2, t1, s, 1, hy);
+    s.y = half_up(d.y);
+    restrict_axis_w(dst, t2, s, 2, hz);
+    free(t1);
+    free(t2);
+}
+
+/* SolveAddP1RN: each fine cell of extents d adds its parent's value
+ * (cuProlongate3D.cu:353-423, 594-616). */
+static void prolong_add(double* fine, const double* coarse, i3 d) {
+    const i3 c = {half_up(d.x), half_up(d.y), half_up(d.z)};
+    for (int z = 0; z < d.z; ++z)
+        for (int x = 0; x < d.x; ++x)
+            for (int y = 0; y < d.y; ++y)
+                AT(fine, y, x, z, d) += AT(coarse, y / 2, x / 2, z / 2, c);
+}
+
 typedef struct {
     int levels;
     i3 dim[32];
@@ -280,25 +307,14 @@ static void vcycle(hierarchy* H, int l, int nu1, int nu2, char bc, const double 
         smooth(H, l, nu1, bc, he, w);
         const size_t n = (size_t)d.x * d.y * d.z;
         double* r = (double*)malloc(n * sizeof(double));
-        double* t1 = (double*)malloc(n * sizeof(double));
-        double* t2 = (double*)malloc(n * sizeof(double));
         apply(r, H->v[l], H->f[l], H->ax[l], H->ay[l], H->az[l], d, H->hx[l], H->hy[l], H->hz[l],
               he, bc, w, 1);
-        i3 s = d;
-        restrict_axis_w(t1, r, s, 0, H->hx[l]);
-        s.x = half_up(d.x);
-        restrict_axis_w(t2, t1, s, 1, H->hy[l]);
-        s.y = half_up(d.y);
-        restrict_axis_w(H->f[l + 1], t2, s, 2, H->hz[l]);
-        free(r); free(t1); free(t2);
+        restrict_cells(H->f[l + 1], r, d, H->hx[l], H->hy[l], H->hz[l]);
+        free(r);
         const i3 c = H->dim[l + 1];
         memset(H->v[l + 1], 0, (size_t)c.x * c.y * c.z * sizeof(double));
         vcycle(H, l + 1, nu1, nu2, bc, he, w);
-        /* SolveAddP1RN: each fine cell adds its parent's correction */
-        for (int z = 0; z < d.z; ++z)
-            for (int x = 0; x < d.x; ++x)
-                for (int y = 0; y < d.y; ++y)
-                    AT(H->v[l], y, x, z, d) += AT(H->v[l + 1], y / 2, x / 2, z / 2, c);
+        prolong_add(H->v[l], H->v[l + 1], d);
         smooth(H, l, nu2, bc, he, w);
     } else {
         coarsest(H->v[l], H->f[l], H->ax[l], H->ay[l], H->az[l], d, H->hx[l], H->hy[l], H->hz[l],
@@ -384,4 +400,48 @@ double orc_mgfrac3d_rel_residual(const double* v, const double* f, const double*
     for (size_t i = 0; i < n; ++i) { sr += r[i] * r[i]; sf += f[i] * f[i]; }
     free(r);
     return sqrt(sr) / sqrt(sf);
+}
+
+/* ---- test hooks: the transfer operators and the level data on their own ---- */
+
+int orc_mg_restrict(double* dst, const double* src, int nx, int ny, int nz, const double* hx,
+                    const double* hy, const double* hz) {
+    if (!dst || !src || !hx || !hy || !hz || nx < 1 || ny < 1 || nz < 1) return ORC_INVALID;
+    restrict_cells(dst, src, (i3){nx, ny, nz}, hx, hy, hz);
+    return ORC_OK;
+}
+
+int orc_mg_prolong_add(double* fine, const double* coarse, int nx, int ny, int nz) {
+    if (!fine || !coarse || nx < 1 || ny < 1 || nz < 1) return ORC_INVALID;
+    prolong_add(fine, coarse, (i3){nx, ny, nz});
+    return ORC_OK;
+}
+
+int orc_mg_restrict_faces(double* dst, const double* src, int fx, int fy, int fz, int normal) {
+    if (!dst || !src || fx < 1 || fy < 1 || fz < 1 || normal < 0 || normal > 2) return ORC_INVALID;
+    restrict_faces(dst, src, (i3){fx, fy, fz}, normal);
+    return ORC_OK;
+}
+
+int orc_mg_level(const double* ax, const double* ay, const double* az, int nx, int ny, int nz,
+                 double dx, double dy, double dz, int level, double* ax_out, double* ay_out,
+                 double* az_out, double* hx_out, double* hy_out, double* hz_out) {
+    if (!ax || !ay || !az || nx < 2 || ny < 2 || nz < 2 || level < 0) return ORC_INVALID;
+    hierarchy H;
+    build_hierarchy(&H, ax, ay, az, (i3){nx, ny, nz}, dx, dy, dz);
+    if (level >= H.levels) {
+        free_hierarchy(&H);
+        return ORC_INVALID;
+    }
+    const i3 d = H.dim[level];
+    const size_t sx = (size_t)(d.x + 1) * d.y * d.z, sy = (size_t)d.x * (d.y + 1) * d.z,
+                 sz = (size_t)d.x * d.y * (d.z + 1);
+    if (ax_out) memcpy(ax_out, H.ax[level], sx * sizeof(double));
+    if (ay_out) memcpy(ay_out, H.ay[level], sy * sizeof(double));
+    if (az_out) memcpy(az_out, H.az[level], sz * sizeof(double));
+    if (hx_out) memcpy(hx_out, H.hx[level], d.x * sizeof(double));
+    if (hy_out) memcpy(hy_out, H.hy[level], d.y * sizeof(double));
+    if (hz_out) memcpy(hz_out, H.hz[level], d.z * sizeof(double));
+    free_hierarchy(&H);
+    return ORC_OK;
 }

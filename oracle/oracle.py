@@ -70,6 +70,10 @@ def _setup(L):
     L.orc_wj_frac3d.argtypes = [_D] * 6 + [i, i, i, d, d, d, c, d, i]
     L.orc_mgfrac3d_rel_residual.argtypes = [_D] * 5 + [i, i, i, d, d, d, c]
     L.orc_mgfrac3d_rel_residual.restype = d
+    L.orc_mg_restrict.argtypes = [_D, _D, i, i, i, _D, _D, _D]
+    L.orc_mg_prolong_add.argtypes = [_D, _D, i, i, i]
+    L.orc_mg_restrict_faces.argtypes = [_D, _D, i, i, i, i]
+    L.orc_mg_level.argtypes = [_D, _D, _D, i, i, i, d, d, d, i] + [_D] * 6
     L.orc_gradient3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d, c]
     L.orc_divergence3d_frac.argtypes = [_D] * 7 + [i, i, i, d, d, d]
     L.orc_sweep3d.argtypes = [_D] * 5 + [i, i, i, d, d, d]
@@ -340,6 +344,61 @@ def mg_rel_residual(v, f, ax, ay, az, dx, bc="n"):
     ax, ay, az = _fr(ax, ay, az)
     return lib().orc_mgfrac3d_rel_residual(_p(v), _p(f), _p(ax), _p(ay), _p(az), nx, ny, nz,
                                            *map(float, dx), _b(bc))
+
+
+def _half_up(n):
+    return (n + 1) // 2
+
+
+def mg_restrict(src, hx, hy, hz):
+    """Spacing-weighted restriction of a cell field (nz, nx, ny) to ceil-halved
+    extents (CuRestrict3D::SolveP1RN with dx, cuRestrict3D.cu:801-970)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    nz, nx, ny = src.shape
+    h = [np.ascontiguousarray(a, dtype=np.float64) for a in (hx, hy, hz)]
+    assert [len(a) for a in h] == [nx, ny, nz]
+    dst = np.empty((_half_up(nz), _half_up(nx), _half_up(ny)))
+    _check(lib().orc_mg_restrict(_p(dst), _p(src), nx, ny, nz, *map(_p, h)), "mg_restrict")
+    return dst
+
+
+def mg_prolong_add(fine, coarse):
+    """fine + (each fine cell's parent value), CuProlongate3D::SolveAddP1RN."""
+    out = np.array(fine, dtype=np.float64, order="C")
+    nz, nx, ny = out.shape
+    coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+    assert coarse.shape == (_half_up(nz), _half_up(nx), _half_up(ny))
+    _check(lib().orc_mg_prolong_add(_p(out), _p(coarse), nx, ny, nz), "mg_prolong_add")
+    return out
+
+
+def mg_restrict_faces(src, normal):
+    """Face-fraction restriction (SolveP1Ax/Ay/Az, cuRestrict3D.cu:972-1140) of a
+    face array (fz, fx, fy) whose normal axis is 0 (x), 1 (y) or 2 (z)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    fz, fx, fy = src.shape
+    ext = [fx, fy, fz]
+    out = [(e >> 1) + 1 if a == normal else _half_up(e) for a, e in enumerate(ext)]
+    dst = np.empty((out[2], out[0], out[1]))
+    _check(lib().orc_mg_restrict_faces(_p(dst), _p(src), fx, fy, fz, int(normal)),
+           "mg_restrict_faces")
+    return dst
+
+
+def mg_level(ax, ay, az, dx, level):
+    """Face fractions and per-axis cell spacings of one level of the hierarchy the
+    MG builds from fine fractions (CuMultigridFrac3D ctor, cuMultigridFrac3D.cu:9-134)."""
+    ax, ay, az = _fr(ax, ay, az)
+    nz, nx, ny = ax.shape[0], ax.shape[1] - 1, ax.shape[2]
+    levels = mg_levels((nx, ny, nz))
+    if not 0 <= level < len(levels):
+        raise ValueError(f"level {level} outside 0..{len(levels) - 1}")
+    cx, cy, cz = levels[level]
+    out = [np.empty((cz, cx + 1, cy)), np.empty((cz, cx, cy + 1)), np.empty((cz + 1, cx, cy)),
+           np.empty(cx), np.empty(cy), np.empty(cz)]
+    _check(lib().orc_mg_level(_p(ax), _p(ay), _p(az), nx, ny, nz, *map(float, dx), int(level),
+                              *map(_p, out)), "mg_level")
+    return out
 
 
 # ---- field operators --------------------------------------------------------

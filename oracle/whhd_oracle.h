@@ -131,6 +131,18 @@ int orc_wj_frac3d(double* dst, const double* src, const double* f, const double*
 double orc_mgfrac3d_rel_residual(const double* v, const double* f, const double* ax,
                                  const double* ay, const double* az, int nx, int ny, int nz,
                                  double dx, double dy, double dz, char bc);
+/* The transfer operators of the V-cycle on their own: spacing-weighted cell
+ * restriction (extents of the fine field and its per-axis spacings),
+ * prolongation added into a fine field, face-fraction restriction (extents of
+ * the fine face array and its normal axis 0/1/2), and the fractions and
+ * spacings of one level of the hierarchy. */
+int orc_mg_restrict(double* dst, const double* src, int nx, int ny, int nz, const double* hx,
+                    const double* hy, const double* hz);
+int orc_mg_prolong_add(double* fine, const double* coarse, int nx, int ny, int nz);
+int orc_mg_restrict_faces(double* dst, const double* src, int fx, int fy, int fz, int normal);
+int orc_mg_level(const double* ax, const double* ay, const double* az, int nx, int ny, int nz,
+                 double dx, double dy, double dz, int level, double* ax_out, double* ay_out,
+                 double* az_out, double* hx_out, double* hy_out, double* hz_out);
 /* ---- field operators (fields_oracle.c) ---- */
 /* Gradient3D_Frac_P for uniform spacing: face gradients xv (nx+1,ny,nz), yv, zv of p. */
 int orc_gradient3d_frac(double* xv, double* yv, double* zv, const double* p, const double* ax,
