@@ -351,6 +351,15 @@ enum { WHHD_REDUCE_SUM = 0, WHHD_REDUCE_SUM_SQ = 1, WHHD_REDUCE_SUM_ABS = 2, WHH
 WHHD_API int whhd_reduce(const float* x, long long n, int op, double* scratch, double* out,
                          void* stream);
 WHHD_API int whhd_reduce_scratch_doubles(void);
+/* whhd_reduce without the workspace argument: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+ * partials live in a static device array of whhd_reduce_scratch_doubles()
+ * doubles (8 KiB + 8 B per GPU, held until the process ends), which every n
+ * fits.  Runs on the calling thread's current device; calls that overlap on
+ * one GPU share that array and race.  Same kernels and results as whhd_reduce.
+ * Synchronizes `stream`. */
+WHHD_API int whhd_reduce_noalloc(const float* x, long long n, int op, double* out, void* stream);
 
 /* ---- sweep + DST comparator ---------------------------------------------- */
 typedef struct whhd_dst3d_s* whhd_dst3d_t;

@@ -97,6 +97,7 @@ SIGNATURES = {
     "whhd_mul": (_I, [_P, _P, _P, ctypes.c_longlong, _P]),
     "whhd_reduce": (_I, [_P, ctypes.c_longlong, _I, _P, ctypes.POINTER(_D), _P]),
     "whhd_reduce_scratch_doubles": (_I, []),
+    "whhd_reduce_noalloc": (_I, [_P, ctypes.c_longlong, _I, ctypes.POINTER(_D), _P]),
     "whhd_dst3d_create": (_I, [_I, _I, _I, _D, _D, _D, ctypes.POINTER(_P)]),
     "whhd_dst3d_solve": (_I, [_P, _P, _P, _P]),
     "whhd_dst3d_destroy": (None, [_P]),

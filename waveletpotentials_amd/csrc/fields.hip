@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "whhd_internal.h"
 
@@ -154,6 +155,34 @@ __global__ void reduce_final_kernel(double* __restrict__ out, const double* __re
     if (threadIdx.x == 0) out[0] = red_op(red_op(l[0], l[1], op), red_op(l[2], l[3], op), op);
 }
 
+// Device scratch of whhd_reduce_noalloc: kRedBlocks FP64 partials and the
+// result (8 KiB + 8 B per GPU, held until the process ends).  Each GPU holds its
+// own copy at its own address; host code uses only the address that
+// hipGetSymbolAddress returns for the current device, never the symbol itself.
+static __device__ __align__(256) double scratch[kRedBlocks + 1];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int reduce_static_scratch(double** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "reduce_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(WHHD_INVALID_ARG, "reduce_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "reduce_noalloc: hipGetSymbolAddress");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WHHD_OK;
+}
+
 int check_face_args(int nx, int ny, int nz, double dx, double dy, double dz, char bc,
                     const char* who) {
     if (nx < 1 || ny < 1 || nz < 1 || nz + 1 > 65535 ||
@@ -280,5 +309,26 @@ WHHD_API int whhd_reduce(const float* x, long long n, int op, double* scratch, d
 }
 
 WHHD_API int whhd_reduce_scratch_doubles(void) { return whhd::kRedBlocks + 1; }
+
+WHHD_API int whhd_reduce_noalloc(const float* x, long long n, int op, double* out, void* stream) {
+    using namespace whhd;
+    if (!x || !out || n < 0 || op < WHHD_REDUCE_SUM || op > WHHD_REDUCE_MAX_ABS)
+        return fail(WHHD_INVALID_ARG, "reduce_noalloc: bad argument");
+    double* sc = nullptr;
+    const int st = reduce_static_scratch(&sc);
+    if (st != WHHD_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // every partial slot is written by its block and the result slot by the
+    // final kernel, so nothing left by an earlier call is read
+    hipLaunchKernelGGL(reduce_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, sc, x,
+                       (int64_t)n, op);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(256), 0, s, sc + kRedBlocks, sc,
+                       kRedBlocks, op);
+    hipError_t e = done("reduce_kernels");
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, sc + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "reduce_noalloc");
+}
 
 }  // extern "C"
