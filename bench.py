@@ -33,6 +33,9 @@ def parse():
     p.add_argument("--n", type=int, default=256, help="cells per axis")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-runs", type=int, default=5)
+    p.add_argument("--no-learned", action="store_true",
+                   help="recover workload: skip the learned-init multigrid measurement")
+    p.add_argument("--learned-seeds", type=int, default=4)
     p.add_argument("--workload", choices=("recover", "batch", "solids", "learned"),
                    default="recover",
                    help="recover: one N^3 grid (the headline); batch: 64 grids of 32^3..128^3 "
@@ -51,9 +54,9 @@ def array_bytes(N):
 
 def stage_bytes(stage, ab):
     """Algorithmic bytes of one traced stage: a line pass reads and writes its
-    array once; the projection reads three velocity and writes three potential
-    arrays once."""
-    if stage == "project":
+    array once; the projection, alone or fused with the potentials' inverse z
+    pass, reads three velocity and writes three potential arrays once."""
+    if stage.startswith("project"):
         return sum(ab[k] for k in ("xv", "yv", "zv", "qx", "qy", "qz"))
     return 2 * ab[stage.split()[1]]
 
@@ -179,7 +182,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (curl of a seeded Fourier potential, |k|<=16, |k|^-11/6)",
         "config": {"workload": f"3D vector-potential recovery {N}^3 MAC grid, curl CDF3.7 'n', "
-                               f"div CDF4.6 'd', 19 stages", "cells": cells},
+                               f"div CDF4.6 'd', {len(stages)} launches", "cells": cells},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes"] if traffic else None,
@@ -217,7 +220,38 @@ def main():
         out["accuracy"]["q_vs_fp64_oracle_rel_l2"] = O.rel_l2(ref, q_gpu)
     else:
         out["cpu_baseline"] = None
+    if not args.no_learned:
+        out["learned_init"] = learned_summary(args.learned_seeds)
     print(json.dumps(out))
+
+
+def learned_summary(n_seeds):
+    """The second half of the metric: the fractional multigrid on held-out 192^3
+    solid-obstacle seeds (8 spheres), GPU time and V-cycles to an FP64 relative
+    residual of 1e-6 from zero and from the U-Net's prediction (prediction
+    included); speedup = summed plain time / summed learned time."""
+    import statistics
+
+    import waveletpotentials_amd as W
+    from waveletpotentials_amd import train_unet as TU
+
+    n = (192, 192, 192)
+    ckpt = os.path.join(ROOT, "waveletpotentials_amd", "weights", "unet_solids192.bin")
+    factor = W.UNet3D.checkpoint_factor(ckpt)
+    net = W.UNet3D(n, factor=factor)
+    net.load(ckpt)
+    rows = TU.evaluate(net, n, [TU.EVAL_SEED0 + i for i in range(max(1, n_seeds))])
+    pm = sum(r["plain_ms"] for r in rows)
+    lm = sum(r["learned_ms"] for r in rows)
+    return {"speedup": round(pm / lm, 3), "target": 2.0, "seeds": len(rows),
+            "network_grid": f"{192 // factor}^3 (factor {factor})",
+            "plain_cycles_mean": statistics.mean(r["plain_cycles"] for r in rows),
+            "learned_cycles_mean": statistics.mean(r["learned_cycles"] for r in rows),
+            "plain_ms_mean": round(pm / len(rows), 3), "learned_ms_mean": round(lm / len(rows), 3),
+            "plain_res_max": max(r["plain_res"] for r in rows),
+            "learned_res_max": max(r["learned_res"] for r in rows),
+            "prediction_rel_err_mean": statistics.mean(r["prediction_rel_err"] for r in rows),
+            "rows": rows}
 
 
 def bench_batch(args):
@@ -355,16 +389,12 @@ def bench_solids(args):
     f = P["f"]
     u0 = [P[k].clone() for k in ("xv", "yv", "zv")]
     u = [t.clone() for t in u0]
-    g = [torch.empty_like(t) for t in u0]
 
     def step():
         for a, b in zip(u, u0):
             a.copy_(b)
         hi, lo, cyc, res = WL.mg_solve_from(mg, f, None, 1e-6, 400)
-        grad.SubtractGradient3D_Frac_P(*u, hi, P["ax"], P["ay"], P["az"], n, P["dx"], "n")
-        grad.Gradient3D_Frac_P(*g, lo, P["ax"], P["ay"], P["az"], n, P["dx"], "n")
-        for a, b, fr in zip(u, g, (P["ax"], P["ay"], P["az"])):
-            a.sub_(fr * b)
+        grad.SubtractGradient3D_Frac_P2(*u, hi, lo, P["ax"], P["ay"], P["az"], n, P["dx"], "n")
         return cyc, res
 
     for _ in range(args.warmup):

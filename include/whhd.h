@@ -333,6 +333,15 @@ WHHD_API int whhd_subtract_gradient_frac(float* xv, float* yv, float* zv, const 
                                          const float* ax, const float* ay, const float* az, int nx,
                                          int ny, int nz, double dx, double dy, double dz, char bc,
                                          void* stream);
+/* The same step for a potential held as the sum of two floats (the
+ * multigrid's p_hi + p_lo from whhd_mgfrac3d_solve_tol): u <- a (u - grad p_hi
+ * - grad p_lo), the two gradients taken separately, in one pass
+ * (WaveletPotentials/TestLocalProjection.cpp:690-701 with p = p_hi + p_lo).
+ * No reference counterpart as a single call. */
+WHHD_API int whhd_subtract_gradient_frac2(float* xv, float* yv, float* zv, const float* p_hi,
+                                          const float* p_lo, const float* ax, const float* ay,
+                                          const float* az, int nx, int ny, int nz, double dx,
+                                          double dy, double dz, char bc, void* stream);
 /* CuConvergence::GetB_3d_frac_vel(b, xv, yv, zv, ax, ay, az, dim, dx, bc)
  * (WaveletPotentials/cuConvergence.cu:40-124, 435-447): b = -div(a u) per cell,
  * over all six faces (both of the reference's kernels compute this value). */

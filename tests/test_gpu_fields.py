@@ -55,6 +55,15 @@ def test_gradient_and_divergence(whhd, orc, bc, n):
     for got, uu, gg, a in zip(du, u, ref, fr):
         want = a * (uu - gg)
         assert np.abs(host(got) - want).max() <= 1e-5 * max(np.abs(want).max(), 1.0)
+    # two-float potential: u <- a (u - grad p_hi - grad p_lo) against the FP64
+    # oracle gradient of p_hi + p_lo (the low part is not rounded away)
+    p_lo = (rng.standard_normal((nz, nx, ny)) * 1e-4).astype(np.float32).astype(np.float64)
+    ref_sum = orc.gradient3d_frac(p + p_lo, *fr, h, bc)
+    du = [cuda(a) for a in u]
+    whhd.CuGradient.GetInstance().SubtractGradient3D_Frac_P2(*du, cuda(p), cuda(p_lo), *dfr, n, h, bc)
+    for got, uu, gg, a in zip(du, u, ref_sum, fr):
+        want = a * (uu - gg)
+        assert np.abs(host(got) - want).max() <= 1e-5 * max(np.abs(want).max(), 1.0)
 
 
 def test_elementwise_and_reductions(whhd):

@@ -111,7 +111,9 @@ def test_adam_steps_reduce_loss_and_checkpoint_roundtrip(whhd, tmp_path):
 def test_trained_network_saves_vcycles(whhd):
     """The committed checkpoint (trained on 2000 generated 192^3 solid-obstacle
     solves) against the plain multigrid on two held-out seeds: the solve from
-    its prediction reaches 1e-6 in fewer V-cycles than from zero."""
+    its prediction reaches 1e-6 in at least 4% fewer V-cycles than from zero in
+    total (the checkpoint measures 1.064x on these two seeds; a regression below
+    that floor fails)."""
     import os
     from waveletpotentials_amd import train_unet as TU
     ckpt = os.path.join(os.path.dirname(whhd.__file__), "weights", "unet_solids192.bin")
@@ -123,4 +125,6 @@ def test_trained_network_saves_vcycles(whhd):
         assert r["plain_res"] <= 1e-6 and r["learned_res"] <= 1e-6
         assert r["learned_cycles"] <= r["plain_cycles"]
         assert r["prediction_rel_err"] < 0.6
-    assert sum(r["learned_cycles"] for r in rows) < sum(r["plain_cycles"] for r in rows)
+    ratio = sum(r["plain_cycles"] for r in rows) / sum(r["learned_cycles"] for r in rows)
+    print(f"learned-init cycle ratio {ratio:.3f}", flush=True)
+    assert ratio >= 1.04

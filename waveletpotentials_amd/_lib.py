@@ -92,6 +92,7 @@ SIGNATURES = {
     "whhd_unet_destroy": (None, [_P]),
     "whhd_gradient3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
     "whhd_subtract_gradient_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _C, _P]),
+    "whhd_subtract_gradient_frac2": (_I, [_P] * 8 + [_I, _I, _I, _D, _D, _D, _C, _P]),
     "whhd_divergence3d_frac": (_I, [_P] * 7 + [_I, _I, _I, _D, _D, _D, _P]),
     "whhd_axpy": (_I, [_P, _P, ctypes.c_longlong, ctypes.c_float, _P]),
     "whhd_mul": (_I, [_P, _P, _P, ctypes.c_longlong, _P]),

@@ -662,6 +662,18 @@ class CuGradient:
                                                 *dim, *(float(v) for v in dx), _b(bc), _stream()),
               "SubtractGradient3D_Frac_P")
 
+    def SubtractGradient3D_Frac_P2(self, xv, yv, zv, p_hi, p_lo, ax, ay, az, dim, dx, bc="n"):
+        """u <- a * (u - grad p_hi - grad p_lo) in one pass, for a potential held
+        as the sum of two floats (the multigrid's SolveToTolerance pair)."""
+        dim = tuple(int(v) for v in dim)
+        nx, ny, nz = dim
+        check(lib().whhd_subtract_gradient_frac2(*_face_ptrs(dim, xv, yv, zv),
+                                                 _ptr(p_hi, numel=nx * ny * nz, name="p_hi"),
+                                                 _ptr(p_lo, numel=nx * ny * nz, name="p_lo"),
+                                                 *_face_ptrs(dim, ax, ay, az, ("ax", "ay", "az")),
+                                                 *dim, *(float(v) for v in dx), _b(bc), _stream()),
+              "SubtractGradient3D_Frac_P2")
+
 
 class CuConvergence:
     """CuConvergence (WaveletPotentials/cuConvergence.cuh): divergence of a*u."""

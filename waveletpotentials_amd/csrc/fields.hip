@@ -56,12 +56,15 @@ __device__ __forceinline__ float face_grad(const float* __restrict__ p, float a,
     return k == 0 ? p[i_hi] / e : -p[i_lo] / e;
 }
 
-// FUSE = false: out = grad p (Gradient3D_Frac_P); FUSE = true: u <- a * (u - grad p).
+// FUSE = false: out = grad p (Gradient3D_Frac_P); FUSE = true: u <- a * (u - grad p),
+// where p = p + p2 when p2 is given (a potential held as a sum of two floats,
+// whose gradients are taken separately so the low part is not rounded away).
 template <bool FUSE>
 __global__ void gradient_kernel(float* __restrict__ ux, float* __restrict__ uy,
                                 float* __restrict__ uz, const float* __restrict__ p,
                                 const float* __restrict__ ax, const float* __restrict__ ay,
-                                const float* __restrict__ az, FaceGrid g) {
+                                const float* __restrict__ az, FaceGrid g,
+                                const float* __restrict__ p2 = nullptr) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
@@ -70,22 +73,25 @@ __global__ void gradient_kernel(float* __restrict__ ux, float* __restrict__ uy,
     if (y < ny && z < g.nz) {
         const int64_t f = ixf(x, y, z, nx, ny);
         const float a = ax[f];
-        const float gr = face_grad(p, a, x, nx, icell(x < nx ? x : nx - 1, y, z, nx, ny),
-                                   icell(x > 0 ? x - 1 : 0, y, z, nx, ny), g.hx, g.ex);
+        const int64_t hi = icell(x < nx ? x : nx - 1, y, z, nx, ny), lo = icell(x > 0 ? x - 1 : 0, y, z, nx, ny);
+        float gr = face_grad(p, a, x, nx, hi, lo, g.hx, g.ex);
+        if (FUSE && p2) gr += face_grad(p2, a, x, nx, hi, lo, g.hx, g.ex);
         ux[f] = FUSE ? a * (ux[f] - gr) : gr;
     }
     if (x < nx && z < g.nz) {
         const int64_t f = iyf(x, y, z, nx, ny);
         const float a = ay[f];
-        const float gr = face_grad(p, a, y, ny, icell(x, y < ny ? y : ny - 1, z, nx, ny),
-                                   icell(x, y > 0 ? y - 1 : 0, z, nx, ny), g.hy, g.ey);
+        const int64_t hi = icell(x, y < ny ? y : ny - 1, z, nx, ny), lo = icell(x, y > 0 ? y - 1 : 0, z, nx, ny);
+        float gr = face_grad(p, a, y, ny, hi, lo, g.hy, g.ey);
+        if (FUSE && p2) gr += face_grad(p2, a, y, ny, hi, lo, g.hy, g.ey);
         uy[f] = FUSE ? a * (uy[f] - gr) : gr;
     }
     if (x < nx && y < ny) {
         const int64_t f = izf(x, y, z, nx, ny);
         const float a = az[f];
-        const float gr = face_grad(p, a, z, g.nz, icell(x, y, z < g.nz ? z : g.nz - 1, nx, ny),
-                                   icell(x, y, z > 0 ? z - 1 : 0, nx, ny), g.hz, g.ez);
+        const int64_t hi = icell(x, y, z < g.nz ? z : g.nz - 1, nx, ny), lo = icell(x, y, z > 0 ? z - 1 : 0, nx, ny);
+        float gr = face_grad(p, a, z, g.nz, hi, lo, g.hz, g.ez);
+        if (FUSE && p2) gr += face_grad(p2, a, z, g.nz, hi, lo, g.hz, g.ez);
         uz[f] = FUSE ? a * (uz[f] - gr) : gr;
     }
 }
@@ -250,6 +256,22 @@ WHHD_API int whhd_subtract_gradient_frac(float* xv, float* yv, float* zv, const 
     hipLaunchKernelGGL(gradient_kernel<true>, node_grid(nx, ny, nz), dim3(64, 4), 0,
                        (hipStream_t)stream, xv, yv, zv, p, ax, ay, az,
                        face_grid(nx, ny, nz, dx, dy, dz, bc));
+    hipError_t e = done("subtract_gradient_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "subtract gradient kernel");
+}
+
+WHHD_API int whhd_subtract_gradient_frac2(float* xv, float* yv, float* zv, const float* p_hi,
+                                          const float* p_lo, const float* ax, const float* ay,
+                                          const float* az, int nx, int ny, int nz, double dx,
+                                          double dy, double dz, char bc, void* stream) {
+    using namespace whhd;
+    if (!xv || !yv || !zv || !p_hi || !p_lo || !ax || !ay || !az)
+        return fail(WHHD_INVALID_ARG, "subtract_gradient_frac2: null argument");
+    int st = check_face_args(nx, ny, nz, dx, dy, dz, bc, "subtract_gradient_frac2");
+    if (st != WHHD_OK) return st;
+    hipLaunchKernelGGL(gradient_kernel<true>, node_grid(nx, ny, nz), dim3(64, 4), 0,
+                       (hipStream_t)stream, xv, yv, zv, p_hi, ax, ay, az,
+                       face_grid(nx, ny, nz, dx, dy, dz, bc), p_lo);
     hipError_t e = done("subtract_gradient_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "subtract gradient kernel");
 }

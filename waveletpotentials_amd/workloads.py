@@ -41,14 +41,9 @@ def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-6, max_cycles=80
     mg = api.CuMultigridFrac3D(ax, ay, az, dims, dx, "n")
     p, p_lo = torch.empty_like(b), torch.empty_like(b)
     cycles, res, _ = mg.SolveToTolerance(p, p_lo, None, b, tol, max_cycles)
-    grad = api.CuGradient.GetInstance()
-    # u <- a (u - grad p) for the high part, then u <- u - a grad p_lo
-    grad.SubtractGradient3D_Frac_P(xv, yv, zv, p, ax, ay, az, dims, dx, "n")
-    gx, gy, gz = torch.empty_like(xv), torch.empty_like(yv), torch.empty_like(zv)
-    grad.Gradient3D_Frac_P(gx, gy, gz, p_lo, ax, ay, az, dims, dx, "n")
-    xv.sub_(ax * gx)
-    yv.sub_(ay * gy)
-    zv.sub_(az * gz)
+    # u <- a (u - grad p_hi - grad p_lo) in one pass
+    api.CuGradient.GetInstance().SubtractGradient3D_Frac_P2(xv, yv, zv, p, p_lo, ax, ay, az, dims,
+                                                           dx, "n")
     return cycles, res
 
 
