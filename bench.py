@@ -36,12 +36,13 @@ def parse():
     p.add_argument("--no-learned", action="store_true",
                    help="recover workload: skip the learned-init multigrid measurement")
     p.add_argument("--learned-seeds", type=int, default=4)
-    p.add_argument("--workload", choices=("recover", "batch", "solids", "learned"),
+    p.add_argument("--workload", choices=("recover", "batch", "solids", "learned", "sweep"),
                    default="recover",
                    help="recover: one N^3 grid (the headline); batch: 64 grids of 32^3..128^3 "
                         "with spheres, one HIP graph per batch; solids: 192^3 with 8 spheres, "
                         "fractional multigrid to 1e-6 and the velocity correction; learned: "
-                        "the same solve from the U-Net's prediction against from zero")
+                        "the same solve from the U-Net's prediction against from zero; sweep: "
+                        "the sweep + DST comparator on the same N^3 input as recover")
     return p.parse_args()
 
 
@@ -93,6 +94,9 @@ def main():
         return
     if args.workload == "learned":
         bench_learned(args)
+        return
+    if args.workload == "sweep":
+        bench_sweep(args)
         return
 
     import numpy as np
@@ -366,6 +370,75 @@ def _line(metric, value, unit, args, ms, data, config, higher=True, dtype="f32")
             "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": higher,
             "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data,
             "config": config}
+
+
+def bench_sweep(args):
+    """The sweep + DST comparator (CuParallelSweep3D::Solve + Project, velocity
+    bc 'n') on the recover workload's N^3 input: a step is the z sweep, the
+    interior node divergence, the DST-I Poisson solve on the (N-1)^3 interior
+    nodes (six line passes and the eigenvalue scaling) and the gradient
+    correction."""
+    import numpy as np
+    import torch
+
+    import waveletpotentials_amd as W
+    from waveletpotentials_amd import inputs as I
+    from waveletpotentials_amd import workloads as WL
+
+    torch.cuda.set_device(0)
+    N = args.n
+    n = (N, N, N)
+    cells = N ** 3
+    dx = (1.0 / N,) * 3
+    xv, yv, zv = WL.spectral_velocity(n, 16, I.seed_for(1))
+    sw = W.CuParallelSweep3D(n, dx, "n")
+    q = [torch.empty(sh, device="cuda") for sh in ((N + 1, N, N + 1), (N + 1, N + 1, N),
+                                                    (N, N + 1, N + 1))]
+
+    def step():
+        sw.Solve(*q, xv, yv, zv)
+        sw.Project(*q)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the DST solve alone, timed with events on the current stream
+    b = torch.randn((N - 1, N - 1, N - 1), device="cuda")
+    phi = torch.empty_like(b)
+    dst = W.CuDst3D((N - 1, N - 1, N - 1), dx)
+    dst.Solve(phi, b)
+    e0, e1 = torch.cuda.Event(True), torch.cuda.Event(True)
+    e0.record()
+    for _ in range(5):
+        dst.Solve(phi, b)
+    e1.record()
+    torch.cuda.synchronize()
+    dst_ms = e0.elapsed_time(e1) / 5
+    # the DST's algorithmic bytes: six line passes and the scaling, each reading
+    # and writing the (N-1)^3 array once
+    dst_bytes = 7 * 2 * 4 * (N - 1) ** 3
+    achieved = dst_bytes / (dst_ms * 1e-3) / 1e9
+    # accuracy: divergence of the projected potential's curl against the input
+    import oracle as O
+    got = [t.double().cpu().numpy() for t in q]
+    u = [t.double().cpu().numpy() for t in (xv, yv, zv)]
+    curl = O.curl3d(*got, n, dx)
+    out = _line("sweep + DST comparator Mcells/s @%d^3" % N, round(cells / (ms * 1e-3) / 1e6, 1),
+                "Mcells/s", args, ms, "synthetic (curl of a seeded Fourier potential, |k|<=16)",
+                {"workload": f"CuParallelSweep3D Solve + Project {N}^3, velocity bc 'n', DST-I on "
+                             f"{N - 1}^3 interior nodes", "cells": cells})
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "DST-I Poisson solve (7 streaming stages / solve time)",
+                       "dst_solve_ms": round(dst_ms, 4), "algorithmic_bytes": dst_bytes}
+    out["accuracy"] = {"curl_q_vs_u_rel_l2": O.rel_l2(u, list(curl))}
+    out["cpu_baseline"] = None
+    print(json.dumps(out))
 
 
 def bench_solids(args):

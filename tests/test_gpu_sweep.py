@@ -1,9 +1,14 @@
 """GPU parity of the sweep + DST comparator against the FP64 oracle.
 
 Tolerances (FP32 vs FP64): sweep potentials 1e-5 relative to the largest
-magnitude (running sums of up to 64 terms); DST solve 1e-5 relative (255-term
-FP32 dot products per axis); the projected potential 1e-4 relative, and its
-node divergence below 1e-4 of the largest edge term.
+magnitude (running sums of up to 64 terms); DST solve 1e-5 relative (FP32
+transforms: FFT lengths 2^L - 1, dense 255-term dot products otherwise); the
+projected potential 1e-4 relative, and its node divergence below 2e-4 of the
+largest edge term: the Laplacian of the FP32 roundoff left in phi by the
+inverse transforms multiplies it by up to lambda_max / lambda_min ~ (N/pi)^2,
+and an FP32 FFT of 32-128 points rounds about twice as much as a dense
+product (1.8e-7 against 1.0e-7 relative per transform at n = 63, numpy
+float32 emulation), so 64^3 reaches 1.1e-4 of the edge term.
 """
 import numpy as np
 import pytest
@@ -48,7 +53,7 @@ def test_sweep_and_project(whhd, orc, n):
     div = ((qx[1:-1, 1:, 1:-1] - qx[1:-1, :-1, 1:-1]) / h[0]
            + (qy[1:-1, 1:-1, 1:] - qy[1:-1, 1:-1, :-1]) / h[1]
            + (qz[1:, 1:-1, 1:-1] - qz[:-1, 1:-1, 1:-1]) / h[2])
-    assert np.abs(div).max() <= 1e-4 * scale / min(h)
+    assert np.abs(div).max() <= 2e-4 * scale / min(h)
 
 
 @pytest.mark.parametrize("shape,h", [((15, 15, 15), (0.1, 0.1, 0.1)), ((31, 17, 23), (0.1, 0.2, 0.15))])
@@ -67,7 +72,31 @@ def test_dst_solve(whhd, orc, shape, h):
     assert np.abs(host(t) - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("shape,h", [((15, 15), (0.1, 0.1)), ((127, 33), (0.01, 0.03)), ((1, 7), (0.5, 0.2))])
+@pytest.mark.parametrize("shape,h", [((255, 15, 31), (0.01, 0.1, 0.05)), ((63, 127, 7), (0.02, 0.01, 0.2)),
+                                     ((31, 63, 255), (0.05, 0.03, 0.004))])
+def test_dst_fft_lengths(whhd, orc, shape, h):
+    """Lengths n with n + 1 a power of two take the FFT kernel on every axis:
+    FP32 against the FP64 oracle at 1e-5, FP64 at 1e-12, out of place and in place."""
+    rng = np.random.default_rng(sum(shape) + 7)
+    n1, n2, n3 = shape
+    b = rng.standard_normal((n3, n1, n2))
+    ref = orc.dst3d_solve(b.astype(np.float32).astype(np.float64), h)
+    d = whhd.CuDst3D(shape, h)
+    phi = torch.empty((n3, n1, n2), device="cuda")
+    d.Solve(phi, cuda(b))
+    assert np.abs(host(phi) - ref).max() <= 1e-5 * np.abs(ref).max()
+    t = cuda(b)
+    d.Solve(t, t)
+    assert np.abs(host(t) - ref).max() <= 1e-5 * np.abs(ref).max()
+    ref64 = orc.dst3d_solve(b, h)
+    d64 = whhd.CuDst3D(shape, h, dtype=torch.float64)
+    phi64 = torch.empty((n3, n1, n2), device="cuda", dtype=torch.float64)
+    d64.Solve(phi64, cuda64(b))
+    assert np.abs(host(phi64) - ref64).max() <= 1e-12 * np.abs(ref64).max()
+
+
+@pytest.mark.parametrize("shape,h", [((15, 15), (0.1, 0.1)), ((127, 33), (0.01, 0.03)), ((1, 7), (0.5, 0.2)),
+                                     ((255, 63), (0.004, 0.02))])
 def test_dst2d_solve(whhd, orc, shape, h):
     """CuDst (2D) against the oracle, out of place and in place."""
     rng = np.random.default_rng(sum(shape) + 1)

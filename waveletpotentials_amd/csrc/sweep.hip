@@ -64,25 +64,45 @@ __global__ void sweep_kernel(T* __restrict__ qx, T* __restrict__ qy,
 
 // Top gauge for velocity bc 'n' (cuSweepingQ_x and the clears that follow it):
 // per y, qp(x) = sum_{x' < x} qx(x', y, top) * dx; the top planes of qx and qy
-// become zero and the top plane of qz takes -qp / dz.
+// become zero and the top plane of qz takes -qp / dz.  A workgroup takes 64
+// consecutive y (one per lane, coalesced rows) and splits x over its eight
+// wavefronts: each sums its run of x, the runs' offsets come through LDS, and
+// each then walks its run again writing qz.
+constexpr int kGaugeWaves = 8;
+
 template <class T>
-__global__ void top_gauge_kernel(T* __restrict__ qx, T* __restrict__ qy,
-                                 T* __restrict__ qz, int nx, int ny, int nz, T dx,
-                                 T dz) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y > ny) return;
-    const int64_t tqx = (int64_t)nz * nx * (ny + 1);        // top plane of qx
-    const int64_t tqy = (int64_t)nz * (nx + 1) * ny;        // top plane of qy
+__global__ void __launch_bounds__(64 * kGaugeWaves) top_gauge_kernel(T* __restrict__ qx, T* __restrict__ qy,
+                                                                     T* __restrict__ qz, int nx, int ny, int nz,
+                                                                     T dx, T dz) {
+    __shared__ T part[kGaugeWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int y = blockIdx.x * 64 + lane;
+    const bool ok = y <= ny;
+    const int64_t tqx = (int64_t)nz * nx * (ny + 1);              // top plane of qx
+    const int64_t tqy = (int64_t)nz * (nx + 1) * ny;              // top plane of qy
     const int64_t tqz = (int64_t)(nz - 1) * (nx + 1) * (ny + 1);  // top plane of qz
+    const int run = (nx + kGaugeWaves - 1) / kGaugeWaves;
+    const int xa = 1 + w * run, xb = min(nx, (w + 1) * run);      // x in [xa, xb]
+    T sum = T(0);
+    if (ok) {
+#pragma unroll 8
+        for (int x = xa; x <= xb; ++x) sum += qx[tqx + y + (int64_t)(x - 1) * (ny + 1)] * dx;
+    }
+    part[w][lane] = sum;
+    __syncthreads();
     T qp = T(0);
-    for (int x = 1; x <= nx; ++x) {
-        T* e = qx + tqx + y + (int64_t)(x - 1) * (ny + 1);
-        qp += *e * dx;
-        *e = T(0);
-        qz[tqz + y + (int64_t)x * (ny + 1)] -= qp / dz;
+    for (int k = 0; k < w; ++k) qp += part[k][lane];
+    if (ok) {
+#pragma unroll 8
+        for (int x = xa; x <= xb; ++x) {
+            T* e = qx + tqx + y + (int64_t)(x - 1) * (ny + 1);
+            qp += *e * dx;
+            *e = T(0);
+            qz[tqz + y + (int64_t)x * (ny + 1)] -= qp / dz;
+        }
     }
     if (y < ny)
-        for (int x = 0; x <= nx; ++x) qy[tqy + y + (int64_t)x * ny] = T(0);
+        for (int x = w; x <= nx; x += kGaugeWaves) qy[tqy + y + (int64_t)x * ny] = T(0);
 }
 
 // Velocity bc 'd' bottom plane (SweepingQy_x, WaveletPotentials/cuParallelSweep3D.cu:
@@ -269,6 +289,211 @@ __global__ void __launch_bounds__(256) dst_pass_kernel(T* __restrict__ out,
         }
 }
 
+// ---- DST-I through a half-length complex FFT ---------------------------------
+// For a line of n samples with M = n + 1 a power of two (4 <= M <= 1024), the
+// orthonormal DST-I X_j = c sum_i x_i sin(pi (i+1)(j+1) / M), c = sqrt(2/M), is
+// F_{j+1} c with F_k = sum_{m=1}^{M-1} f_m sin(pi m k / M), f_m = x_{m-1}, f_0 =
+// f_M = 0, computed in O(M log M) (the odd/even split of the sine transform
+// through a real DFT of length M):
+//   g_m = sin(pi m / M) (f_m + f_{M-m}) + (f_m - f_{M-m}) / 2, m = 0..M-1;
+//   G = DFT_M(g) from the P = M/2 point complex FFT of h_t = g_{2t} + i g_{2t+1}:
+//       G_k = E_k + e^{-2 pi i k / M} O_k, E_k = (H_k + conj H_{P-k}) / 2,
+//       O_k = (H_k - conj H_{P-k}) / (2i);
+//   F_{2k} = -Im G_k (k >= 1), F_1 = Re G_0 / 2, F_{2k+1} = F_{2k-1} + Re G_k.
+// (the sine-symmetric part of g carries the odd outputs, the antisymmetric
+// part the even ones).  A workgroup stages a tile of lines in LDS with
+// coalesced accesses (rows of consecutive y for the x and z axes, whole
+// contiguous lines for y), each half-wavefront transforms one line at a time
+// (an in-place radix-2 FFT over its lanes, a prefix sum for the odd outputs),
+// and the tile is written back.  The transform is its own inverse.
+//
+// Per-axis table (T): sin(pi m / M) for m < M, then e^{-2 pi i t / P} for
+// t < P/2 (re, im), then e^{-2 pi i k / M} for k < P (re, im).
+constexpr int kFftThreads = 256;
+
+template <class T>
+struct Cx {
+    T r, i;
+};
+
+// inclusive prefix sum over each half (32 lanes) of the wavefront
+template <class T>
+__device__ __forceinline__ T half_scan(T v, int sl) {
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const T u = __shfl_up(v, o, 32);
+        if (sl >= o) v += u;
+    }
+    return v;
+}
+
+// lines: li = inner + outer * n_inner; line li starts at inner * s_inner +
+// outer * s_outer and its samples are `step` apart; a tile is `tl` lines
+// li = tile * tl + l.  A tile is read whole before it is written, and no other
+// tile touches its lines, so `out` may be `in`.
+// With `lam_s` set, the outputs are divided by the eigenvalue sum of their
+// node: lam_q[li / lam_div] + lam_r[li % lam_div] for line li plus lam_s[j]
+// for sample j (the Poisson solve's scaling, fused into the last forward pass).
+// reps = 2 transforms each staged line twice (the solve's last forward pass,
+// the scaling and its first inverse pass share the axis and the tile).
+struct DstScale {
+    const void* lam_q;
+    const void* lam_r;
+    const void* lam_s;
+    int64_t lam_div;
+};
+
+template <class T, int LOGM>
+__global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* in,
+                                                              const T* __restrict__ tab, int n,
+                                                              int64_t lines, int64_t n_inner, int64_t s_inner,
+                                                              int64_t s_outer, int64_t step, int tl, T c,
+                                                              DstScale sc, int reps) {
+    extern __shared__ unsigned char dst_smem[];
+    constexpr int M = 1 << LOGM, P = M >> 1, rs = M + 1;  // row stride: conflict-free strided rows
+    T* rows = (T*)dst_smem;                                  // tl x rs
+    Cx<T>* scr = (Cx<T>*)(rows + (size_t)tl * rs);           // 4 waves x 2 lines x P
+    int64_t* bases = (int64_t*)(scr + 8 * P);                // tl line starts, -1 past the end
+    T* lsum = (T*)(bases + tl);                              // tl eigenvalue sums (scaling)
+    T* ltab = lsum + tl;                                     // the axis table (2.5 M values)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t li0 = (int64_t)blockIdx.x * tl;
+    const T* lam_s = (const T*)sc.lam_s;
+    if (tid < tl) {
+        const int64_t li = li0 + tid;
+        const int64_t o = li / n_inner, i = li - o * n_inner;
+        bases[tid] = li < lines ? i * s_inner + o * s_outer : -1;
+        if (lam_s && li < lines) {
+            const int64_t q = li / sc.lam_div, r = li - q * sc.lam_div;
+            lsum[tid] = ((const T*)sc.lam_q)[q] + ((const T*)sc.lam_r)[r];
+        }
+    }
+    for (int e = tid; e < M + 3 * P; e += kFftThreads) ltab[e] = tab[e];
+    __syncthreads();
+    // ---- stage the tile: row slots 0 and M hold f_0 = f_M = 0, slots 1..n the line
+    if (step == 1) {
+        for (int l = 0; l < tl; ++l) {
+            const int64_t b = bases[l];
+            for (int j = tid; j <= M; j += kFftThreads)
+                rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? in[b + (j - 1)] : T(0);
+        }
+    } else {
+        // tl divides the workgroup: each thread keeps one line of the tile
+        const int l = tid % tl, jstep = kFftThreads / tl;
+        const int64_t b = bases[l];
+        for (int j = tid / tl; j <= M; j += jstep)
+            rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? in[b + (int64_t)(j - 1) * step] : T(0);
+    }
+    __syncthreads();
+    const T* sn = ltab;
+    const Cx<T>* tw = (const Cx<T>*)(ltab + M);
+    const Cx<T>* ek = tw + P / 2;
+    // each wavefront transforms two lines at a time, one per half (32 lanes):
+    // two independent chains of LDS round trips in flight
+    const int half_id = lane >> 5, sl = lane & 31;
+    Cx<T>* x = scr + (wave * 2 + half_id) * P;
+    for (int l0 = wave * 2; l0 < tl; l0 += 2 * (kFftThreads / 64))
+    for (int rep = 0; rep < reps; ++rep) {
+        const int l = l0 + half_id;
+        const bool live = l < tl;
+        T* f = rows + (live ? l : l0) * rs;
+        // h_t = g_{2t} + i g_{2t+1}, stored bit-reversed for the in-place FFT
+#pragma unroll
+        for (int t = sl; t < P; t += 32) {
+            Cx<T> h;
+            {
+                const int m = 2 * t;
+                const T a = f[m], b = f[M - m];
+                h.r = sn[m] * (a + b) + T(0.5) * (a - b);
+            }
+            {
+                const int m = 2 * t + 1;
+                const T a = f[m], b = f[M - m];
+                h.i = sn[m] * (a + b) + T(0.5) * (a - b);
+            }
+            const int r = (int)(__brev((unsigned)t) >> (33 - LOGM));
+            x[r] = h;
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int sh = 0; sh < LOGM - 1; ++sh) {
+            const int half = 1 << sh;
+            const int tstep = P >> (sh + 1);
+#pragma unroll
+            for (int bfl = sl; bfl < P / 2; bfl += 32) {
+                const int g = bfl >> sh, k = bfl & (half - 1);
+                const int i0 = g * 2 * half + k, i1 = i0 + half;
+                const Cx<T> w = tw[k * tstep];
+                const Cx<T> a = x[i0], b = x[i1];
+                const T tr = w.r * b.r - w.i * b.i, ti = w.r * b.i + w.i * b.r;
+                x[i0] = Cx<T>{a.r + tr, a.i + ti};
+                x[i1] = Cx<T>{a.r - tr, a.i - ti};
+            }
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+        }
+        // G_k -> outputs; lane holds the `per` consecutive k of its block
+        constexpr int per = P >= 32 ? P / 32 : 1;
+        const int k0 = sl * per;
+        T run = T(0);
+        T rk[per], ik[per];
+#pragma unroll
+        for (int q = 0; q < per; ++q) {
+            const int k = k0 + q;
+            rk[q] = T(0);
+            ik[q] = T(0);
+            if (k < P) {
+                const Cx<T> hk = x[k], hm = x[(P - k) & (P - 1)];
+                const T er = T(0.5) * (hk.r + hm.r), ei = T(0.5) * (hk.i - hm.i);
+                const T dr = hk.r - hm.r, di = hk.i + hm.i;
+                const T orr = T(0.5) * di, oi = T(-0.5) * dr;
+                const Cx<T> e = ek[k];
+                rk[q] = er + e.r * orr - e.i * oi;
+                ik[q] = ei + e.r * oi + e.i * orr;
+                if (k == 0) rk[q] *= T(0.5);
+            }
+            run += rk[q];
+        }
+        T pre = half_scan(run, sl) - run;  // exclusive prefix of the odd-output sums
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < per; ++q) {
+                const int k = k0 + q;
+                if (k < P) {
+                    pre += rk[q];
+                    T xe = c * pre;                         // X_{2k} = c F_{2k+1}
+                    T xo = -c * ik[q];                      // X_{2k-1} = c F_{2k}
+                    if (lam_s && rep == 0) {
+                        xe /= lsum[l] + lam_s[2 * k];
+                        if (k >= 1) xo /= lsum[l] + lam_s[2 * k - 1];
+                    }
+                    f[1 + 2 * k] = xe;
+                    if (k >= 1) f[2 * k] = xo;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+    }
+    __syncthreads();
+    // ---- write the tile back (slots 1..n)
+    if (step == 1) {
+        for (int l = 0; l < tl; ++l) {
+            const int64_t b = bases[l];
+            if (b < 0) break;
+            for (int j = tid; j < n; j += kFftThreads) out[b + j] = rows[l * rs + 1 + j];
+        }
+    } else {
+        const int l = tid % tl, jstep = kFftThreads / tl;
+        const int64_t b = bases[l];
+        if (b >= 0)
+            for (int j = tid / tl; j < n; j += jstep) out[b + (int64_t)j * step] = rows[l * rs + 1 + j];
+    }
+}
+
 // phi_hat /= lambda_x + lambda_y + lambda_z (cuDstProj_dx)
 template <class T>
 __global__ void eigen_scale_kernel(T* __restrict__ a, const T* __restrict__ lx,
@@ -297,6 +522,9 @@ struct whhd_dst3d_s {
     int n1, n2, n3;
     bool planar = false;  // 2D solver (CuDst): no z passes, no z eigenvalue
     bool fp64 = false;
+    // per axis: log2(n + 1) when the FFT kernel applies (S holds its table),
+    // else 0 (S holds the dense orthonormal sine matrix)
+    int logm[3] = {0, 0, 0};
     void* S[3] = {nullptr, nullptr, nullptr};
     void* lam[3] = {nullptr, nullptr, nullptr};
     void* tmp = nullptr;
@@ -327,9 +555,18 @@ void dst_release(whhd_dst3d_s* h) {
 template <class T>
 constexpr bool is64() { return sizeof(T) == sizeof(double); }
 
+// Lines per FFT tile: a divisor of the workgroup size, with the staged rows
+// within 20 KB of LDS.
+int dst_fft_tile(int M, size_t elem) {
+    int tl = 64;
+    while (tl > 4 && (size_t)tl * (M + 1) * elem > 20 * 1024) tl >>= 1;
+    return tl;
+}
+
 // one DST-I pass along axis a (0: x, 1: y, 2: z) of an (n1, n2, n3) y-fastest array
 template <class T>
-int dst_pass(const whhd_dst3d_s* h, T* out, const T* in, int a, hipStream_t s) {
+int dst_pass(const whhd_dst3d_s* h, T* out, const T* in, int a, hipStream_t s,
+             bool scale = false, int reps = 1) {
     const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
     int n, m;
     int64_t si, sj, sb;
@@ -340,6 +577,41 @@ int dst_pass(const whhd_dst3d_s* h, T* out, const T* in, int a, hipStream_t s) {
         n = n2; m = n1 * n3; si = n2; sj = 1; sb = 0; nb = 1;
     } else {
         n = n3; m = n1 * n2; si = 1; sj = (int64_t)n1 * n2; sb = 0; nb = 1;
+    }
+    if (h->logm[a]) {
+        const int lm = h->logm[a], M = 1 << lm, P = M / 2;
+        int64_t lines, n_inner, s_inner, s_outer, step;
+        if (a == 0) {
+            lines = (int64_t)n2 * n3; n_inner = n2; s_inner = 1; s_outer = (int64_t)n1 * n2; step = n2;
+        } else if (a == 1) {
+            lines = (int64_t)n1 * n3; n_inner = lines; s_inner = n2; s_outer = 0; step = 1;
+        } else {
+            lines = (int64_t)n1 * n2; n_inner = lines; s_inner = 1; s_outer = 0; step = (int64_t)n1 * n2;
+        }
+        const int tl = dst_fft_tile(M, sizeof(T));
+        const size_t lds = (size_t)tl * (M + 1) * sizeof(T) + 8 * (size_t)P * 2 * sizeof(T) +
+                           (size_t)tl * sizeof(int64_t) + (size_t)tl * sizeof(T) +
+                           (size_t)(M + 3 * P) * sizeof(T);
+        const dim3 grid((unsigned)((lines + tl - 1) / tl));
+        DstScale sc{nullptr, nullptr, nullptr, 1};
+        if (scale) {
+            // lines of the z pass are li = y + x n2; of the planar solver's y pass li = x
+            if (a == 2) sc = DstScale{h->lam[0], h->lam[1], h->lam[2], n2};
+            else sc = DstScale{h->lam[2], h->lam[0], h->lam[1], n1};
+        }
+        const T c = (T)std::sqrt(2.0 / M);
+        const T* tab = (const T*)h->S[a];
+#define WHHD_DST_FFT(L) \
+    case L: hipLaunchKernelGGL((dst_fft_kernel<T, L>), grid, dim3(kFftThreads), lds, s, out, in, tab, n, \
+                               lines, n_inner, s_inner, s_outer, step, tl, c, sc, reps); break;
+        switch (lm) {
+            WHHD_DST_FFT(2) WHHD_DST_FFT(3) WHHD_DST_FFT(4) WHHD_DST_FFT(5) WHHD_DST_FFT(6)
+            WHHD_DST_FFT(7) WHHD_DST_FFT(8) WHHD_DST_FFT(9) WHHD_DST_FFT(10)
+            default: return fail(WHHD_INVALID_ARG, "dst: FFT length out of range");
+        }
+#undef WHHD_DST_FFT
+        hipError_t e = done("dst_fft_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "dst fft kernel");
     }
     dim3 grid((unsigned)((m + kTI - 1) / kTI), (unsigned)((n + kTK - 1) / kTK), nb);
     hipLaunchKernelGGL(dst_pass_kernel<T>, grid, dim3(256), 0, s, out, in, (const T*)h->S[a], n, m,
@@ -371,14 +643,34 @@ int dst3d_create_t(int n1, int n2, int n3, double dx, double dy, double dz, bool
         // orthonormal DST-I, S(k, j) = sqrt(2/(n+1)) sin(pi (k+1)(j+1) / (n+1)), and the
         // eigenvalues (2 - 2 cos(pi (k+1)/(n+1))) / h^2 of the 3-point -d2/dx2 (none
         // on z for the 2D solver)
-        std::vector<T> S((size_t)n * n), lam(n);
-        const double sc = std::sqrt(2.0 / (n + 1));
-        for (int k = 0; k < n; ++k) {
-            for (int j = 0; j < n; ++j)
-                S[(size_t)k * n + j] = (T)(sc * std::sin(M_PI * (double)((int64_t)(k + 1) * (j + 1) % (2 * (n + 1))) / (n + 1)));
+        const int M = n + 1;
+        const bool fft = M >= 4 && M <= 1024 && (M & (M - 1)) == 0;
+        std::vector<T> S, lam(n);
+        if (fft) {
+            // sin(pi m / M), m < M; e^{-2 pi i t / P}, t < P/2; e^{-2 pi i k / M}, k < P
+            int lm = 0;
+            while ((1 << lm) < M) ++lm;
+            h->logm[a] = lm;
+            const int P = M / 2;
+            for (int m = 0; m < M; ++m) S.push_back((T)std::sin(M_PI * m / M));
+            for (int t = 0; t < P / 2; ++t) {
+                S.push_back((T)std::cos(2.0 * M_PI * t / P));
+                S.push_back((T)-std::sin(2.0 * M_PI * t / P));
+            }
+            for (int k = 0; k < P; ++k) {
+                S.push_back((T)std::cos(2.0 * M_PI * k / M));
+                S.push_back((T)-std::sin(2.0 * M_PI * k / M));
+            }
+        } else {
+            S.resize((size_t)n * n);
+            const double sc = std::sqrt(2.0 / (n + 1));
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j)
+                    S[(size_t)k * n + j] = (T)(sc * std::sin(M_PI * (double)((int64_t)(k + 1) * (j + 1) % (2 * (n + 1))) / (n + 1)));
+        }
+        for (int k = 0; k < n; ++k)
             lam[k] = (planar && a == 2) ? T(0)
                                         : (T)((2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1))) / (hs[a] * hs[a]));
-        }
         if (hipMalloc(&h->S[a], S.size() * sizeof(T)) != hipSuccess ||
             hipMalloc(&h->lam[a], lam.size() * sizeof(T)) != hipSuccess ||
             hipMemcpy(h->S[a], S.data(), S.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
@@ -405,6 +697,13 @@ int dst3d_solve_t(const whhd_dst3d_s* h, T* phi, const T* b, hipStream_t s) {
     const T* const* lam = (const T* const*)h->lam;
     if (h->planar) {
         // b -> tmp (x) -> phi (y), scale, phi -> tmp (y) -> phi (x)
+        if (h->logm[0] && h->logm[1]) {
+            // FFT passes stage whole lines per tile, so they run in place: x, then
+            // forward y, scaling and inverse y in one pass, then x
+            TRY(dst_pass<T>(h, phi, b, 0, s));
+            TRY(dst_pass<T>(h, phi, phi, 1, s, true, 2));
+            return dst_pass<T>(h, phi, phi, 0, s);
+        }
         TRY(dst_pass<T>(h, tmp, b, 0, s));
         TRY(dst_pass<T>(h, phi, tmp, 1, s));
         hipLaunchKernelGGL(eigen_scale_kernel<T>, grid_xyz(h->n1, h->n2, 1), dim3(64, 4), 0, s, phi,
@@ -415,6 +714,15 @@ int dst3d_solve_t(const whhd_dst3d_s* h, T* phi, const T* b, hipStream_t s) {
         return dst_pass<T>(h, phi, tmp, 0, s);
     }
     // b -> tmp (x) -> phi (y) -> tmp (z), scale, tmp -> phi (z) -> tmp (y) -> phi (x)
+    if (h->logm[0] && h->logm[1] && h->logm[2]) {
+        // FFT passes stage whole lines per tile, so they run in place: x, y, then
+        // forward z, scaling and inverse z in one pass, then y, x
+        TRY(dst_pass<T>(h, phi, b, 0, s));
+        TRY(dst_pass<T>(h, phi, phi, 1, s));
+        TRY(dst_pass<T>(h, phi, phi, 2, s, true, 2));
+        TRY(dst_pass<T>(h, phi, phi, 1, s));
+        return dst_pass<T>(h, phi, phi, 0, s);
+    }
     TRY(dst_pass<T>(h, tmp, b, 0, s));
     TRY(dst_pass<T>(h, phi, tmp, 1, s));
     TRY(dst_pass<T>(h, tmp, phi, 2, s));
@@ -485,8 +793,8 @@ int sweep3d_solve_t(whhd_sweep3d_t h, T* qx, T* qy, T* qz, const T* xv, const T*
     hipLaunchKernelGGL(sweep_kernel<T>, grid_xyz(h->nx + 1, h->ny + 1, 1), dim3(64, 4), 0, s, qx, qy,
                        qz, xv, yv, h->nx, h->ny, h->nz, (T)h->dz, d ? 1 : 0);
     if (!d)
-        hipLaunchKernelGGL(top_gauge_kernel<T>, dim3((h->ny + 1 + 255) / 256), dim3(256), 0, s, qx,
-                           qy, qz, h->nx, h->ny, h->nz, (T)h->dx, (T)h->dz);
+        hipLaunchKernelGGL(top_gauge_kernel<T>, dim3((h->ny + 1 + 63) / 64), dim3(64 * kGaugeWaves), 0, s,
+                           qx, qy, qz, h->nx, h->ny, h->nz, (T)h->dx, (T)h->dz);
     hipError_t e = done("sweep_kernels");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "sweep kernels");
 }
