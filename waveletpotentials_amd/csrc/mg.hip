@@ -461,75 +461,145 @@ __global__ void __launch_bounds__(1024) norm_final_kernel(double* __restrict__ o
 // ---- the coarse levels of a V-cycle in one workgroup -----------------------
 // Below a few thousand cells a level's launches are latency-bound (a few
 // microseconds each, six per level per cycle).  This kernel runs the V-cycle
-// from level `first` to the coarsest and back in one workgroup of 1024
-// threads, with the same sweep, residual, restriction and prolongation as the
-// per-level kernels (workgroup barriers between steps) and the coarsest
-// level's sweeps in FP64.  The residual of a level goes to its Jacobi partner
-// array, which is free once the smoothing has ended in v.
-struct LevelDev {
-    LevelArgs a;
-    float *f, *v, *t;
+// from the first small level to the coarsest and back in one workgroup, with
+// every small level's iterate, partner and right-hand side in LDS and each
+// cell's face coefficients precomputed at create (the same FP32 values and
+// summation order as the per-level kernels), so a sweep is seven coefficient
+// loads, seven LDS reads and a few FMAs per cell.  The coarsest level's sweeps
+// run in FP64, on one wavefront when it has at most 64 cells.
+constexpr int kSmallThreads = 256;
+constexpr int kMaxSmall = 16;
+
+struct SmallLevel {
+    int nx, ny, nz, n;
+    const float* hx;
+    const float* hy;
+    const float* hz;
+    const float* coef;    // 7 x n: y-, y+, x-, x+, z-, z+ (0 without a neighbour), diagonal
+    const double* diag64; // the coarsest level: FP64 diagonal (walls included)
+    int ov, ot, of;       // LDS float offsets of v, t, f
 };
 
+struct SmallCycle {
+    SmallLevel lv[kMaxSmall];
+    int nlev;             // small levels, the coarsest included
+    int o64;              // LDS float offset of the coarsest level's FP64 ping-pong
+    float* v0;            // the first small level's iterate (global)
+    const float* f0;      // and its right-hand side
+};
+
+// Face coefficients of every cell of one level, in stencil()'s order.
 template <bool Z>
-__device__ void blk_jacobi(const LevelDev& L, float* __restrict__ dst, const float* __restrict__ src,
-                           float w) {
-    const LevelArgs& a = L.a;
-    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx), z = (int)(i / ((int64_t)a.ny * a.nx));
-        float diag, off;
-        stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
-        dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (L.f[i] - off) / diag : 0.f;
+__global__ void small_coef_kernel(float* __restrict__ coef, double* __restrict__ diag64, LevelArgs a) {
+    const int n = a.nx * a.ny * a.nz;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int y = i % a.ny, x = (i / a.ny) % a.nx, z = i / (a.ny * a.nx);
+    const int iay = y + x * (a.ny + 1) + z * a.nx * (a.ny + 1);
+    const int iax = y + x * a.ny + z * (a.nx + 1) * a.ny;
+    const int sz = a.nx * a.ny;
+    const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
+    const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i], a.az[i + sz]};
+    const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
+                         x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
+                         z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
+    const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
+    float diag = 0.f;
+    double d64 = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        float c = 0.f;
+        if (in[k] || Z) {
+            c = face_cf<float>(fr[k], hn[k], hh[k]);
+            diag += c;
+            d64 += (double)c;
+        }
+        coef[k * n + i] = in[k] ? c : 0.f;
     }
-    __syncthreads();
+    coef[6 * n + i] = diag;
+    if (diag64) diag64[i] = d64;
 }
 
-template <bool Z>
-__device__ void blk_smooth(const LevelDev& L, int nu, float w) {
-    float* a = L.v;
-    float* b = L.t;
-    for (int k = 0; k < nu; ++k) {
-        blk_jacobi<Z>(L, b, a, w);
+// Neighbour indices of cell i (clamped into the level; a missing neighbour
+// has a zero coefficient).
+struct Nb {
+    int k[6];
+};
+__device__ __forceinline__ Nb small_nb(int i, int ny, int sz, int n) {
+    Nb r;
+    r.k[0] = i > 0 ? i - 1 : i;
+    r.k[1] = i + 1 < n ? i + 1 : i;
+    r.k[2] = i - ny >= 0 ? i - ny : i;
+    r.k[3] = i + ny < n ? i + ny : i;
+    r.k[4] = i - sz >= 0 ? i - sz : i;
+    r.k[5] = i + sz < n ? i + sz : i;
+    return r;
+}
+
+// off = -sum_k c_k src[nb_k] in stencil()'s order
+__device__ __forceinline__ float small_off(const SmallLevel& L, const float* src, int i, float& diag) {
+    const Nb nb = small_nb(i, L.ny, L.nx * L.ny, L.n);
+    float off = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) off -= L.coef[k * L.n + i] * src[nb.k[k]];
+    diag = L.coef[6 * L.n + i];
+    return off;
+}
+
+__device__ void small_smooth(const SmallLevel& L, float* sm, int nu, float w) {
+    float* a = sm + L.ov;
+    float* b = sm + L.ot;
+    const float* f = sm + L.of;
+    for (int it = 0; it < nu; ++it) {
+        for (int i = threadIdx.x; i < L.n; i += kSmallThreads) {
+            float diag;
+            const float off = small_off(L, a, i, diag);
+            b[i] = diag > 1e-12f ? (1.f - w) * a[i] + w * (f[i] - off) / diag : 0.f;
+        }
+        __syncthreads();
         float* t = a;
         a = b;
         b = t;
     }
-    if (a != L.v) {
-        const int64_t n = (int64_t)L.a.nx * L.a.ny * L.a.nz;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) L.v[i] = a[i];
+    if (a != sm + L.ov) {
+        for (int i = threadIdx.x; i < L.n; i += kSmallThreads) sm[L.ov + i] = a[i];
         __syncthreads();
     }
 }
 
 template <bool Z>
-__global__ void __launch_bounds__(1024) vcycle_coarse_kernel(const LevelDev* __restrict__ lv,
-                                                             int first, int nlev, int nu1, int nu2,
-                                                             float w, double* __restrict__ cbuf,
-                                                             int csweeps) {
-    const int last = nlev - 1;
-    for (int l = first; l < last; ++l) {
-        const LevelDev L = lv[l], C = lv[l + 1];
-        blk_smooth<Z>(L, nu1, w);
-        const LevelArgs& a = L.a;
-        const int64_t n = (int64_t)a.nx * a.ny * a.nz;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx), z = (int)(i / ((int64_t)a.ny * a.nx));
-            float diag, off;
-            stencil<Z, float>(a, [&](int64_t j) { return L.v[j]; }, x, y, z, i, diag, off);
-            L.t[i] = diag > 1e-12f ? L.f[i] - (diag * L.v[i] + off) : 0.f;
+__global__ void __launch_bounds__(kSmallThreads) vcycle_small_kernel(SmallCycle c, int nu1, int nu2,
+                                                                    float w, int csweeps) {
+    extern __shared__ double small_lds[];
+    float* sm = (float*)small_lds;
+    const int tid = threadIdx.x;
+    const int last = c.nlev - 1;
+    {
+        const SmallLevel& L = c.lv[0];
+        for (int i = tid; i < L.n; i += kSmallThreads) {
+            sm[L.of + i] = c.f0[i];
+            sm[L.ov + i] = 0.f;
         }
         __syncthreads();
-        // restriction of the residual (restrict_kernel) and a zero coarse iterate
-        const int nx = a.nx, ny = a.ny, nz = a.nz;
-        const int cx = C.a.nx, cy = C.a.ny, cz = C.a.nz;
-        const int64_t nc = (int64_t)cx * cy * cz;
-        for (int64_t i = threadIdx.x; i < nc; i += blockDim.x) {
-            const int y = (int)(i % cy), x = (int)((i / cy) % cx), z = (int)(i / ((int64_t)cy * cx));
+    }
+    for (int l = 0; l < last; ++l) {
+        const SmallLevel& L = c.lv[l];
+        const SmallLevel& C = c.lv[l + 1];
+        small_smooth(L, sm, nu1, w);
+        // residual into the partner array
+        for (int i = tid; i < L.n; i += kSmallThreads) {
+            float diag;
+            const float off = small_off(L, sm + L.ov, i, diag);
+            sm[L.ot + i] = diag > 1e-12f ? sm[L.of + i] - (diag * sm[L.ov + i] + off) : 0.f;
+        }
+        __syncthreads();
+        // spacing-weighted restriction (restrict_kernel) and a zero coarse iterate
+        const float* t = sm + L.ot;
+        for (int i = tid; i < C.n; i += kSmallThreads) {
+            const int y = i % C.ny, x = (i / C.ny) % C.nx, z = i / (C.ny * C.nx);
             const int x0 = 2 * x, y0 = 2 * y, z0 = 2 * z;
-            const int x1 = x0 + 1 < nx ? x0 + 1 : -1, y1 = y0 + 1 < ny ? y0 + 1 : -1,
-                      z1 = z0 + 1 < nz ? z0 + 1 : -1;
-            const int64_t sx = ny, sz = (int64_t)nx * ny;
+            const int x1 = x0 + 1 < L.nx ? x0 + 1 : -1, y1 = y0 + 1 < L.ny ? y0 + 1 : -1,
+                      z1 = z0 + 1 < L.nz ? z0 + 1 : -1;
+            const int sx = L.ny, sz = L.nx * L.ny;
             float accz = 0.f, wz = 0.f;
             for (int kz = 0; kz < 2; ++kz) {
                 const int zz = kz ? z1 : z0;
@@ -538,66 +608,79 @@ __global__ void __launch_bounds__(1024) vcycle_coarse_kernel(const LevelDev* __r
                 for (int kx = 0; kx < 2; ++kx) {
                     const int xx = kx ? x1 : x0;
                     if (xx < 0) continue;
-                    const int64_t b = xx * sx + zz * sz;
-                    float accy = a.hy[y0] * L.t[b + y0], wy = a.hy[y0];
+                    const int b = xx * sx + zz * sz;
+                    float accy = L.hy[y0] * t[b + y0], wy = L.hy[y0];
                     if (y1 >= 0) {
-                        accy += a.hy[y1] * L.t[b + y1];
-                        wy += a.hy[y1];
+                        accy += L.hy[y1] * t[b + y1];
+                        wy += L.hy[y1];
                     }
-                    accx += a.hx[xx] * (accy / wy);
-                    wx += a.hx[xx];
+                    accx += L.hx[xx] * (accy / wy);
+                    wx += L.hx[xx];
                 }
-                accz += a.hz[zz] * (accx / wx);
-                wz += a.hz[zz];
+                accz += L.hz[zz] * (accx / wx);
+                wz += L.hz[zz];
             }
-            C.f[i] = accz / wz;
-            C.v[i] = 0.f;
+            sm[C.of + i] = accz / wz;
+            sm[C.ov + i] = 0.f;
         }
         __syncthreads();
     }
     {
-        // coarsest level: FP64 sweeps (coarsest_kernel); small levels in LDS,
-        // larger ones in cbuf
-        const LevelDev L = lv[last];
-        const LevelArgs& a = L.a;
-        const int64_t n = (int64_t)a.nx * a.ny * a.nz;
-        if (n <= 256) {
-            coarsest_small<Z>(L.v, L.f, a, csweeps, (double)w);
-        } else {
-            double* A = cbuf;
-            double* B = cbuf + n;
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) A[i] = L.v[i];
-            __syncthreads();
-            for (int it = 0; it < csweeps; ++it) {
-                for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-                    const int y = (int)(i % a.ny), x = (int)((i / a.ny) % a.nx),
-                              z = (int)(i / ((int64_t)a.ny * a.nx));
-                    double diag, off;
-                    stencil<Z, double>(a, [&](int64_t j) { return A[j]; }, x, y, z, i, diag, off);
-                    B[i] = diag > 1e-12
-                               ? (1.0 - (double)w) * A[i] + (double)w * ((double)L.f[i] - off) / diag
-                               : 0.0;
-                }
-                __syncthreads();
-                double* t = A;
-                A = B;
-                B = t;
+        // coarsest level: FP64 sweeps, one cell per thread, coefficients in registers
+        const SmallLevel& L = c.lv[last];
+        double* S = (double*)(sm + c.o64);  // 2 x L.n doubles
+        const bool one_wave = L.n <= 64;
+        if (!one_wave || tid < 64) {
+            const int i = tid;
+            double cf[6] = {0, 0, 0, 0, 0, 0}, diag = 0.0, fi = 0.0;
+            Nb nb{};
+            if (i < L.n) {
+                nb = small_nb(i, L.ny, L.nx * L.ny, L.n);
+                for (int k = 0; k < 6; ++k) cf[k] = (double)L.coef[k * L.n + i];
+                diag = L.diag64[i];
+                fi = (double)sm[L.of + i];
+                S[i] = (double)sm[L.ov + i];
             }
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) L.v[i] = (float)A[i];
+            if (one_wave) {
+                __builtin_amdgcn_wave_barrier();
+                asm volatile("" ::: "memory");
+            } else {
+                __syncthreads();
+            }
+            int cur = 0;
+            for (int it = 0; it < csweeps; ++it) {
+                if (i < L.n) {
+                    const double* a = S + cur * L.n;
+                    double o = 0.0;
+                    for (int k = 0; k < 6; ++k) o -= cf[k] * a[nb.k[k]];
+                    S[(cur ^ 1) * L.n + i] = diag > 1e-12 ? (1.0 - (double)w) * a[i] + (double)w * (fi - o) / diag
+                                                          : 0.0;
+                }
+                if (one_wave) {
+                    __builtin_amdgcn_wave_barrier();
+                    asm volatile("" ::: "memory");
+                } else {
+                    __syncthreads();
+                }
+                cur ^= 1;
+            }
+            if (i < L.n) sm[L.ov + i] = (float)S[cur * L.n + i];
         }
         __syncthreads();
     }
-    for (int l = last - 1; l >= first; --l) {
-        const LevelDev L = lv[l], C = lv[l + 1];
-        const int64_t n = (int64_t)L.a.nx * L.a.ny * L.a.nz;
-        const int cx = C.a.nx, cy = C.a.ny;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const int y = (int)(i % L.a.ny), x = (int)((i / L.a.ny) % L.a.nx),
-                      z = (int)(i / ((int64_t)L.a.ny * L.a.nx));
-            L.v[i] += C.v[(y >> 1) + (int64_t)(x >> 1) * cy + (int64_t)(z >> 1) * cx * cy];
+    for (int l = last - 1; l >= 0; --l) {
+        const SmallLevel& L = c.lv[l];
+        const SmallLevel& C = c.lv[l + 1];
+        for (int i = tid; i < L.n; i += kSmallThreads) {
+            const int y = i % L.ny, x = (i / L.ny) % L.nx, z = i / (L.ny * L.nx);
+            sm[L.ov + i] += sm[C.ov + (y >> 1) + (x >> 1) * C.ny + (z >> 1) * C.nx * C.ny];
         }
         __syncthreads();
-        blk_smooth<Z>(L, nu2, w);
+        small_smooth(L, sm, nu2, w);
+    }
+    {
+        const SmallLevel& L = c.lv[0];
+        for (int i = tid; i < L.n; i += kSmallThreads) c.v0[i] = sm[L.ov + i];
     }
 }
 
@@ -614,7 +697,8 @@ struct whhd_mgfrac3d_s {
     double* host = nullptr;   // pinned host copy of the totals
     double* cbuf = nullptr;   // FP64 ping-pong of the coarsest level (2 * its cells)
     double* fac64 = nullptr;  // finest level's FP64 face factors: ylo, yhi, xlo, xhi, zlo, zhi
-    whhd::LevelDev* dlev = nullptr;  // device copy of the level table
+    whhd::SmallCycle small{};        // the small levels' table (one-workgroup V-cycle)
+    size_t small_lds = 0;            // its LDS bytes
     int fuse_from = -1;              // levels from here down run in one workgroup
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
@@ -708,16 +792,18 @@ int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
 
 int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s) {
     const Level& L = h->lv[l];
-    if (h->fuse_from >= 0 && (int)l >= h->fuse_from && l + 1 < h->lv.size()) {
+    if (h->fuse_from >= 0 && (int)l == h->fuse_from) {
+        // the small levels start from a zero iterate in LDS (the caller's clear
+        // of this level's v is not needed)
         const int cs = h->bc == 'z' ? 20 : 50;
         if (h->bc == 'z')
-            hipLaunchKernelGGL(vcycle_coarse_kernel<true>, dim3(1), dim3(1024), 0, s, h->dlev, (int)l,
-                               (int)h->lv.size(), nu1, nu2, w, h->cbuf, cs);
+            hipLaunchKernelGGL(vcycle_small_kernel<true>, dim3(1), dim3(kSmallThreads), h->small_lds, s,
+                               h->small, nu1, nu2, w, cs);
         else
-            hipLaunchKernelGGL(vcycle_coarse_kernel<false>, dim3(1), dim3(1024), 0, s, h->dlev, (int)l,
-                               (int)h->lv.size(), nu1, nu2, w, h->cbuf, cs);
-        hipError_t e = launch_status("mg_vcycle_coarse_kernel");
-        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg coarse V-cycle kernel");
+            hipLaunchKernelGGL(vcycle_small_kernel<false>, dim3(1), dim3(kSmallThreads), h->small_lds, s,
+                               h->small, nu1, nu2, w, cs);
+        hipError_t e = launch_status("mg_vcycle_small_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg small-level V-cycle kernel");
     }
     if (l + 1 >= h->lv.size()) return coarsest(h, L, w, s);
     const Level& C = h->lv[l + 1];
@@ -727,7 +813,7 @@ int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStr
                        L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y, C.d.z);
     hipError_t e = launch_status("mg_restrict_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg restrict kernel");
-    TRY(fill(C.v, 0.f, cells(C.d), s));
+    if ((int)l + 1 != h->fuse_from) TRY(fill(C.v, 0.f, cells(C.d), s));
     TRY(vcycle(h, l + 1, nu1, nu2, w, s));
     hipLaunchKernelGGL(prolong_add_kernel, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, L.v,
                        C.v, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y);
@@ -839,18 +925,52 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
             st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
     }
     if (st == WHHD_OK) {
-        // levels of at most kFuseCells cells (not the finest) run fused
+        // levels of at most kFuseCells cells (not the finest) run in one
+        // workgroup, when the coarsest has at most 256 cells and the small
+        // levels fit in LDS
         constexpr int64_t kFuseCells = 4096;
-        std::vector<LevelDev> dl;
-        for (size_t l = 0; l < h->lv.size(); ++l) {
-            const Level& L = h->lv[l];
-            dl.push_back(LevelDev{args_of(h, L), L.f, L.v, L.t});
-            if (h->fuse_from < 0 && l >= 1 && cells(L.d) <= kFuseCells) h->fuse_from = (int)l;
+        const int nl = (int)h->lv.size();
+        int from = -1;
+        for (int l = 1; l < nl; ++l)
+            if (cells(h->lv[l].d) <= kFuseCells) { from = l; break; }
+        if (from >= 0 && from < nl - 1 && nl - from <= kMaxSmall && cells(h->lv[nl - 1].d) <= 256) {
+            SmallCycle& c = h->small;
+            c.nlev = nl - from;
+            int off = 0;
+            for (int k = 0; k < c.nlev && st == WHHD_OK; ++k) {
+                const Level& L = h->lv[from + k];
+                SmallLevel& S = c.lv[k];
+                S.nx = L.d.x; S.ny = L.d.y; S.nz = L.d.z;
+                S.n = (int)cells(L.d);
+                S.hx = L.hx; S.hy = L.hy; S.hz = L.hz;
+                float* coef = (float*)alloc((size_t)7 * S.n * sizeof(float));
+                double* d64 = k == c.nlev - 1 ? (double*)alloc((size_t)S.n * sizeof(double)) : nullptr;
+                if (!coef || (k == c.nlev - 1 && !d64)) {
+                    st = fail(WHHD_OOM, "mgfrac3d_create: small-level tables");
+                    break;
+                }
+                S.coef = coef;
+                S.diag64 = d64;
+                const LevelArgs a = args_of(h, L);
+                if (h->bc == 'z')
+                    hipLaunchKernelGGL(small_coef_kernel<true>, dim3((S.n + 255) / 256), dim3(256), 0, s,
+                                       coef, d64, a);
+                else
+                    hipLaunchKernelGGL(small_coef_kernel<false>, dim3((S.n + 255) / 256), dim3(256), 0, s,
+                                       coef, d64, a);
+                if (hipGetLastError() != hipSuccess) st = fail(WHHD_HIP_ERROR, "mgfrac3d_create: small-level tables");
+                S.ov = off; off += S.n;
+                S.ot = off; off += S.n;
+                S.of = off; off += S.n;
+            }
+            off = (off + 1) & ~1;  // 8-byte alignment for the FP64 ping-pong
+            c.o64 = off;
+            off += 2 * 2 * (int)cells(h->lv[nl - 1].d);
+            c.v0 = h->lv[from].v;
+            c.f0 = h->lv[from].f;
+            h->small_lds = (size_t)off * sizeof(float);
+            if (st == WHHD_OK && h->small_lds <= 60 * 1024) h->fuse_from = from;
         }
-        h->dlev = (LevelDev*)alloc(dl.size() * sizeof(LevelDev));
-        if (!h->dlev ||
-            hipMemcpy(h->dlev, dl.data(), dl.size() * sizeof(LevelDev), hipMemcpyHostToDevice) != hipSuccess)
-            st = fail(WHHD_OOM, "mgfrac3d_create: level table");
     }
     if (st == WHHD_OK) {
         // uniform finest spacing: interior faces 2 / (2 h h), wall faces 2 / ((e + h) h)
