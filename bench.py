@@ -150,13 +150,7 @@ def main():
     ab = array_bytes(N)
     for st in stages:
         st["bytes"] = stage_bytes(st["stage"], ab)
-    # calibration kernel of known size (16 Mi floats read, 16 Mi written) for
-    # scaling rocprof byte counters
-    cal_src = torch.empty(16 * 1024 * 1024, device="cuda")
-    cal_dst = torch.empty_like(cal_src)
-    for _ in range(3):
-        torch.mul(cal_src, 1.0, out=cal_dst)
-    torch.cuda.synchronize()
+    calibrate()
     stage_total_us = sum(st["us"] for st in stages)
     bytes_per_solve = sum(st["bytes"] for st in stages)
     # dominant kernel = the instantiation with the largest summed time
@@ -227,6 +221,17 @@ def main():
     if not args.no_learned:
         out["learned_init"] = learned_summary(args.learned_seeds)
     print(json.dumps(out))
+
+
+def calibrate():
+    """A kernel of known size (16 Mi floats read, 16 Mi written) for scaling the
+    rocprof byte counters of a profiled run."""
+    import torch
+    cal_src = torch.empty(16 * 1024 * 1024, device="cuda")
+    cal_dst = torch.empty_like(cal_src)
+    for _ in range(3):
+        torch.mul(cal_src, 1.0, out=cal_dst)
+    torch.cuda.synchronize()
 
 
 def learned_summary(n_seeds):
@@ -399,6 +404,8 @@ def bench_sweep(args):
         sw.Solve(*q, xv, yv, zv)
         sw.Project(*q)
 
+    calibrate()
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -462,6 +469,8 @@ def bench_solids(args):
     f = P["f"]
     u0 = [P[k].clone() for k in ("xv", "yv", "zv")]
     u = [t.clone() for t in u0]
+
+    calibrate()
 
     def step():
         for a, b in zip(u, u0):

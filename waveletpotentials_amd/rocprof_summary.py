@@ -25,6 +25,7 @@ import statistics
 
 CAL_BYTES = 16 * 1024 * 1024 * 4  # calibration kernel: bytes read and bytes written
 CAL_NAME = "vectorized_elementwise_kernel"
+CAL_OP = "MulFunctor<float>"  # torch.mul(src, 1.0, out=dst) on FP32 tensors
 
 _WT = {"Cdf37": "cdf3.7 n", "Cdf46": "cdf4.6 d"}
 
@@ -61,11 +62,25 @@ def _one(pattern):
     return hits[0]
 
 
-def counters(d):
+def counters(d, by_grid=False):
     out = collections.defaultdict(list)
     with open(_one(os.path.join(d, "*counter_collection.csv"))) as f:
         for r in csv.DictReader(f):
-            out[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)  # KiB -> bytes
+            key = (r["Kernel_Name"], int(r["Grid_Size"])) if by_grid else r["Kernel_Name"]
+            out[key].append(float(r["Counter_Value"]) * 1024.0)  # KiB -> bytes
+    return out
+
+
+def durations_by_grid(trace_dir):
+    """(kernel name, total work-items) -> per-dispatch durations in ns."""
+    out = collections.defaultdict(list)
+    hits = glob.glob(os.path.join(trace_dir, "*kernel_trace.csv"))
+    if not hits:
+        return out
+    with open(hits[0]) as f:
+        for r in csv.DictReader(f):
+            g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+            out[(r["Kernel_Name"], g)].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     return out
 
 
@@ -75,7 +90,7 @@ def summarize(trace_dir, fetch_dir, write_dir):
         for r in csv.DictReader(f):
             stats[r["Name"]] = r
     fetch, write = counters(fetch_dir), counters(write_dir)
-    cal = [k for k in fetch if CAL_NAME in k]
+    cal = [k for k in fetch if CAL_NAME in k and CAL_OP in k]
     if not cal:
         raise RuntimeError("calibration kernel not found in the counter runs")
     f_scale = CAL_BYTES / statistics.median(fetch[cal[0]])
@@ -83,7 +98,7 @@ def summarize(trace_dir, fetch_dir, write_dir):
     kernels = {}
     for name, r in stats.items():
         label = kernel_label(name)
-        row = kernels.setdefault(label, {"calls": 0, "total_ns": 0.0})
+        row = kernels.setdefault(label, {"calls": 0, "total_ns": 0.0, "library": "whhd::" in name})
         row["calls"] += int(r["Calls"])
         row["total_ns"] += float(r["TotalDurationNs"])
         if name in fetch:
@@ -92,13 +107,30 @@ def summarize(trace_dir, fetch_dir, write_dir):
             row["write_bytes"] = statistics.median(write[name]) * w_scale
     for row in kernels.values():
         row["avg_us"] = row["total_ns"] / row["calls"] / 1000.0
-    return kernels, f_scale, w_scale
+    # library kernels launched at several sizes (one per multigrid level, say):
+    # one row per launch size
+    durs = durations_by_grid(trace_dir)
+    fg, wg = counters(fetch_dir, True), counters(write_dir, True)
+    sizes = collections.defaultdict(set)
+    for (name, g) in durs:
+        sizes[name].add(g)
+    by_size = []
+    for (name, g), d in durs.items():
+        if "whhd::" not in name or len(sizes[name]) < 2:
+            continue
+        row = {"kernel": kernel_label(name), "grid": g, "calls": len(d),
+               "avg_us": statistics.mean(d) / 1000.0}
+        if (name, g) in fg:
+            row["fetch_bytes"] = statistics.median(fg[(name, g)]) * f_scale
+        if (name, g) in wg:
+            row["write_bytes"] = statistics.median(wg[(name, g)]) * w_scale
+        by_size.append(row)
+    by_size.sort(key=lambda r: (r["kernel"], -r["grid"]))
+    return kernels, f_scale, w_scale, by_size
 
 
-def write_outputs(kernels, f_scale, w_scale, prefix, command):
-    lib = {k: v for k, v in kernels.items()
-           if k.startswith(("lp_kernel", "project3d", "line_tile", "mg_", "frac", "levelset",
-                            "eikonal", "cut_kernel"))}
+def write_outputs(kernels, f_scale, w_scale, prefix, command, by_size=()):
+    lib = {k: v for k, v in kernels.items() if v.get("library")}
     with open(prefix + ".json", "w") as f:
         json.dump({"command": command, "fetch_scale": f_scale, "write_scale": w_scale,
                    "kernels": {k: {"avg_us": round(v["avg_us"], 3), "calls": v["calls"],
@@ -108,11 +140,25 @@ def write_outputs(kernels, f_scale, w_scale, prefix, command):
     lines = [f"# rocprofv3 summary: {os.path.basename(prefix)}", "",
              "Command (three runs; byte counters scaled by the calibration kernel, "
              f"FETCH_SIZE x{f_scale:.3f}, WRITE_SIZE x{w_scale:.3f}):", "", "```", command, "```",
-             "", "| kernel | calls | avg us | HBM fetch MB/launch | HBM write MB/launch |",
-             "|---|---:|---:|---:|---:|"]
+             "", "HBM GB/s = (fetch + write bytes) / average duration; peak 8000 GB/s.", "",
+             "| kernel | calls | avg us | HBM fetch MB/launch | HBM write MB/launch | HBM GB/s | % of peak |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
     for k, v in sorted(lib.items(), key=lambda kv: -kv[1]["total_ns"]):
+        b = v.get("fetch_bytes", 0) + v.get("write_bytes", 0)
+        gbs = b / (v["avg_us"] * 1e-6) / 1e9 if v["avg_us"] > 0 else 0.0
         lines.append(f"| `{k}` | {v['calls']} | {v['avg_us']:.2f} | "
-                     f"{v.get('fetch_bytes', 0) / 1e6:.1f} | {v.get('write_bytes', 0) / 1e6:.1f} |")
+                     f"{v.get('fetch_bytes', 0) / 1e6:.1f} | {v.get('write_bytes', 0) / 1e6:.1f} | "
+                     f"{gbs:.0f} | {100 * gbs / 8000:.1f}% |")
+    if by_size:
+        lines += ["", "Kernels launched at several sizes, one row per launch size (work-items):", "",
+                  "| kernel | work-items | calls | avg us | HBM fetch MB/launch | HBM write MB/launch | HBM GB/s | % of peak |",
+                  "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        for v in by_size:
+            b = v.get("fetch_bytes", 0) + v.get("write_bytes", 0)
+            gbs = b / (v["avg_us"] * 1e-6) / 1e9 if v["avg_us"] > 0 else 0.0
+            lines.append(f"| `{v['kernel']}` | {v['grid']} | {v['calls']} | {v['avg_us']:.2f} | "
+                         f"{v.get('fetch_bytes', 0) / 1e6:.1f} | {v.get('write_bytes', 0) / 1e6:.1f} | "
+                         f"{gbs:.0f} | {100 * gbs / 8000:.1f}% |")
     with open(prefix + ".md", "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -148,8 +194,8 @@ def main():
         write_trace_only(a.dirs[0], a.dirs[1], a.command)
         return
     trace_dir, fetch_dir, write_dir, out_prefix = a.dirs
-    kernels, fs, ws = summarize(trace_dir, fetch_dir, write_dir)
-    write_outputs(kernels, fs, ws, out_prefix, a.command)
+    kernels, fs, ws, by_size = summarize(trace_dir, fetch_dir, write_dir)
+    write_outputs(kernels, fs, ws, out_prefix, a.command, by_size)
 
 
 if __name__ == "__main__":
