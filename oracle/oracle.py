@@ -189,6 +189,40 @@ def recover3d(xv, yv, zv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6", with
     return (qx, qy, qz, xo, yo, zo) if with_vout else (qx, qy, qz)
 
 
+_lib32 = None
+_F = ctypes.POINTER(ctypes.c_float)
+
+
+def lib_f32():
+    """liboracle_f32.so: the FP32 build of the recovery path (orc_fp32.c), for timing."""
+    global _lib32
+    if _lib32 is None:
+        path = os.path.join(_HERE, "liboracle_f32.so")
+        if not os.path.exists(path):
+            build()
+        _lib32 = ctypes.CDLL(path)
+        i, c, f = ctypes.c_int, ctypes.c_char, ctypes.c_float
+        _lib32.orc_recover3d.argtypes = [_F] * 9 + [i, i, i, f, f, f, c, i, i]
+    return _lib32
+
+
+def recover3d_f32(xv, yv, zv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6"):
+    """recover3d computed in FP32 by liboracle_f32.so; float32 inputs and outputs."""
+    nx, ny, nz = n
+
+    def pf(a):
+        assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(_F)
+
+    q = [np.zeros(s, dtype=np.float32) for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny),
+                                                  (nz, nx + 1, ny + 1))]
+    ins = [np.ascontiguousarray(a, dtype=np.float32) for a in (xv, yv, zv)]
+    _check(lib_f32().orc_recover3d(pf(q[0]), pf(q[1]), pf(q[2]), None, None, None,
+                                   *(pf(a) for a in ins), nx, ny, nz, dx[0], dx[1], dx[2],
+                                   _b(bc), _wt(wt_curl), _wt(wt_div)), "recover3d (FP32)")
+    return tuple(q)
+
+
 def recover2d(xv, yv, n, dx, bc="n", wt_curl="cdf3.7", wt_div="cdf4.6"):
     nx, ny = n
     qz = field(nx + 1, ny + 1)

@@ -33,6 +33,19 @@ def test_recover3d_anisotropic(orc):
     assert orc.rel_l2([xv, yv, zv], list(c)) < 1e-11
 
 
+@pytest.mark.parametrize("n,dx", [((32, 32, 32), (1 / 32,) * 3), ((16, 8, 32), (1 / 16, 1 / 8, 1 / 32))])
+def test_recover3d_fp32_build_matches_fp64(orc, n, dx):
+    """The FP32 build of the oracle (the like-for-like CPU timing baseline) runs
+    the same algorithm: its potentials equal the FP64 oracle's to FP32 roundoff,
+    and a planted error would show far above that."""
+    q = I.spectral_potential3d(n, 3, 7)
+    xv, yv, zv = I.curl3d(*q, dx)
+    r64 = orc.recover3d(xv, yv, zv, n, dx)
+    r32 = orc.recover3d_f32(xv, yv, zv, n, dx)
+    assert all(a.dtype == np.float32 and a.shape == b.shape for a, b in zip(r32, r64))
+    assert orc.rel_l2(list(r64), [a.astype(np.float64) for a in r32]) < 5e-6
+
+
 def test_recover3d_vout_is_input_for_divfree(orc):
     N = 16
     n = (N, N, N)

@@ -135,3 +135,72 @@ def test_sweep2d(orc):
         err = max(np.abs(a - b).max() for a, b in zip(c, vel))
         scale = max(np.abs(b).max() for b in vel)
         assert (err < 1e-10 * scale) == exact, (bc, err)
+
+
+def _project_zero_np(q, n, h, ref_eigen=False, ref_edges=False):
+    """numpy ProjectZero with two of the reference's choices switchable:
+    ref_eigen divides the transformed right-hand side by the eigenvalue sum
+    times hx*hy (cuDstProj_dx's scale, WaveletPotentials/cuDst3D.cu:194-197, 322-323)
+    instead of by the sum of the per-axis eigenvalues 1/h_a^2 (2 - 2 cos);
+    ref_edges divides the first boundary edge of qy by hx and of qx by hy
+    (cuGradientMinus3D_zero_Q, WaveletPotentials/cuGradient.cu:2382-2385)."""
+    from scipy.fft import dstn, idstn
+    nx, ny, nz = n
+    hx, hy, hz = h
+    qx, qy, qz = (a.copy() for a in q)
+    b = -((qx[1:-1, 1:, 1:-1] - qx[1:-1, :-1, 1:-1]) / hx
+          + (qy[1:-1, 1:-1, 1:] - qy[1:-1, 1:-1, :-1]) / hy
+          + (qz[1:, 1:-1, 1:-1] - qz[:-1, 1:-1, 1:-1]) / hz)
+    lam = [2 - 2 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) for m in (nz - 1, nx - 1, ny - 1)]
+    if ref_eigen:
+        den = (lam[0][:, None, None] + lam[1][None, :, None] + lam[2][None, None, :]) / (hx * hy)
+    else:
+        den = lam[0][:, None, None] / hz ** 2 + lam[1][None, :, None] / hx ** 2 + lam[2][None, None, :] / hy ** 2
+    phi = np.pad(idstn(dstn(b, type=1, norm="ortho") / den, type=1, norm="ortho"), 1)
+    qx -= (phi[:, 1:, :] - phi[:, :-1, :]) / hx
+    qy -= (phi[:, :, 1:] - phi[:, :, :-1]) / hy
+    qz -= (phi[1:, :, :] - phi[:-1, :, :]) / hz
+    if ref_edges:
+        qx[:, 0, :] += phi[:, 1, :] / hx - phi[:, 1, :] / hy
+        qy[:, :, 0] += phi[:, :, 1] / hy - phi[:, :, 1] / hx
+    return qx, qy, qz
+
+
+def _interior_div(q, h):
+    qx, qy, qz = q
+    return ((qx[1:-1, 1:, 1:-1] - qx[1:-1, :-1, 1:-1]) / h[0]
+            + (qy[1:-1, 1:-1, 1:] - qy[1:-1, 1:-1, :-1]) / h[1]
+            + (qz[1:, 1:-1, 1:-1] - qz[:-1, 1:-1, 1:-1]) / h[2])
+
+
+def test_reference_dst_choices_on_anisotropic_cells(orc):
+    """Size of the two anisotropic-cell differences from the reference that the
+    library and the oracle keep (per-axis eigenvalues; each boundary edge divided
+    by its own axis spacing).  The numpy restatement with both switches off is
+    the oracle's ProjectZero; with cubic cells the reference's choices give the
+    same result; on a (1/10, 1/24, 1/18) cell the reference's two choices move q
+    by 30% (rel L2; the eigenvalue scaling alone 28%, the edge spacing alone 10%)
+    and leave a divergence of 0.96 max|q| / h_min at the interior nodes, where the
+    kept choices leave none."""
+    n = (10, 12, 9)
+    nx, ny, nz = n
+    rng = np.random.default_rng(11)
+    q = [rng.standard_normal(s) for s in ((nz + 1, nx, ny + 1), (nz + 1, nx + 1, ny),
+                                          (nz, nx + 1, ny + 1))]
+    scale = max(np.abs(a).max() for a in q)
+    cubic = (0.05, 0.05, 0.05)
+    assert orc.rel_l2(list(orc.project_zero(*q, n, cubic)), list(_project_zero_np(q, n, cubic))) < 1e-12
+    assert orc.rel_l2(list(_project_zero_np(q, n, cubic)),
+                      list(_project_zero_np(q, n, cubic, True, True))) < 1e-12
+    h = (1 / 10, 1 / 24, 1 / 18)
+    ours = _project_zero_np(q, n, h)
+    assert orc.rel_l2(list(orc.project_zero(*q, n, h)), list(ours)) < 1e-12
+    assert np.abs(_interior_div(ours, h)).max() < 1e-9 * scale / min(h)
+    both = _project_zero_np(q, n, h, True, True)
+    eig = _project_zero_np(q, n, h, ref_eigen=True)
+    edges = _project_zero_np(q, n, h, ref_edges=True)
+    d_both, d_eig, d_edges = (orc.rel_l2(list(ours), list(v)) for v in (both, eig, edges))
+    div_both = np.abs(_interior_div(both, h)).max() / (scale / min(h))
+    print(f"rel L2 vs kept: both {d_both:.3f}, eigenvalues {d_eig:.3f}, edges {d_edges:.3f}; "
+          f"max |div| / (max|q| / h_min) with both: {div_both:.3f}")
+    assert 0.2 < d_eig < 0.4 and 0.05 < d_edges < 0.15 and div_both > 0.3
