@@ -93,8 +93,9 @@ __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int 
 }
 
 // One weighted Jacobi sweep dst <- src (cuWjFrac3D_n / _z); cells whose
-// faces are all closed get 0.
-template <bool Z>
+// faces are all closed get 0.  ZERO: src is zero (the first sweep of a V-cycle
+// on a correction) and is not read; the arithmetic is the same with zeros.
+template <bool Z, bool ZERO = false>
 __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__ src,
                               const float* __restrict__ f, LevelArgs a, float w) {
     const int y = blockIdx.x * 64 + threadIdx.x;
@@ -103,8 +104,9 @@ __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
-    dst[i] = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
+    stencil<Z, float>(a, [&](int64_t j) { return ZERO ? 0.f : src[j]; }, x, y, z, i, diag, off);
+    const float v = ZERO ? 0.f : src[i];
+    dst[i] = diag > 1e-12f ? (1.f - w) * v + w * (f[i] - off) / diag : 0.f;
 }
 
 // Coarsest-level sweeps in FP64 for at most 256 cells: one cell per thread,
@@ -284,6 +286,28 @@ __global__ void pair_add_kernel(float* __restrict__ hi, float* __restrict__ lo,
     lo[i] = (a - (s - bb)) + (t - bb);
 }
 
+// The last sweep of a refinement cycle fused with the pair update: the
+// correction e = J(src) of each cell (jacobi_kernel's arithmetic) goes
+// straight into hi + lo (pair_add_kernel's TwoSum) and is not stored.
+template <bool Z>
+__global__ void jacobi_pair_kernel(float* __restrict__ hi, float* __restrict__ lo,
+                                   const float* __restrict__ src, const float* __restrict__ f, LevelArgs a,
+                                   float w) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (y >= a.ny || x >= a.nx) return;
+    const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+    float diag, off;
+    stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
+    const float e = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
+    const float hv = hi[i], t = lo[i] + e;
+    const float s = hv + t;
+    const float bb = s - hv;
+    hi[i] = s;
+    lo[i] = (hv - (s - bb)) + (t - bb);
+}
+
 // Residual f - A v (cuWjGetFracRes_3d_n / _z).
 template <bool Z>
 __global__ void residual_kernel(float* __restrict__ r, const float* __restrict__ v,
@@ -298,42 +322,58 @@ __global__ void residual_kernel(float* __restrict__ r, const float* __restrict__
     r[i] = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
 }
 
-// SolveP1RN with spacings (cuRestrict3D.cu:939-970): the coarse value is the
-// spacing-weighted average of the up-to-2x2x2 children, axis by axis.
-__global__ void restrict_kernel(float* __restrict__ dst, const float* __restrict__ src,
-                                const float* __restrict__ hx, const float* __restrict__ hy,
-                                const float* __restrict__ hz, int nx, int ny, int nz, int cx,
-                                int cy, int cz) {
-    const int y = blockIdx.x * 64 + threadIdx.x;
-    const int x = blockIdx.y * 4 + threadIdx.y;
-    const int z = blockIdx.z;
-    if (y >= cy || x >= cx) return;
-    const int x0 = 2 * x, y0 = 2 * y, z0 = 2 * z;
-    const int x1 = x0 + 1 < nx ? x0 + 1 : -1, y1 = y0 + 1 < ny ? y0 + 1 : -1,
-              z1 = z0 + 1 < nz ? z0 + 1 : -1;
-    const int64_t sx = ny, sz = (int64_t)nx * ny;
+// The residual (residual_kernel) and its restriction to the coarse rhs,
+// SolveP1RN with spacings (cuRestrict3D.cu:939-970: the spacing-weighted
+// average of the up-to-2x2x2 children, axis by axis, y then x then z), in one
+// pass, so the V-cycle's residual never goes to memory.  A (64, 4, 2)
+// workgroup forms the residuals of its fine cells (one per thread, as
+// residual_kernel) in LDS; 64 of its threads then restrict them into the
+// 32 x 2 coarse cells they cover.  Same FP32 values and order as the two
+// separate kernels.
+template <bool Z>
+__global__ void __launch_bounds__(512) residual_restrict_kernel(float* __restrict__ dst, const float* __restrict__ v,
+                                                                const float* __restrict__ f, LevelArgs a,
+                                                                int cx, int cy) {
+    __shared__ float rs[2][4][64];
+    const int ty = threadIdx.x, tx = threadIdx.y, tz = threadIdx.z;
+    const int y = blockIdx.x * 64 + ty, x = blockIdx.y * 4 + tx, z = blockIdx.z * 2 + tz;
+    float r = 0.f;
+    if (y < a.ny && x < a.nx && z < a.nz) {
+        const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+        float diag, off;
+        stencil<Z, float>(a, [&](int64_t j) { return v[j]; }, x, y, z, i, diag, off);
+        r = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
+    }
+    rs[tz][tx][ty] = r;
+    __syncthreads();
+    const int t = ty + 64 * (tx + 4 * tz);
+    if (t >= 64) return;
+    const int cyl = t & 31, cxl = t >> 5;  // coarse cell of this block
+    const int yc = blockIdx.x * 32 + cyl, xc = blockIdx.y * 2 + cxl, zc = blockIdx.z;
+    if (yc >= cy || xc >= cx) return;
+    const int y0 = 2 * yc, x0 = 2 * xc, z0 = 2 * zc;
+    const bool hy1 = y0 + 1 < a.ny, hx1 = x0 + 1 < a.nx, hz1 = z0 + 1 < a.nz;
     float accz = 0.f, wz = 0.f;
     for (int kz = 0; kz < 2; ++kz) {
-        const int zz = kz ? z1 : z0;
-        if (zz < 0) continue;
+        if (kz && !hz1) continue;
         float accx = 0.f, wx = 0.f;
         for (int kx = 0; kx < 2; ++kx) {
-            const int xx = kx ? x1 : x0;
-            if (xx < 0) continue;
-            const int64_t b = xx * sx + zz * sz;
-            float accy = hy[y0] * src[b + y0], wy = hy[y0];
-            if (y1 >= 0) {
-                accy += hy[y1] * src[b + y1];
-                wy += hy[y1];
+            if (kx && !hx1) continue;
+            const float* row = rs[kz][2 * cxl + kx];
+            float accy = a.hy[y0] * row[2 * cyl], wy = a.hy[y0];
+            if (hy1) {
+                accy += a.hy[y0 + 1] * row[2 * cyl + 1];
+                wy += a.hy[y0 + 1];
             }
-            accx += hx[xx] * (accy / wy);
-            wx += hx[xx];
+            accx += a.hx[x0 + kx] * (accy / wy);
+            wx += a.hx[x0 + kx];
         }
-        accz += hz[zz] * (accx / wx);
-        wz += hz[zz];
+        accz += a.hz[z0 + kz] * (accx / wx);
+        wz += a.hz[z0 + kz];
     }
-    dst[y + (int64_t)x * cy + (int64_t)z * cx * cy] = accz / wz;
+    dst[yc + (int64_t)xc * cy + (int64_t)zc * cx * cy] = accz / wz;
 }
+
 
 // SolveAddP1RN (cuProlongate3D.cu:594-616): every fine cell adds its parent.
 __global__ void prolong_add_kernel(float* __restrict__ v, const float* __restrict__ vc, int nx,
@@ -592,7 +632,7 @@ __global__ void __launch_bounds__(kSmallThreads) vcycle_small_kernel(SmallCycle 
             sm[L.ot + i] = diag > 1e-12f ? sm[L.of + i] - (diag * sm[L.ov + i] + off) : 0.f;
         }
         __syncthreads();
-        // spacing-weighted restriction (restrict_kernel) and a zero coarse iterate
+        // spacing-weighted restriction (residual_restrict_kernel) and a zero coarse iterate
         const float* t = sm + L.ot;
         for (int i = tid; i < C.n; i += kSmallThreads) {
             const int y = i % C.ny, x = (i / C.ny) % C.nx, z = i / (C.ny * C.nx);
@@ -718,15 +758,18 @@ hipError_t launch_status(const char* name) {
     return hipGetLastError();
 }
 
+// One sweep dst <- J(src); zero: src is zero and is not read.
 int jacobi(const whhd_mgfrac3d_s* h, const Level& L, float* dst, const float* src, const float* f,
-           float w, hipStream_t s) {
+           float w, hipStream_t s, bool zero = false) {
     const LevelArgs a = args_of(h, L);
-    if (h->bc == 'z')
-        hipLaunchKernelGGL(jacobi_kernel<true>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, dst,
-                           src, f, a, w);
-    else
-        hipLaunchKernelGGL(jacobi_kernel<false>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
-                           dst, src, f, a, w);
+    const dim3 g = grid_of(L.d.x, L.d.y, L.d.z), b(64, 4);
+    if (h->bc == 'z') {
+        if (zero) hipLaunchKernelGGL((jacobi_kernel<true, true>), g, b, 0, s, dst, src, f, a, w);
+        else hipLaunchKernelGGL((jacobi_kernel<true, false>), g, b, 0, s, dst, src, f, a, w);
+    } else {
+        if (zero) hipLaunchKernelGGL((jacobi_kernel<false, true>), g, b, 0, s, dst, src, f, a, w);
+        else hipLaunchKernelGGL((jacobi_kernel<false, false>), g, b, 0, s, dst, src, f, a, w);
+    }
     hipError_t e = launch_status("mg_jacobi_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi kernel");
 }
@@ -761,12 +804,28 @@ int64_t cells(const Dim3& d) { return (int64_t)d.x * d.y * d.z; }
 #define TRY(e) do { int s_ = (e); if (s_ != WHHD_OK) return s_; } while (0)
 
 // n Jacobi sweeps on level L (SolveFrac, cuWeightedJacobi3D.cu:1075-1105); the
-// result stays in L.v.
-int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t s) {
+// result stays in L.v.  zero: L.v holds zero on entry, so the first sweep does
+// not read it (and with no sweep it is written).
+// hi, lo (n > 0, zero false): the last sweep adds its result to the pair
+// hi + lo instead of storing it (L.v is then left as it was).
+int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t s, bool zero,
+           float* hi = nullptr, float* lo = nullptr) {
     float* a = L.v;
     float* b = L.t;
+    if (zero && n == 0) return fill(L.v, 0.f, cells(L.d), s);
     for (int k = 0; k < n; ++k) {
-        TRY(jacobi(h, L, b, a, L.f, w, s));
+        if (hi && k == n - 1) {
+            const LevelArgs la = args_of(h, L);
+            if (h->bc == 'z')
+                hipLaunchKernelGGL(jacobi_pair_kernel<true>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
+                                   hi, lo, a, L.f, la, w);
+            else
+                hipLaunchKernelGGL(jacobi_pair_kernel<false>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0,
+                                   s, hi, lo, a, L.f, la, w);
+            hipError_t e = launch_status("mg_jacobi_pair_kernel");
+            return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi pair kernel");
+        }
+        TRY(jacobi(h, L, b, a, L.f, w, s, zero && k == 0));
         std::swap(a, b);
     }
     if (a != L.v) TRY(copy(L.v, a, cells(L.d), s));
@@ -790,11 +849,32 @@ int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg coarsest kernel");
 }
 
-int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s) {
+// One V-cycle from level l down.  zero: the level's iterate L.v is zero on
+// entry (every coarse level's correction; the refinement's error) and is not
+// read or cleared first.
+int vcycle_body(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s, bool zero,
+                float* hi, float* lo);
+
+// hi, lo: the cycle's result is added to the pair hi + lo (the refinement),
+// fused into the last post-smoothing sweep when there is one.
+int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s, bool zero,
+           float* hi = nullptr, float* lo = nullptr) {
+    const Level& L = h->lv[l];
+    const bool fused = hi && nu2 > 0 && l + 1 < h->lv.size() && !(h->fuse_from >= 0 && (int)l == h->fuse_from);
+    TRY(vcycle_body(h, l, nu1, nu2, w, s, zero, fused ? hi : nullptr, fused ? lo : nullptr));
+    if (!hi || fused) return WHHD_OK;
+    const int64_t n = cells(L.d);
+    hipLaunchKernelGGL(pair_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, hi, lo, L.v, n);
+    hipError_t e = launch_status("mg_pair_add_kernel");
+    return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg pair update kernel");
+}
+
+int vcycle_body(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStream_t s, bool zero,
+                float* hi, float* lo) {
     const Level& L = h->lv[l];
     if (h->fuse_from >= 0 && (int)l == h->fuse_from) {
-        // the small levels start from a zero iterate in LDS (the caller's clear
-        // of this level's v is not needed)
+        // the small levels start from a zero iterate in LDS (coarse levels are
+        // entered with zero)
         const int cs = h->bc == 'z' ? 20 : 50;
         if (h->bc == 'z')
             hipLaunchKernelGGL(vcycle_small_kernel<true>, dim3(1), dim3(kSmallThreads), h->small_lds, s,
@@ -805,21 +885,27 @@ int vcycle(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, hipStr
         hipError_t e = launch_status("mg_vcycle_small_kernel");
         return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg small-level V-cycle kernel");
     }
-    if (l + 1 >= h->lv.size()) return coarsest(h, L, w, s);
+    if (l + 1 >= h->lv.size()) {
+        if (zero) TRY(fill(L.v, 0.f, cells(L.d), s));
+        return coarsest(h, L, w, s);
+    }
     const Level& C = h->lv[l + 1];
-    TRY(smooth(h, L, nu1, w, s));
-    TRY(residual(h, L, h->r, L.v, L.f, s));
-    hipLaunchKernelGGL(restrict_kernel, grid_of(C.d.x, C.d.y, C.d.z), dim3(64, 4), 0, s, C.f, h->r,
-                       L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y, C.d.z);
-    hipError_t e = launch_status("mg_restrict_kernel");
-    if (e != hipSuccess) return hip_fail(e, "mg restrict kernel");
-    if ((int)l + 1 != h->fuse_from) TRY(fill(C.v, 0.f, cells(C.d), s));
-    TRY(vcycle(h, l + 1, nu1, nu2, w, s));
+    TRY(smooth(h, L, nu1, w, s, zero));
+    // residual and its restriction to the coarse rhs in one pass
+    const LevelArgs a = args_of(h, L);
+    const dim3 rg((L.d.y + 63) / 64, (L.d.x + 3) / 4, (L.d.z + 1) / 2), rb(64, 4, 2);
+    if (h->bc == 'z')
+        hipLaunchKernelGGL(residual_restrict_kernel<true>, rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+    else
+        hipLaunchKernelGGL(residual_restrict_kernel<false>, rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+    hipError_t e = launch_status("mg_residual_restrict_kernel");
+    if (e != hipSuccess) return hip_fail(e, "mg residual-restrict kernel");
+    TRY(vcycle(h, l + 1, nu1, nu2, w, s, true));
     hipLaunchKernelGGL(prolong_add_kernel, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, L.v,
                        C.v, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y);
     e = launch_status("mg_prolong_add_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg prolong kernel");
-    return smooth(h, L, nu2, w, s);
+    return smooth(h, L, nu2, w, s, false, hi, lo);
 }
 
 void release(whhd_mgfrac3d_s* h) {
@@ -1017,7 +1103,7 @@ int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const flo
     if (pre_v) TRY(copy(L0.v, pre_v, n, s));
     else TRY(fill(L0.v, 0.f, n, s));
     TRY(copy(L0.f, f, n, s));
-    for (int i = 0; i < n_iters; ++i) TRY(vcycle(h, 0, nu1, nu2, w, s));
+    for (int i = 0; i < n_iters; ++i) TRY(vcycle(h, 0, nu1, nu2, w, s, false));
     return copy(dst, L0.v, n, s);
 }
 
@@ -1062,12 +1148,7 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         if (e != hipSuccess) return hip_fail(e, "mgfrac3d_solve_tol: read-back");
         res = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
         if (res <= tol || c >= max_cycles) break;
-        TRY(fill(L0.v, 0.f, n, s));
-        TRY(vcycle(h, 0, nu1, nu2, w, s));
-        hipLaunchKernelGGL(pair_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst,
-                           lo, L0.v, n);
-        e = launch_status("mg_pair_add_kernel");
-        if (e != hipSuccess) return hip_fail(e, "mg pair update kernel");
+        TRY(vcycle(h, 0, nu1, nu2, w, s, true, dst, lo));
     }
     if (cycles) *cycles = c;
     if (rel) *rel = res;
