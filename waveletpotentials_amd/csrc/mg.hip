@@ -28,10 +28,26 @@ int half_up(int n) { return (n & 1) ? (n >> 1) + 1 : n >> 1; }
 
 dim3 grid_of(int nx, int ny, int nz) { return dim3((ny + 63) / 64, (nx + 3) / 4, nz); }
 
-// Coefficient of one face, cuWjFrac3D_n / _z: A * 2 / ((h_nbr + h) * h).
-template <class T>
-__device__ __forceinline__ T face_cf(T a, T hn, T h) {
-    return a / (hn + h) / h * T(2);
+// Coefficient of one face, cuWjFrac3D_n / _z: A * 2 / ((h_nbr + h) * h).  The
+// spacing factor 2 / ((h_nbr + h) h) of every face position is tabulated per
+// level (face_factor_kernel, FP64 then rounded), so a coefficient is one
+// multiply: the smoothing kernels were bound by the twelve IEEE divisions a
+// cell's faces cost otherwise.  Table layout per level: y- (ny), y+ (ny),
+// x- (nx), x+ (nx), z- (nz), z+ (nz).
+__global__ void face_factor_kernel(float* __restrict__ kf, const float* __restrict__ hx,
+                                   const float* __restrict__ hy, const float* __restrict__ hz, int nx,
+                                   int ny, int nz, float ex, float ey, float ez) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * (nx + ny + nz)) return;
+    const float* hs;
+    int n, j, up;
+    float e;
+    if (i < 2 * ny) { hs = hy; n = ny; j = i % ny; up = i >= ny; e = ey; }
+    else if (i < 2 * (ny + nx)) { hs = hx; n = nx; j = (i - 2 * ny) % nx; up = i >= 2 * ny + nx; e = ex; }
+    else { hs = hz; n = nz; j = (i - 2 * (ny + nx)) % nz; up = i >= 2 * (ny + nx) + nz; e = ez; }
+    const double h = hs[j];
+    const double hn = up ? (j < n - 1 ? hs[j + 1] : e) : (j > 0 ? hs[j - 1] : e);
+    kf[i] = (float)(2.0 / ((hn + h) * h));
 }
 
 struct Level {
@@ -39,12 +55,26 @@ struct Level {
     float *f = nullptr, *v = nullptr, *t = nullptr;  // rhs, iterate, Jacobi partner
     float *ax = nullptr, *ay = nullptr, *az = nullptr;
     float *hx = nullptr, *hy = nullptr, *hz = nullptr;
+    float* kf = nullptr;  // face spacing factors (face_factor_kernel)
 };
 
 struct LevelArgs {
     const float *ax, *ay, *az, *hx, *hy, *hz;
     int nx, ny, nz;
     float ex, ey, ez;  // exterior spacing at the walls ('z')
+    const float* kf;   // face spacing factors
+};
+
+// Spacing factors of the six faces of cell (x, y, z), in stencil()'s order.
+struct FaceK {
+    const float* kf;
+    int nx, ny, nz;
+    __device__ float ym(int y) const { return kf[y]; }
+    __device__ float yp(int y) const { return kf[ny + y]; }
+    __device__ float xm(int x) const { return kf[2 * ny + x]; }
+    __device__ float xp(int x) const { return kf[2 * ny + nx + x]; }
+    __device__ float zm(int z) const { return kf[2 * (ny + nx) + z]; }
+    __device__ float zp(int z) const { return kf[2 * (ny + nx) + nz + z]; }
 };
 
 // Diagonal and off-diagonal sum of cell (x, y, z) applied to the iterate,
@@ -58,35 +88,36 @@ __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int 
     diag = T(0);
     off = T(0);
     // y faces: ay index (y, x, z) with extents (nx, ny+1, nz)
+    const FaceK k{a.kf, a.nx, a.ny, a.nz};
     const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
     if (y > 0 || Z) {
-        const T cf = face_cf<float>(a.ay[iay], y > 0 ? a.hy[y - 1] : a.ey, a.hy[y]);
+        const T cf = a.ay[iay] * k.ym(y);
         diag += cf;
         if (y > 0) off -= cf * src(i - 1);
     }
     if (y < a.ny - 1 || Z) {
-        const T cf = face_cf<float>(a.ay[iay + 1], y < a.ny - 1 ? a.hy[y + 1] : a.ey, a.hy[y]);
+        const T cf = a.ay[iay + 1] * k.yp(y);
         diag += cf;
         if (y < a.ny - 1) off -= cf * src(i + 1);
     }
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
     if (x > 0 || Z) {
-        const T cf = face_cf<float>(a.ax[iax], x > 0 ? a.hx[x - 1] : a.ex, a.hx[x]);
+        const T cf = a.ax[iax] * k.xm(x);
         diag += cf;
         if (x > 0) off -= cf * src(i - sx);
     }
     if (x < a.nx - 1 || Z) {
-        const T cf = face_cf<float>(a.ax[iax + a.ny], x < a.nx - 1 ? a.hx[x + 1] : a.ex, a.hx[x]);
+        const T cf = a.ax[iax + a.ny] * k.xp(x);
         diag += cf;
         if (x < a.nx - 1) off -= cf * src(i + sx);
     }
     if (z > 0 || Z) {
-        const T cf = face_cf<float>(a.az[i], z > 0 ? a.hz[z - 1] : a.ez, a.hz[z]);
+        const T cf = a.az[i] * k.zm(z);
         diag += cf;
         if (z > 0) off -= cf * src(i - sz);
     }
     if (z < a.nz - 1 || Z) {
-        const T cf = face_cf<float>(a.az[i + sz], z < a.nz - 1 ? a.hz[z + 1] : a.ez, a.hz[z]);
+        const T cf = a.az[i + sz] * k.zp(z);
         diag += cf;
         if (z < a.nz - 1) off -= cf * src(i + sz);
     }
@@ -131,14 +162,12 @@ __device__ void coarsest_small(float* __restrict__ v, const float* __restrict__ 
         const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
         const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i],
                              a.az[i + sz]};
-        const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
-                             x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
-                             z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
-        const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
+        const FaceK fk{a.kf, a.nx, a.ny, a.nz};
+        const float kk[6] = {fk.ym(y), fk.yp(y), fk.xm(x), fk.xp(x), fk.zm(z), fk.zp(z)};
         const int off[6] = {-1, 1, -sx, sx, -sz, sz};
         for (int k = 0; k < 6; ++k)
             if (has[k]) {
-                cf[k] = face_cf<float>(fr[k], hn[k], hh[k]);
+                cf[k] = fr[k] * kk[k];
                 diag += cf[k];
                 if (in[k]) nb[k] = i + off[k];
             }
@@ -540,16 +569,14 @@ __global__ void small_coef_kernel(float* __restrict__ coef, double* __restrict__
     const int sz = a.nx * a.ny;
     const bool in[6] = {y > 0, y < a.ny - 1, x > 0, x < a.nx - 1, z > 0, z < a.nz - 1};
     const float fr[6] = {a.ay[iay], a.ay[iay + 1], a.ax[iax], a.ax[iax + a.ny], a.az[i], a.az[i + sz]};
-    const float hn[6] = {y > 0 ? a.hy[y - 1] : a.ey, y < a.ny - 1 ? a.hy[y + 1] : a.ey,
-                         x > 0 ? a.hx[x - 1] : a.ex, x < a.nx - 1 ? a.hx[x + 1] : a.ex,
-                         z > 0 ? a.hz[z - 1] : a.ez, z < a.nz - 1 ? a.hz[z + 1] : a.ez};
-    const float hh[6] = {a.hy[y], a.hy[y], a.hx[x], a.hx[x], a.hz[z], a.hz[z]};
+    const FaceK fk{a.kf, a.nx, a.ny, a.nz};
+    const float kk[6] = {fk.ym(y), fk.yp(y), fk.xm(x), fk.xp(x), fk.zm(z), fk.zp(z)};
     float diag = 0.f;
     double d64 = 0.0;
     for (int k = 0; k < 6; ++k) {
         float c = 0.f;
         if (in[k] || Z) {
-            c = face_cf<float>(fr[k], hn[k], hh[k]);
+            c = fr[k] * kk[k];
             diag += c;
             d64 += (double)c;
         }
@@ -750,7 +777,7 @@ namespace {
 
 LevelArgs args_of(const whhd_mgfrac3d_s* h, const Level& L) {
     return LevelArgs{L.ax, L.ay, L.az, L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z,
-                     h->he[0], h->he[1], h->he[2]};
+                     h->he[0], h->he[1], h->he[2], L.kf};
 }
 
 hipError_t launch_status(const char* name) {
@@ -998,6 +1025,19 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         last = d;
         d = {half_up(d.x), half_up(d.y), half_up(d.z)};
         min_dim = std::min(d.x, std::min(d.y, d.z));
+    }
+    for (Level& L : h->lv) {
+        if (st != WHHD_OK) break;
+        const int nk = 2 * (L.d.x + L.d.y + L.d.z);
+        L.kf = (float*)alloc((size_t)nk * 4);
+        if (!L.kf) {
+            st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+            break;
+        }
+        hipLaunchKernelGGL(face_factor_kernel, dim3((nk + 255) / 256), dim3(256), 0, s, L.kf, L.hx, L.hy,
+                           L.hz, L.d.x, L.d.y, L.d.z, h->he[0], h->he[1], h->he[2]);
+        hipError_t e = launch_status("mg_face_factor_kernel");
+        if (e != hipSuccess) st = hip_fail(e, "mgfrac3d_create: face factors");
     }
     if (st == WHHD_OK) {
         const dim3 g = grid_of(nx, ny, nz);
