@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(1024) norm_final_kernel(double* __restrict__ o
 // summation order as the per-level kernels), so a sweep is seven coefficient
 // loads, seven LDS reads and a few FMAs per cell.  The coarsest level's sweeps
 // run in FP64, on one wavefront when it has at most 64 cells.
-constexpr int kSmallThreads = 256;
+constexpr int kSmallThreads = 1024;
 constexpr int kMaxSmall = 16;
 
 struct SmallLevel {
@@ -714,14 +714,15 @@ __global__ void __launch_bounds__(kSmallThreads) vcycle_small_kernel(SmallCycle 
             } else {
                 __syncthreads();
             }
+            // w / diag once: a multiply per sweep instead of an FP64 division
+            const double wd = diag > 1e-12 ? (double)w / diag : 0.0;
             int cur = 0;
             for (int it = 0; it < csweeps; ++it) {
                 if (i < L.n) {
                     const double* a = S + cur * L.n;
                     double o = 0.0;
                     for (int k = 0; k < 6; ++k) o -= cf[k] * a[nb.k[k]];
-                    S[(cur ^ 1) * L.n + i] = diag > 1e-12 ? (1.0 - (double)w) * a[i] + (double)w * (fi - o) / diag
-                                                          : 0.0;
+                    S[(cur ^ 1) * L.n + i] = diag > 1e-12 ? (1.0 - (double)w) * a[i] + wd * (fi - o) : 0.0;
                 }
                 if (one_wave) {
                     __builtin_amdgcn_wave_barrier();
