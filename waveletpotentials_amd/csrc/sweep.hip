@@ -399,6 +399,41 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
     // two independent chains of LDS round trips in flight
     const int half_id = lane >> 5, sl = lane & 31;
     Cx<T>* x = scr + (wave * 2 + half_id) * P;
+    // Per-lane constants, the same for every line: the sine weights of the lane's
+    // h values, the twiddles of its butterflies and the post-processing factors
+    // (registers instead of an LDS read per use).  The radix-2 stages of the
+    // in-place FFT run in pairs as radix-4 butterflies: stage sh pairs (j, j+half)
+    // and (j+2half, j+3half) with twiddle wa, stage sh+1 pairs (j, j+2half) with
+    // wb and (j+half, j+3half) with wb * W^(P/4) = -i wb, the same arithmetic as
+    // the two radix-2 stages with half the LDS round trips.  An odd stage count
+    // starts with the twiddle-free stage 0.
+    constexpr int NH = P >= 32 ? P / 32 : 1;       // h values per lane
+    constexpr int NS = LOGM - 1;                    // radix-2 stages
+    constexpr int S0 = NS & 1;                      // a lone stage 0 first
+    constexpr int NP4 = NS / 2;                     // radix-4 stage pairs
+    constexpr int NB4 = P / 4 >= 32 ? P / 128 : 1;  // radix-4 butterflies per lane
+    constexpr int per = P >= 32 ? P / 32 : 1;      // outputs k per lane
+    T snr[NH][2];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+        const int t = sl + 32 * q;
+        snr[q][0] = t < P ? sn[2 * t] : T(0);
+        snr[q][1] = t < P ? sn[2 * t + 1] : T(0);
+    }
+    Cx<T> twr[NP4 > 0 ? NP4 : 1][NB4][2];
+#pragma unroll
+    for (int ps = 0; ps < NP4; ++ps) {
+        const int sh = S0 + 2 * ps, half = 1 << sh;
+#pragma unroll
+        for (int q = 0; q < NB4; ++q) {
+            const int bfl = sl + 32 * q, k = bfl & (half - 1);
+            twr[ps][q][0] = bfl < P / 4 ? tw[k * (P >> (sh + 1))] : Cx<T>{T(1), T(0)};
+            twr[ps][q][1] = bfl < P / 4 ? tw[k * (P >> (sh + 2))] : Cx<T>{T(1), T(0)};
+        }
+    }
+    Cx<T> ekr[per];
+#pragma unroll
+    for (int q = 0; q < per; ++q) ekr[q] = sl * per + q < P ? ek[sl * per + q] : Cx<T>{T(0), T(0)};
     for (int l0 = wave * 2; l0 < tl; l0 += 2 * (kFftThreads / 64))
     for (int rep = 0; rep < reps; ++rep) {
         const int l = l0 + half_id;
@@ -406,42 +441,68 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
         T* f = rows + (live ? l : l0) * rs;
         // h_t = g_{2t} + i g_{2t+1}, stored bit-reversed for the in-place FFT
 #pragma unroll
-        for (int t = sl; t < P; t += 32) {
-            Cx<T> h;
-            {
-                const int m = 2 * t;
-                const T a = f[m], b = f[M - m];
-                h.r = sn[m] * (a + b) + T(0.5) * (a - b);
+        for (int q = 0; q < NH; ++q) {
+            const int t = sl + 32 * q;
+            if (t < P) {
+                Cx<T> h;
+                {
+                    const int m = 2 * t;
+                    const T a = f[m], b = f[M - m];
+                    h.r = snr[q][0] * (a + b) + T(0.5) * (a - b);
+                }
+                {
+                    const int m = 2 * t + 1;
+                    const T a = f[m], b = f[M - m];
+                    h.i = snr[q][1] * (a + b) + T(0.5) * (a - b);
+                }
+                const int r = (int)(__brev((unsigned)t) >> (33 - LOGM));
+                x[r] = h;
             }
-            {
-                const int m = 2 * t + 1;
-                const T a = f[m], b = f[M - m];
-                h.i = sn[m] * (a + b) + T(0.5) * (a - b);
-            }
-            const int r = (int)(__brev((unsigned)t) >> (33 - LOGM));
-            x[r] = h;
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
-#pragma unroll
-        for (int sh = 0; sh < LOGM - 1; ++sh) {
-            const int half = 1 << sh;
-            const int tstep = P >> (sh + 1);
+        if constexpr (S0) {
 #pragma unroll
             for (int bfl = sl; bfl < P / 2; bfl += 32) {
-                const int g = bfl >> sh, k = bfl & (half - 1);
-                const int i0 = g * 2 * half + k, i1 = i0 + half;
-                const Cx<T> w = tw[k * tstep];
-                const Cx<T> a = x[i0], b = x[i1];
-                const T tr = w.r * b.r - w.i * b.i, ti = w.r * b.i + w.i * b.r;
-                x[i0] = Cx<T>{a.r + tr, a.i + ti};
-                x[i1] = Cx<T>{a.r - tr, a.i - ti};
+                const Cx<T> a = x[2 * bfl], b = x[2 * bfl + 1];
+                x[2 * bfl] = Cx<T>{a.r + b.r, a.i + b.i};
+                x[2 * bfl + 1] = Cx<T>{a.r - b.r, a.i - b.i};
+            }
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+        }
+#pragma unroll
+        for (int ps = 0; ps < NP4; ++ps) {
+            const int sh = S0 + 2 * ps, half = 1 << sh;
+#pragma unroll
+            for (int q = 0; q < NB4; ++q) {
+                const int bfl = sl + 32 * q;
+                if (bfl < P / 4) {
+                    const int g = bfl >> sh, k = bfl & (half - 1);
+                    const int j = g * 4 * half + k;
+                    Cx<T> e0 = x[j], e1 = x[j + half], e2 = x[j + 2 * half], e3 = x[j + 3 * half];
+                    const Cx<T> wa = twr[ps][q][0], wb = twr[ps][q][1];
+                    const Cx<T> wc{wb.i, -wb.r};  // -i wb
+                    auto bfly = [](Cx<T>& a, Cx<T>& b, const Cx<T>& w) {
+                        const T tr = w.r * b.r - w.i * b.i, ti = w.r * b.i + w.i * b.r;
+                        const Cx<T> s{a.r + tr, a.i + ti}, d{a.r - tr, a.i - ti};
+                        a = s;
+                        b = d;
+                    };
+                    bfly(e0, e1, wa);
+                    bfly(e2, e3, wa);
+                    bfly(e0, e2, wb);
+                    bfly(e1, e3, wc);
+                    x[j] = e0;
+                    x[j + half] = e1;
+                    x[j + 2 * half] = e2;
+                    x[j + 3 * half] = e3;
+                }
             }
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
         }
         // G_k -> outputs; lane holds the `per` consecutive k of its block
-        constexpr int per = P >= 32 ? P / 32 : 1;
         const int k0 = sl * per;
         T run = T(0);
         T rk[per], ik[per];
@@ -455,7 +516,7 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
                 const T er = T(0.5) * (hk.r + hm.r), ei = T(0.5) * (hk.i - hm.i);
                 const T dr = hk.r - hm.r, di = hk.i + hm.i;
                 const T orr = T(0.5) * di, oi = T(-0.5) * dr;
-                const Cx<T> e = ek[k];
+                const Cx<T> e = ekr[q];
                 rk[q] = er + e.r * orr - e.i * oi;
                 ik[q] = ei + e.r * oi + e.i * orr;
                 if (k == 0) rk[q] *= T(0.5);
