@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "whhd_internal.h"
@@ -318,15 +319,32 @@ struct Cx {
     T r, i;
 };
 
-// inclusive prefix sum over each half (32 lanes) of the wavefront
+// inclusive prefix sum over each half (32 lanes) of the wavefront.  FP32: DPP
+// row shifts by 1, 2, 4, 8 within each row of 16 lanes (lanes without a source
+// add 0), then row_bcast:15 adds the last lane of rows 0 and 2 to rows 1 and 3
+// (row mask 0xA keeps the two halves apart); FP64 through lane shuffles.
 template <class T>
 __device__ __forceinline__ T half_scan(T v, int sl) {
+    if constexpr (sizeof(T) == 4) {
+        auto step = [&](auto ctrl, auto rmask, auto bound) {
+            const int u = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), decltype(ctrl)::value,
+                                                      decltype(rmask)::value, 0xF, decltype(bound)::value);
+            v += __builtin_bit_cast(float, u);
+        };
+        step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xF>{}, std::true_type{});
+        step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xF>{}, std::true_type{});
+        step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xF>{}, std::true_type{});
+        step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xF>{}, std::true_type{});
+        step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{}, std::false_type{});
+        return v;
+    } else {
 #pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const T u = __shfl_up(v, o, 32);
-        if (sl >= o) v += u;
+        for (int o = 1; o < 32; o <<= 1) {
+            const T u = __shfl_up(v, o, 32);
+            if (sl >= o) v += u;
+        }
+        return v;
     }
-    return v;
 }
 
 // lines: li = inner + outer * n_inner; line li starts at inner * s_inner +
@@ -535,8 +553,14 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
                     T xe = c * pre;                         // X_{2k} = c F_{2k+1}
                     T xo = -c * ik[q];                      // X_{2k-1} = c F_{2k}
                     if (lam_s && rep == 0) {
-                        xe /= lsum[l] + lam_s[2 * k];
-                        if (k >= 1) xo /= lsum[l] + lam_s[2 * k - 1];
+                        if constexpr (sizeof(T) == 4) {
+                            // reciprocal (within 1 ulp) instead of two IEEE divisions
+                            xe *= __builtin_amdgcn_rcpf(lsum[l] + lam_s[2 * k]);
+                            if (k >= 1) xo *= __builtin_amdgcn_rcpf(lsum[l] + lam_s[2 * k - 1]);
+                        } else {
+                            xe /= lsum[l] + lam_s[2 * k];
+                            if (k >= 1) xo /= lsum[l] + lam_s[2 * k - 1];
+                        }
                     }
                     f[1 + 2 * k] = xe;
                     if (k >= 1) f[2 * k] = xo;
