@@ -425,19 +425,27 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
     // wb and (j+half, j+3half) with wb * W^(P/4) = -i wb, the same arithmetic as
     // the two radix-2 stages with half the LDS round trips.  An odd stage count
     // starts with the twiddle-free stage 0.
-    constexpr int NH = P >= 32 ? P / 32 : 1;       // h values per lane
     constexpr int NS = LOGM - 1;                    // radix-2 stages
     constexpr int S0 = NS & 1;                      // a lone stage 0 first
     constexpr int NP4 = NS / 2;                     // radix-4 stage pairs
     constexpr int NB4 = P / 4 >= 32 ? P / 128 : 1;  // radix-4 butterflies per lane
     constexpr int per = P >= 32 ? P / 32 : 1;      // outputs k per lane
-    T snr[NH][2];
+    // the first stage (stage 0 alone, or the first radix-4 pair, whose twiddles
+    // are 1 and -i) runs on the h values in registers: a lane forms the G h
+    // values that land on bit-reversed positions r .. r + G - 1 (t0 + 0, P/2,
+    // and for G = 4 also P/4, 3P/4) and writes the butterflies' outputs
+    constexpr int G = S0 ? 2 : 4;
+    constexpr int NQ = P / G >= 32 ? P / G / 32 : 1;  // groups per lane
+    T snr[NQ][G][2];
 #pragma unroll
-    for (int q = 0; q < NH; ++q) {
-        const int t = sl + 32 * q;
-        snr[q][0] = t < P ? sn[2 * t] : T(0);
-        snr[q][1] = t < P ? sn[2 * t + 1] : T(0);
-    }
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int t = sl + 32 * q + (u == 1 ? P / 2 : u == 2 ? P / 4 : u == 3 ? 3 * P / 4 : 0);
+            const bool ok = sl + 32 * q < P / G;
+            snr[q][u][0] = ok ? sn[2 * t] : T(0);
+            snr[q][u][1] = ok ? sn[2 * t + 1] : T(0);
+        }
     Cx<T> twr[NP4 > 0 ? NP4 : 1][NB4][2];
 #pragma unroll
     for (int ps = 0; ps < NP4; ++ps) {
@@ -457,40 +465,51 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
         const int l = l0 + half_id;
         const bool live = l < tl;
         T* f = rows + (live ? l : l0) * rs;
-        // h_t = g_{2t} + i g_{2t+1}, stored bit-reversed for the in-place FFT
+        // h_t = g_{2t} + i g_{2t+1} in bit-reversed order, through the first stage
+        auto bfly = [](Cx<T>& a, Cx<T>& b, const Cx<T>& w) {
+            const T tr = w.r * b.r - w.i * b.i, ti = w.r * b.i + w.i * b.r;
+            const Cx<T> s{a.r + tr, a.i + ti}, d{a.r - tr, a.i - ti};
+            a = s;
+            b = d;
+        };
 #pragma unroll
-        for (int q = 0; q < NH; ++q) {
-            const int t = sl + 32 * q;
-            if (t < P) {
-                Cx<T> h;
-                {
-                    const int m = 2 * t;
-                    const T a = f[m], b = f[M - m];
-                    h.r = snr[q][0] * (a + b) + T(0.5) * (a - b);
+        for (int q = 0; q < NQ; ++q) {
+            const int t0 = sl + 32 * q;
+            if (t0 < P / G) {
+                Cx<T> e[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int t = t0 + (u == 1 ? P / 2 : u == 2 ? P / 4 : u == 3 ? 3 * P / 4 : 0);
+                    {
+                        const int m = 2 * t;
+                        const T a = f[m], b = f[M - m];
+                        e[u].r = snr[q][u][0] * (a + b) + T(0.5) * (a - b);
+                    }
+                    {
+                        const int m = 2 * t + 1;
+                        const T a = f[m], b = f[M - m];
+                        e[u].i = snr[q][u][1] * (a + b) + T(0.5) * (a - b);
+                    }
                 }
-                {
-                    const int m = 2 * t + 1;
-                    const T a = f[m], b = f[M - m];
-                    h.i = snr[q][1] * (a + b) + T(0.5) * (a - b);
+                const int r = (int)(__brev((unsigned)t0) >> (33 - LOGM));
+                if constexpr (G == 2) {
+                    x[r] = Cx<T>{e[0].r + e[1].r, e[0].i + e[1].i};
+                    x[r + 1] = Cx<T>{e[0].r - e[1].r, e[0].i - e[1].i};
+                } else {
+                    const Cx<T> one{T(1), T(0)}, mi{T(0), T(-1)};
+                    bfly(e[0], e[1], one);
+                    bfly(e[2], e[3], one);
+                    bfly(e[0], e[2], one);
+                    bfly(e[1], e[3], mi);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[r + u] = e[u];
                 }
-                const int r = (int)(__brev((unsigned)t) >> (33 - LOGM));
-                x[r] = h;
             }
         }
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
-        if constexpr (S0) {
 #pragma unroll
-            for (int bfl = sl; bfl < P / 2; bfl += 32) {
-                const Cx<T> a = x[2 * bfl], b = x[2 * bfl + 1];
-                x[2 * bfl] = Cx<T>{a.r + b.r, a.i + b.i};
-                x[2 * bfl + 1] = Cx<T>{a.r - b.r, a.i - b.i};
-            }
-            __builtin_amdgcn_wave_barrier();
-            asm volatile("" ::: "memory");
-        }
-#pragma unroll
-        for (int ps = 0; ps < NP4; ++ps) {
+        for (int ps = S0 ? 0 : 1; ps < NP4; ++ps) {
             const int sh = S0 + 2 * ps, half = 1 << sh;
 #pragma unroll
             for (int q = 0; q < NB4; ++q) {
@@ -501,12 +520,6 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
                     Cx<T> e0 = x[j], e1 = x[j + half], e2 = x[j + 2 * half], e3 = x[j + 3 * half];
                     const Cx<T> wa = twr[ps][q][0], wb = twr[ps][q][1];
                     const Cx<T> wc{wb.i, -wb.r};  // -i wb
-                    auto bfly = [](Cx<T>& a, Cx<T>& b, const Cx<T>& w) {
-                        const T tr = w.r * b.r - w.i * b.i, ti = w.r * b.i + w.i * b.r;
-                        const Cx<T> s{a.r + tr, a.i + ti}, d{a.r - tr, a.i - ti};
-                        a = s;
-                        b = d;
-                    };
                     bfly(e0, e1, wa);
                     bfly(e2, e3, wa);
                     bfly(e0, e2, wb);
