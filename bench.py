@@ -11,7 +11,10 @@ largest share of the solve: a second run of K solves records HIP events after
 every launch on the stream the library launches on (the library's stage
 trace), giving each launch's duration in context.  `cpu_baseline` times the
 C oracle (a port of the reference algorithm) built in FP32 on this host, with
-the FP64 oracle's figure beside it.
+the FP64 oracle's figure beside it.  The line also carries `learned_init`
+(the learned initial guess against the plain multigrid on held-out 192^3
+seeds) and `secondary` (the batch, solids and sweep workloads' figures from
+a few steps each; `--workload` runs any of them alone).
 """
 import argparse
 import json
@@ -37,6 +40,8 @@ def parse():
     p.add_argument("--no-learned", action="store_true",
                    help="recover workload: skip the learned-init multigrid measurement")
     p.add_argument("--learned-seeds", type=int, default=4)
+    p.add_argument("--no-secondary", action="store_true",
+                   help="recover workload: skip the batch, solids and sweep figures")
     p.add_argument("--workload", choices=("recover", "batch", "solids", "learned", "sweep"),
                    default="recover",
                    help="recover: one N^3 grid (the headline); batch: 64 grids of 32^3..128^3 "
@@ -87,17 +92,10 @@ def main():
               f"--gpus {args.gpus}); the single-GPU number is the whole-job value.", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "batch":
-        bench_batch(args)
-        return
-    if args.workload == "solids":
-        bench_solids(args)
-        return
-    if args.workload == "learned":
-        bench_learned(args)
-        return
-    if args.workload == "sweep":
-        bench_sweep(args)
+    others = {"batch": bench_batch, "solids": bench_solids, "learned": bench_learned,
+              "sweep": bench_sweep}
+    if args.workload in others:
+        print(json.dumps(others[args.workload](args)))
         return
 
     import numpy as np
@@ -226,7 +224,28 @@ def main():
         out["cpu_baseline"] = None
     if not args.no_learned:
         out["learned_init"] = learned_summary(args.learned_seeds)
+    if not args.no_secondary:
+        out["secondary"] = secondary(args)
     print(json.dumps(out))
+
+
+def secondary(args):
+    """The other workloads' lines (batch, solids, sweep), measured in the same
+    run with a few steps each and no CPU legs, condensed to their figures."""
+    sub = argparse.Namespace(**{**vars(args), "steps": 5, "warmup": 1, "no_cpu_baseline": True})
+    out = {}
+    for name, fn in (("batch", bench_batch), ("solids", bench_solids), ("sweep", bench_sweep)):
+        r = fn(sub)
+        row = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "steps")}
+        rf = r.get("roofline") or {}
+        row["roofline_frac"] = rf.get("frac")
+        for k in ("v_cycle_ms", "dst_solve_ms", "kernel"):
+            if k in rf:
+                row[k] = rf[k]
+        if "accuracy" in r:
+            row["accuracy"] = r["accuracy"]
+        out[name] = row
+    return out
 
 
 def calibrate():
@@ -373,7 +392,7 @@ def bench_batch(args):
                                "cores": threads, "kind": "port",
                                "sample": "one pass of the C oracle over all 64 problems "
                                          "(FP64, OpenMP)"}
-    print(json.dumps(out))
+    return out
 
 
 def _line(metric, value, unit, args, ms, data, config, higher=True, dtype="f32"):
@@ -451,7 +470,7 @@ def bench_sweep(args):
                        "dst_solve_ms": round(dst_ms, 4), "algorithmic_bytes": dst_bytes}
     out["accuracy"] = {"curl_q_vs_u_rel_l2": O.rel_l2(u, list(curl))}
     out["cpu_baseline"] = None
-    print(json.dumps(out))
+    return out
 
 
 def bench_solids(args):
@@ -537,7 +556,7 @@ def bench_solids(args):
                                "unit": "Mcells/s", "cores": threads, "kind": "port",
                                "sample": f"2 FP64 V-cycles of the C oracle at 192^3, "
                                          f"scaled to the {cycles} cycles of the GPU solve"}
-    print(json.dumps(out))
+    return out
 
 
 def bench_learned(args):
@@ -597,7 +616,7 @@ def bench_learned(args):
                        "prediction_ms": round(pred_ms, 3)}
     out["cpu_baseline"] = None
     out["rows"] = rows
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":
