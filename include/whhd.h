@@ -262,11 +262,13 @@ WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre
  * unevaluated sum dst + dst_lo of two FP32 arrays (an FP32 iterate alone has a
  * relative-residual floor near 1e-6 on fine grids).  Each cycle forms
  * r = f - A (dst + dst_lo) in FP64 with its FP64 norm, stops when
- * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs one V-cycle
- * (as Solve) on A e = r from e = 0 and adds e to the pair.  pre_v (NULL: zero;
- * may equal dst) starts dst; dst_lo starts at zero.  Writes the V-cycles run
- * and the last relative residual; returns WHHD_NONCONVERGED if tol was not
- * reached.  Synchronizes `stream` once per cycle (16-byte read-back). */
+ * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs two V-cycles
+ * (as Solve; one when the observed convergence rate says one reaches tol, or
+ * one cycle of the budget is left) on A e = r from e = 0 and adds e to the
+ * pair.  pre_v (NULL: zero; may equal dst) starts dst; dst_lo starts at zero.
+ * Writes the V-cycles run and the last relative residual; returns
+ * WHHD_NONCONVERGED if tol was not reached.  Synchronizes `stream` once per
+ * FP64 residual (16-byte read-back). */
 WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
                                      const float* pre_v, const float* f, int max_cycles,
                                      double tol, int nu1, int nu2, float weight, int* cycles,
