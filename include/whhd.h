@@ -262,10 +262,10 @@ WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre
  * unevaluated sum dst + dst_lo of two FP32 arrays (an FP32 iterate alone has a
  * relative-residual floor near 1e-6 on fine grids).  Each cycle forms
  * r = f - A (dst + dst_lo) in FP64 with its FP64 norm, stops when
- * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs two V-cycles
- * (as Solve; one when the observed convergence rate says one reaches tol, or
- * one cycle of the budget is left) on A e = r from e = 0 and adds e to the
- * pair.  pre_v (NULL: zero; may equal dst) starts dst; dst_lo starts at zero.
+ * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs 1-4
+ * V-cycles (as Solve; as many as the observed convergence rate says reach tol,
+ * two before a rate is known, never past max_cycles) on A e = r from e = 0 and
+ * adds e to the pair.  pre_v (NULL: zero; may equal dst) starts dst; dst_lo starts at zero.
  * Writes the V-cycles run and the last relative residual; returns
  * WHHD_NONCONVERGED if tol was not reached.  Synchronizes `stream` once per
  * FP64 residual (16-byte read-back). */

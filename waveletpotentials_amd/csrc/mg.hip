@@ -12,6 +12,7 @@
 // synchronization (the residual queries read back two FP64 sums).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -1190,13 +1191,19 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         const double prev = res;
         res = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
         if (res <= tol || c >= max_cycles) break;
-        // Two V-cycles per FP64 residual (the second on the FP32 residual of
-        // the correction equation, exact enough while e is a correction), one
-        // when the observed rate says one will reach tol or the budget ends.
-        const double rate = (c > 0 && prev > 0) ? std::pow(res / prev, 1.0 / last_k) : 1.0;
-        const int k = (max_cycles - c >= 2 && !(res * rate <= tol)) ? 2 : 1;
-        if (k == 2) TRY(vcycle(h, 0, nu1, nu2, w, s, true));
-        TRY(vcycle(h, 0, nu1, nu2, w, s, k == 1, dst, lo));
+        // Up to kMaxK V-cycles per FP64 residual (all but the first on the
+        // FP32 residual of the correction equation, exact enough while e is a
+        // correction): as many as the observed per-cycle rate says reach tol,
+        // two before a rate is known, never past the budget.
+        constexpr int kMaxK = 4;
+        int k = 2;
+        if (c > 0 && prev > res && res > 0) {
+            const double rate = std::pow(res / prev, 1.0 / last_k);
+            k = (int)std::ceil(std::log(tol / res) / std::log(rate));
+        }
+        k = std::max(1, std::min(std::min(k, kMaxK), max_cycles - c));
+        for (int j = 0; j < k; ++j)
+            TRY(j + 1 < k ? vcycle(h, 0, nu1, nu2, w, s, j == 0) : vcycle(h, 0, nu1, nu2, w, s, j == 0, dst, lo));
         c += k - 1;
         last_k = k;
     }
