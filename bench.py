@@ -513,8 +513,9 @@ def bench_solids(args):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     # V-cycle time and the stage model's bytes per cycle (per finest cell: 4
-    # smoothing sweeps x 6 + residual with its restriction 5.125 + prolong 3.625
-    # = 32.75 floats, x 8/7 for the coarser levels)
+    # smoothing sweeps x 6 + residual with its restriction 5.125 + the coarse
+    # correction the first post-smoothing sweep prolongs 0.125 = 29.25 floats,
+    # x 8/7 for the coarser levels)
     v = torch.zeros_like(f)
     e0, e1 = torch.cuda.Event(True), torch.cuda.Event(True)
     e0.record()
@@ -522,7 +523,7 @@ def bench_solids(args):
     e1.record()
     torch.cuda.synchronize()
     vc_ms = e0.elapsed_time(e1) / 20
-    cyc_bytes = 32.75 * 8.0 / 7.0 * 4 * cells
+    cyc_bytes = 29.25 * 8.0 / 7.0 * 4 * cells
     achieved = cyc_bytes / (vc_ms * 1e-3) / 1e9
     # divergence of the corrected velocity a (u - grad p), which the solve makes
     # zero (the reference checks div(a (u - grad p)) the same way,

@@ -79,7 +79,8 @@ struct FaceK {
 };
 
 // Diagonal and off-diagonal sum of cell (x, y, z) applied to the iterate,
-// read through src(j), accumulated in T; the face coefficients are FP32 (the
+// read through src(j, dy, dx, dz) (a neighbour's index and its offset from the
+// cell), accumulated in T; the face coefficients are FP32 (the
 // operator every level's kernels share).  Z: walls carry a Dirichlet
 // coefficient with the exterior spacing; otherwise closed.
 template <bool Z, class T, class Src>
@@ -94,50 +95,62 @@ __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int 
     if (y > 0 || Z) {
         const T cf = a.ay[iay] * k.ym(y);
         diag += cf;
-        if (y > 0) off -= cf * src(i - 1);
+        if (y > 0) off -= cf * src(i - 1, -1, 0, 0);
     }
     if (y < a.ny - 1 || Z) {
         const T cf = a.ay[iay + 1] * k.yp(y);
         diag += cf;
-        if (y < a.ny - 1) off -= cf * src(i + 1);
+        if (y < a.ny - 1) off -= cf * src(i + 1, 1, 0, 0);
     }
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
     if (x > 0 || Z) {
         const T cf = a.ax[iax] * k.xm(x);
         diag += cf;
-        if (x > 0) off -= cf * src(i - sx);
+        if (x > 0) off -= cf * src(i - sx, 0, -1, 0);
     }
     if (x < a.nx - 1 || Z) {
         const T cf = a.ax[iax + a.ny] * k.xp(x);
         diag += cf;
-        if (x < a.nx - 1) off -= cf * src(i + sx);
+        if (x < a.nx - 1) off -= cf * src(i + sx, 0, 1, 0);
     }
     if (z > 0 || Z) {
         const T cf = a.az[i] * k.zm(z);
         diag += cf;
-        if (z > 0) off -= cf * src(i - sz);
+        if (z > 0) off -= cf * src(i - sz, 0, 0, -1);
     }
     if (z < a.nz - 1 || Z) {
         const T cf = a.az[i + sz] * k.zp(z);
         diag += cf;
-        if (z < a.nz - 1) off -= cf * src(i + sz);
+        if (z < a.nz - 1) off -= cf * src(i + sz, 0, 0, 1);
     }
 }
 
 // One weighted Jacobi sweep dst <- src (cuWjFrac3D_n / _z); cells whose
 // faces are all closed get 0.  ZERO: src is zero (the first sweep of a V-cycle
 // on a correction) and is not read; the arithmetic is the same with zeros.
-template <bool Z, bool ZERO = false>
+// PRO: the sweep's input is src plus the prolonged coarse correction vc
+// (SolveAddP1RN, cuProlongate3D.cu:594-616: every cell adds its parent), formed
+// per read as prolong_add_kernel would store it, so the post-smoothing's first
+// sweep needs no separate prolongation pass.
+template <bool Z, bool ZERO = false, bool PRO = false>
 __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__ src,
-                              const float* __restrict__ f, LevelArgs a, float w) {
+                              const float* __restrict__ f, LevelArgs a, float w,
+                              const float* __restrict__ vc = nullptr, int cx = 0, int cy = 0) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
     const int z = blockIdx.z;
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
+    const int64_t cs = (int64_t)cx * cy;
+    auto in = [&](int64_t j, int dy, int dx, int dz) {
+        if constexpr (ZERO) return 0.f;
+        else if constexpr (PRO)
+            return src[j] + vc[((y + dy) >> 1) + (int64_t)((x + dx) >> 1) * cy + (int64_t)((z + dz) >> 1) * cs];
+        else return src[j];
+    };
     float diag, off;
-    stencil<Z, float>(a, [&](int64_t j) { return ZERO ? 0.f : src[j]; }, x, y, z, i, diag, off);
-    const float v = ZERO ? 0.f : src[i];
+    stencil<Z, float>(a, in, x, y, z, i, diag, off);
+    const float v = in(i, 0, 0, 0);
     dst[i] = diag > 1e-12f ? (1.f - w) * v + w * (f[i] - off) / diag : 0.f;
 }
 
@@ -213,7 +226,7 @@ __global__ void __launch_bounds__(256) coarsest_kernel(float* __restrict__ v,
             const int x = (int)((i / a.ny) % a.nx);
             const int z = (int)(i / ((int64_t)a.ny * a.nx));
             double diag, off;
-            stencil<Z, double>(a, [&](int64_t j) { return A[j]; }, x, y, z, i, diag, off);
+            stencil<Z, double>(a, [&](int64_t j, auto...) { return A[j]; }, x, y, z, i, diag, off);
             B[i] = diag > 1e-12 ? (1.0 - w) * A[i] + w * ((double)f[i] - off) / diag : 0.0;
         }
         __syncthreads();
@@ -246,7 +259,7 @@ __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
     if (y < a.ny && x < a.nx) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
         const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
-        auto val = [&](int64_t j) { return (double)hi[j] + (double)lo[j]; };
+        auto val = [&](int64_t j, auto...) { return (double)hi[j] + (double)lo[j]; };
         double diag = 0.0, off = 0.0;
         const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
         const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
@@ -329,7 +342,7 @@ __global__ void jacobi_pair_kernel(float* __restrict__ hi, float* __restrict__ l
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z, float>(a, [&](int64_t j) { return src[j]; }, x, y, z, i, diag, off);
+    stencil<Z, float>(a, [&](int64_t j, auto...) { return src[j]; }, x, y, z, i, diag, off);
     const float e = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
     const float hv = hi[i], t = lo[i] + e;
     const float s = hv + t;
@@ -348,7 +361,7 @@ __global__ void residual_kernel(float* __restrict__ r, const float* __restrict__
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z, float>(a, [&](int64_t j) { return v[j]; }, x, y, z, i, diag, off);
+    stencil<Z, float>(a, [&](int64_t j, auto...) { return v[j]; }, x, y, z, i, diag, off);
     r[i] = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
 }
 
@@ -371,7 +384,7 @@ __global__ void __launch_bounds__(512) residual_restrict_kernel(float* __restric
     if (y < a.ny && x < a.nx && z < a.nz) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
         float diag, off;
-        stencil<Z, float>(a, [&](int64_t j) { return v[j]; }, x, y, z, i, diag, off);
+        stencil<Z, float>(a, [&](int64_t j, auto...) { return v[j]; }, x, y, z, i, diag, off);
         r = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
     }
     rs[tz][tx][ty] = r;
@@ -787,17 +800,22 @@ hipError_t launch_status(const char* name) {
     return hipGetLastError();
 }
 
-// One sweep dst <- J(src); zero: src is zero and is not read.
+// One sweep dst <- J(src); zero: src is zero and is not read; C (not with
+// zero): the sweep's input is src plus C's correction prolonged.
 int jacobi(const whhd_mgfrac3d_s* h, const Level& L, float* dst, const float* src, const float* f,
-           float w, hipStream_t s, bool zero = false) {
+           float w, hipStream_t s, bool zero = false, const Level* C = nullptr) {
     const LevelArgs a = args_of(h, L);
     const dim3 g = grid_of(L.d.x, L.d.y, L.d.z), b(64, 4);
+    const float* vc = C ? C->v : nullptr;
+    const int cx = C ? C->d.x : 0, cy = C ? C->d.y : 0;
     if (h->bc == 'z') {
-        if (zero) hipLaunchKernelGGL((jacobi_kernel<true, true>), g, b, 0, s, dst, src, f, a, w);
-        else hipLaunchKernelGGL((jacobi_kernel<true, false>), g, b, 0, s, dst, src, f, a, w);
+        if (zero) hipLaunchKernelGGL((jacobi_kernel<true, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        else if (C) hipLaunchKernelGGL((jacobi_kernel<true, false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        else hipLaunchKernelGGL((jacobi_kernel<true, false>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
     } else {
-        if (zero) hipLaunchKernelGGL((jacobi_kernel<false, true>), g, b, 0, s, dst, src, f, a, w);
-        else hipLaunchKernelGGL((jacobi_kernel<false, false>), g, b, 0, s, dst, src, f, a, w);
+        if (zero) hipLaunchKernelGGL((jacobi_kernel<false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        else if (C) hipLaunchKernelGGL((jacobi_kernel<false, false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        else hipLaunchKernelGGL((jacobi_kernel<false, false>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
     }
     hipError_t e = launch_status("mg_jacobi_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi kernel");
@@ -837,12 +855,18 @@ int64_t cells(const Dim3& d) { return (int64_t)d.x * d.y * d.z; }
 // not read it (and with no sweep it is written).
 // hi, lo (n > 0, zero false): the last sweep adds its result to the pair
 // hi + lo instead of storing it (L.v is then left as it was).
+// C (n >= 2, or no pair): the first sweep reads L.v plus C's prolonged correction.
 int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t s, bool zero,
-           float* hi = nullptr, float* lo = nullptr) {
+           float* hi = nullptr, float* lo = nullptr, const Level* C = nullptr) {
     float* a = L.v;
     float* b = L.t;
     if (zero && n == 0) return fill(L.v, 0.f, cells(L.d), s);
     for (int k = 0; k < n; ++k) {
+        if (C && k == 0) {
+            TRY(jacobi(h, L, b, a, L.f, w, s, false, C));
+            std::swap(a, b);
+            continue;
+        }
         if (hi && k == n - 1) {
             const LevelArgs la = args_of(h, L);
             if (h->bc == 'z')
@@ -930,6 +954,9 @@ int vcycle_body(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, h
     hipError_t e = launch_status("mg_residual_restrict_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg residual-restrict kernel");
     TRY(vcycle(h, l + 1, nu1, nu2, w, s, true));
+    // the prolongation rides on the first post-smoothing sweep when that sweep
+    // is not also the pair update
+    if (nu2 >= 2 || (nu2 == 1 && !hi)) return smooth(h, L, nu2, w, s, false, hi, lo, &C);
     hipLaunchKernelGGL(prolong_add_kernel, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s, L.v,
                        C.v, L.d.x, L.d.y, L.d.z, C.d.x, C.d.y);
     e = launch_status("mg_prolong_add_kernel");
