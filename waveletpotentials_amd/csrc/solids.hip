@@ -126,13 +126,14 @@ __global__ void __launch_bounds__(256) eikonal_kernel(float* __restrict__ dst, c
     const int zend = min(z0 + kEikZ, nz);
     for (int z = z0; z < zend; ++z, i += sz) {
         const float above = z < nz - 1 ? src[i + sz] : kFar;
-        if (!(c > -1e-12f)) {
-            dst[i] = c;
-        } else {
-            const float l = x > 0 ? src[i - sx] : kFar, r = x < nx - 1 ? src[i + sx] : kFar;
-            const float d = y > 0 ? src[i - 1] : kFar, u = y < ny - 1 ? src[i + 1] : kFar;
-            dst[i] = eikonal_update(fminf(l, r), fminf(d, u), fminf(below, above), h);
-        }
+        // neighbours through clamped (always valid) indices, replaced by kFar
+        // outside the grid: no branch per load
+        const float l0 = src[i - (x > 0 ? sx : 0)], r0 = src[i + (x < nx - 1 ? sx : 0)];
+        const float d0 = src[i - (y > 0 ? 1 : 0)], u0 = src[i + (y < ny - 1 ? 1 : 0)];
+        const float l = x > 0 ? l0 : kFar, r = x < nx - 1 ? r0 : kFar;
+        const float d = y > 0 ? d0 : kFar, u = y < ny - 1 ? u0 : kFar;
+        const float upd = eikonal_update(fminf(l, r), fminf(d, u), fminf(below, above), h);
+        dst[i] = c > -1e-12f ? upd : c;
         below = c;
         c = above;
     }
