@@ -87,42 +87,37 @@ template <bool Z, class T, class Src>
 __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int y, int z, int64_t i,
                                         T& diag, T& off) {
     const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
-    diag = T(0);
-    off = T(0);
-    // y faces: ay index (y, x, z) with extents (nx, ny+1, nz)
     const FaceK k{a.kf, a.nx, a.ny, a.nz};
+    // Every load is issued from a valid index whether or not the cell has that
+    // face or neighbour (no branch per load); a face the cell does not have
+    // enters with coefficient 0 and a neighbour outside the grid with value 0,
+    // which leaves both sums exactly as if the term were skipped.
     const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
-    if (y > 0 || Z) {
-        const T cf = a.ay[iay] * k.ym(y);
-        diag += cf;
-        if (y > 0) off -= cf * src(i - 1, -1, 0, 0);
-    }
-    if (y < a.ny - 1 || Z) {
-        const T cf = a.ay[iay + 1] * k.yp(y);
-        diag += cf;
-        if (y < a.ny - 1) off -= cf * src(i + 1, 1, 0, 0);
-    }
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
-    if (x > 0 || Z) {
-        const T cf = a.ax[iax] * k.xm(x);
-        diag += cf;
-        if (x > 0) off -= cf * src(i - sx, 0, -1, 0);
-    }
-    if (x < a.nx - 1 || Z) {
-        const T cf = a.ax[iax + a.ny] * k.xp(x);
-        diag += cf;
-        if (x < a.nx - 1) off -= cf * src(i + sx, 0, 1, 0);
-    }
-    if (z > 0 || Z) {
-        const T cf = a.az[i] * k.zm(z);
-        diag += cf;
-        if (z > 0) off -= cf * src(i - sz, 0, 0, -1);
-    }
-    if (z < a.nz - 1 || Z) {
-        const T cf = a.az[i + sz] * k.zp(z);
-        diag += cf;
-        if (z < a.nz - 1) off -= cf * src(i + sz, 0, 0, 1);
-    }
+    const bool ym = y > 0, yp = y < a.ny - 1, xm = x > 0, xp = x < a.nx - 1, zm = z > 0, zp = z < a.nz - 1;
+    const float f0 = a.ay[iay], f1 = a.ay[iay + 1], f2 = a.ax[iax], f3 = a.ax[iax + a.ny], f4 = a.az[i],
+                f5 = a.az[i + sz];
+    const T n0 = src(ym ? i - 1 : i, ym ? -1 : 0, 0, 0), n1 = src(yp ? i + 1 : i, yp ? 1 : 0, 0, 0);
+    const T n2 = src(xm ? i - sx : i, 0, xm ? -1 : 0, 0), n3 = src(xp ? i + sx : i, 0, xp ? 1 : 0, 0);
+    const T n4 = src(zm ? i - sz : i, 0, 0, zm ? -1 : 0), n5 = src(zp ? i + sz : i, 0, 0, zp ? 1 : 0);
+    const float k0 = k.ym(y), k1 = k.yp(y), k2 = k.xm(x), k3 = k.xp(x), k4 = k.zm(z), k5 = k.zp(z);
+    const T c0 = (ym || Z) ? T(f0 * k0) : T(0), c1 = (yp || Z) ? T(f1 * k1) : T(0);
+    const T c2 = (xm || Z) ? T(f2 * k2) : T(0), c3 = (xp || Z) ? T(f3 * k3) : T(0);
+    const T c4 = (zm || Z) ? T(f4 * k4) : T(0), c5 = (zp || Z) ? T(f5 * k5) : T(0);
+    diag = T(0);
+    diag += c0;
+    diag += c1;
+    diag += c2;
+    diag += c3;
+    diag += c4;
+    diag += c5;
+    off = T(0);
+    off -= c0 * (ym ? n0 : T(0));
+    off -= c1 * (yp ? n1 : T(0));
+    off -= c2 * (xm ? n2 : T(0));
+    off -= c3 * (xp ? n3 : T(0));
+    off -= c4 * (zm ? n4 : T(0));
+    off -= c5 * (zp ? n5 : T(0));
 }
 
 // One weighted Jacobi sweep dst <- src (cuWjFrac3D_n / _z); cells whose
