@@ -254,40 +254,34 @@ __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
     if (y < a.ny && x < a.nx) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
         const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
-        auto val = [&](int64_t j, auto...) { return (double)hi[j] + (double)lo[j]; };
-        double diag = 0.0, off = 0.0;
+        auto val = [&](int64_t j) { return (double)hi[j] + (double)lo[j]; };
         const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
         const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
-        if (y > 0 || Z) {
-            const double cf = (double)a.ay[iay] * k.ylo[y];
-            diag += cf;
-            if (y > 0) off -= cf * val(i - 1);
-        }
-        if (y < a.ny - 1 || Z) {
-            const double cf = (double)a.ay[iay + 1] * k.yhi[y];
-            diag += cf;
-            if (y < a.ny - 1) off -= cf * val(i + 1);
-        }
-        if (x > 0 || Z) {
-            const double cf = (double)a.ax[iax] * k.xlo[x];
-            diag += cf;
-            if (x > 0) off -= cf * val(i - sx);
-        }
-        if (x < a.nx - 1 || Z) {
-            const double cf = (double)a.ax[iax + a.ny] * k.xhi[x];
-            diag += cf;
-            if (x < a.nx - 1) off -= cf * val(i + sx);
-        }
-        if (z > 0 || Z) {
-            const double cf = (double)a.az[i] * k.zlo[z];
-            diag += cf;
-            if (z > 0) off -= cf * val(i - sz);
-        }
-        if (z < a.nz - 1 || Z) {
-            const double cf = (double)a.az[i + sz] * k.zhi[z];
-            diag += cf;
-            if (z < a.nz - 1) off -= cf * val(i + sz);
-        }
+        // every load from a valid index, absent terms zeroed (as stencil())
+        const bool ym = y > 0, yp = y < a.ny - 1, xm = x > 0, xp = x < a.nx - 1, zm = z > 0,
+                   zp = z < a.nz - 1;
+        const double f0 = a.ay[iay], f1 = a.ay[iay + 1], f2 = a.ax[iax], f3 = a.ax[iax + a.ny],
+                     f4 = a.az[i], f5 = a.az[i + sz];
+        const double k0 = k.ylo[y], k1 = k.yhi[y], k2 = k.xlo[x], k3 = k.xhi[x], k4 = k.zlo[z],
+                     k5 = k.zhi[z];
+        const double n0 = val(ym ? i - 1 : i), n1 = val(yp ? i + 1 : i), n2 = val(xm ? i - sx : i),
+                     n3 = val(xp ? i + sx : i), n4 = val(zm ? i - sz : i), n5 = val(zp ? i + sz : i);
+        const double c0 = (ym || Z) ? f0 * k0 : 0.0, c1 = (yp || Z) ? f1 * k1 : 0.0;
+        const double c2 = (xm || Z) ? f2 * k2 : 0.0, c3 = (xp || Z) ? f3 * k3 : 0.0;
+        const double c4 = (zm || Z) ? f4 * k4 : 0.0, c5 = (zp || Z) ? f5 * k5 : 0.0;
+        double diag = 0.0, off = 0.0;
+        diag += c0;
+        diag += c1;
+        diag += c2;
+        diag += c3;
+        diag += c4;
+        diag += c5;
+        off -= c0 * (ym ? n0 : 0.0);
+        off -= c1 * (yp ? n1 : 0.0);
+        off -= c2 * (xm ? n2 : 0.0);
+        off -= c3 * (xp ? n3 : 0.0);
+        off -= c4 * (zm ? n4 : 0.0);
+        off -= c5 * (zp ? n5 : 0.0);
         const double fi = f[i];
         const double ri = diag > 1e-12 ? fi - (diag * val(i) + off) : 0.0;
         r[i] = (float)ri;

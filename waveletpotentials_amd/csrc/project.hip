@@ -118,14 +118,32 @@ __global__ void __launch_bounds__(256) project3d_kernel(Proj3Args p) {
     const float sx = p.cx.s[x], sy = p.cy.s[y];
     const float sxx_syy = sx * sx + sy * sy;
     const int zend = min(z0 + ZB, p.nz);
-    for (int z = z0; z < zend; ++z) {
-        const float sz = p.cz.s[z];
+    // The velocities of all the thread's cells are loaded before any store:
+    // as far as the compiler knows the arrays may alias, so a load inside the
+    // loop would wait behind the previous cell's stores and every cell would
+    // pay the whole memory latency.  Indices past the last cell are clamped to
+    // it (always valid, never used).
+    float pvx[ZB], pvy[ZB], pvz[ZB], psz[ZB];
+#pragma unroll
+    for (int k = 0; k < ZB; ++k) {
+        const I kk = min(k, zend - z0 - 1);
 #if WHHD_PROJ_NT
-        const float vx = __builtin_nontemporal_load(p.xv + ixv), vy = __builtin_nontemporal_load(p.yv + iyv),
-                    vz = __builtin_nontemporal_load(p.zv + izv);
+        pvx[k] = __builtin_nontemporal_load(p.xv + ixv + kk * sxv_z);
+        pvy[k] = __builtin_nontemporal_load(p.yv + iyv + kk * syv_z);
+        pvz[k] = __builtin_nontemporal_load(p.zv + izv + kk * szv_z);
 #else
-        const float vx = p.xv[ixv], vy = p.yv[iyv], vz = p.zv[izv];
+        pvx[k] = p.xv[ixv + kk * sxv_z];
+        pvy[k] = p.yv[iyv + kk * syv_z];
+        pvz[k] = p.zv[izv + kk * szv_z];
 #endif
+        psz[k] = p.cz.s[z0 + kk];
+    }
+#pragma unroll
+    for (int k = 0; k < ZB; ++k) {
+        const int z = z0 + k;
+        if (z >= zend) break;
+        const float sz = psz[k];
+        const float vx = pvx[k], vy = pvy[k], vz = pvz[k];
         float lx = 0.f, ly = 0.f, lz = 0.f;
         if (x > 0 && y > 0 && z > 0) {
             const float inv = 1.f / (sxx_syy + sz * sz);
