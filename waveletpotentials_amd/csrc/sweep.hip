@@ -395,19 +395,30 @@ __global__ void __launch_bounds__(kFftThreads) dst_fft_kernel(T* out, const T* i
     }
     for (int e = tid; e < M + 3 * P; e += kFftThreads) ltab[e] = tab[e];
     __syncthreads();
-    // ---- stage the tile: row slots 0 and M hold f_0 = f_M = 0, slots 1..n the line
+    // ---- stage the tile: row slots 0 and M hold f_0 = f_M = 0, slots 1..n the line.
+    // Every load comes from a valid index (slots outside 1..n and lines past the
+    // end read the tile's first line and store 0): no branch per load.
+    const int64_t b0 = bases[0];
     if (step == 1) {
         for (int l = 0; l < tl; ++l) {
             const int64_t b = bases[l];
-            for (int j = tid; j <= M; j += kFftThreads)
-                rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? in[b + (j - 1)] : T(0);
+            const int64_t bb = b >= 0 ? b : b0;
+#pragma unroll 4
+            for (int j = tid; j <= M; j += kFftThreads) {
+                const T v = in[bb + min(max(j - 1, 0), n - 1)];
+                rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? v : T(0);
+            }
         }
     } else {
         // tl divides the workgroup: each thread keeps one line of the tile
         const int l = tid % tl, jstep = kFftThreads / tl;
         const int64_t b = bases[l];
-        for (int j = tid / tl; j <= M; j += jstep)
-            rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? in[b + (int64_t)(j - 1) * step] : T(0);
+        const int64_t bb = b >= 0 ? b : b0;
+#pragma unroll 8
+        for (int j = tid / tl; j <= M; j += jstep) {
+            const T v = in[bb + (int64_t)min(max(j - 1, 0), n - 1) * step];
+            rows[l * rs + j] = (j >= 1 && j <= n && b >= 0) ? v : T(0);
+        }
     }
     __syncthreads();
     const T* sn = ltab;
