@@ -42,6 +42,11 @@ def kernel_label(name):
         return (f"lp_kernel<{_WT.get(m.group(1), m.group(1))},L={m.group(2)},"
                 f"{'fwd' if m.group(4) == 'true' else 'inv'},"
                 f"{'contiguous' if m.group(5) == 'true' else 'strided'}>")
+    m = re.search(r"line_tile_kernel.*Up(Cdf\d\d)E.*EE([fd])Lb([01])ELb([01])E", name)
+    if m:
+        return (f"line_tile_kernel<{_WT.get(m.group(1), m.group(1))},"
+                f"{'f32' if m.group(2) == 'f' else 'f64'},{'fwd' if m.group(3) == '1' else 'inv'},"
+                f"{'strided' if m.group(4) == '1' else 'contiguous'}>")
     m = re.search(r"unet\d+(conv3_wgrad_kernel|conv3_kernel)I((?:L[ib]\d+E)+)E", name)
     if m:
         args = [v if k == "i" else ("true" if v == "1" else "false")
@@ -53,6 +58,12 @@ def kernel_label(name):
         ns = "unet::" if m.group(1).startswith("unet") else ""
         return ns + m.group(2)
     return name.split("(")[0][:80]
+
+
+def is_library(name):
+    """A kernel of this library: demangled under whhd::, or a mangled whhd symbol
+    (rocprofv3 leaves some long template names mangled)."""
+    return "whhd::" in name or name.startswith("_ZN4whhd")
 
 
 def _one(pattern):
@@ -98,7 +109,7 @@ def summarize(trace_dir, fetch_dir, write_dir):
     kernels = {}
     for name, r in stats.items():
         label = kernel_label(name)
-        row = kernels.setdefault(label, {"calls": 0, "total_ns": 0.0, "library": "whhd::" in name})
+        row = kernels.setdefault(label, {"calls": 0, "total_ns": 0.0, "library": is_library(name)})
         row["calls"] += int(r["Calls"])
         row["total_ns"] += float(r["TotalDurationNs"])
         if name in fetch:
@@ -116,7 +127,7 @@ def summarize(trace_dir, fetch_dir, write_dir):
         sizes[name].add(g)
     by_size = []
     for (name, g), d in durs.items():
-        if "whhd::" not in name or len(sizes[name]) < 2:
+        if not is_library(name) or len(sizes[name]) < 2:
             continue
         row = {"kernel": kernel_label(name), "grid": g, "calls": len(d),
                "avg_us": statistics.mean(d) / 1000.0}
