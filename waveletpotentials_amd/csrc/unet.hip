@@ -91,6 +91,11 @@ __global__ void __launch_bounds__(256) conv3_kernel(ConvSrc s, const bf16* __res
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int Xh = s.X >> 1, Yh = s.Y >> 1;
+    // unrolled so the operand loads of the next chunks are in flight while
+    // the matrix cores work on the current one (a serial chain of load, wait,
+    // MFMA per chunk otherwise: small grids, with few wavefronts, were bound
+    // by that latency)
+#pragma unroll 4
     for (int kc = 0; kc < KP / 32; ++kc) {
         const int k0 = kc * 32 + 8 * g;
         const int tap = k0 / CIN, ci = k0 - tap * CIN;
