@@ -447,8 +447,10 @@ __global__ void down_kernel(float* __restrict__ out, const float* __restrict__ i
     const int64_t X = (int64_t)Xc * F, Y = (int64_t)Yc * F;
     float s = 0.f;
     for (int a = 0; a < F; ++a)
+#pragma unroll 4
         for (int b = 0; b < F; ++b) {
             const float* row = in + (((int64_t)zc * F + a) * X + (int64_t)xc * F + b) * Y + (int64_t)yc * F;
+#pragma unroll 4
             for (int d = 0; d < F; ++d) s += row[d];
         }
     out[i] = s / (float)(F * F * F);
@@ -457,15 +459,15 @@ __global__ void down_kernel(float* __restrict__ out, const float* __restrict__ i
 // Fine cell values by trilinear interpolation of the network output at cell
 // centres (coarse coordinate (i + 1/2) / F - 1/2, clamped at the walls), times
 // rms(f) * L^2 * scale.
+// One thread per fine cell on a (64, 4) block over (y, x) and one z plane per
+// grid row (no 64-bit index division per cell).
 __global__ void up_kernel(float* __restrict__ p, const float* __restrict__ out,
                           const double* __restrict__ stats, float len2, int Zc, int Xc, int Yc,
                           int F) {
-    const int64_t Z = (int64_t)Zc * F, X = (int64_t)Xc * F, Y = (int64_t)Yc * F;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Z * X * Y) return;
-    const int y = (int)(i % Y);
-    const int64_t r = i / Y;
-    const int x = (int)(r % X), z = (int)(r / X);
+    const int X = Xc * F, Y = Yc * F;
+    const int y = blockIdx.x * 64 + threadIdx.x, x = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (y >= Y || x >= X) return;
+    const int64_t i = y + (int64_t)Y * (x + (int64_t)X * z);
     auto axis = [&](int c, int nc, int& c0, int& c1, float& w) {
         float u = ((float)c + 0.5f) / (float)F - 0.5f;
         u = fminf(fmaxf(u, 0.f), (float)(nc - 1));
@@ -958,8 +960,9 @@ WHHD_API int whhd_unet_predict(whhd_unet_t h, float* p0, const float* f, const f
         hipLaunchKernelGGL(unet::output_kernel, unet::g1(h->V[0]), dim3(256), 0, st, p0, h->out,
                            unet::stats(h), len2, h->V[0]);
     else
-        hipLaunchKernelGGL(unet::up_kernel, unet::g1(h->V[0] * h->factor * h->factor * h->factor),
-                           dim3(256), 0, st, p0, h->out, unet::stats(h), len2, h->nz, h->nx, h->ny,
+        hipLaunchKernelGGL(unet::up_kernel,
+                           dim3((h->ny * h->factor + 63) / 64, (h->nx * h->factor + 3) / 4, h->nz * h->factor),
+                           dim3(64, 4), 0, st, p0, h->out, unet::stats(h), len2, h->nz, h->nx, h->ny,
                            h->factor);
     return unet::check_launch("unet_output_kernel");
 }
