@@ -265,7 +265,9 @@ WHHD_API int whhd_mgfrac3d_solve(whhd_mgfrac3d_t h, float* dst, const float* pre
  * ||r|| <= tol ||f|| or after max_cycles V-cycles, otherwise runs 1-4
  * V-cycles (as Solve; as many as the observed convergence rate says reach tol,
  * two before a rate is known, never past max_cycles) on A e = r from e = 0 and
- * adds e to the pair.  pre_v (NULL: zero; may equal dst) starts dst; dst_lo starts at zero.
+ * adds e to the pair; those V-cycles read the finest level's fractions rounded
+ * to FP16 (the FP64 residual, which decides convergence, uses the FP32 ones).
+ * pre_v (NULL: zero; may equal dst) starts dst; dst_lo starts at zero.
  * Writes the V-cycles run and the last relative residual; returns
  * WHHD_NONCONVERGED if tol was not reached.  Synchronizes `stream` once per
  * FP64 residual (16-byte read-back). */

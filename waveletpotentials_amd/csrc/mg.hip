@@ -10,6 +10,7 @@
 // (64, 4) block over (y, x) and one z-plane per grid row.  The handle owns
 // the hierarchy; solves enqueue on the caller's stream without host
 // synchronization (the residual queries read back two FP64 sums).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,6 +58,7 @@ struct Level {
     float *ax = nullptr, *ay = nullptr, *az = nullptr;
     float *hx = nullptr, *hy = nullptr, *hz = nullptr;
     float* kf = nullptr;  // face spacing factors (face_factor_kernel)
+    __half *axh = nullptr, *ayh = nullptr, *azh = nullptr;  // finest level: FP16 copies
 };
 
 struct LevelArgs {
@@ -64,7 +66,17 @@ struct LevelArgs {
     int nx, ny, nz;
     float ex, ey, ez;  // exterior spacing at the walls ('z')
     const float* kf;   // face spacing factors
+    // FP16 copies of the fractions (the finest level inside the tolerance
+    // solve's V-cycles), or null
+    const __half *axh, *ayh, *azh;
 };
+
+// Fraction j of a face array, from the FP16 copy when H16.
+template <bool H16>
+__device__ __forceinline__ float frac_at(const float* f32, const __half* f16, int64_t j) {
+    if constexpr (H16) return __half2float(f16[j]);
+    else return f32[j];
+}
 
 // Spacing factors of the six faces of cell (x, y, z), in stencil()'s order.
 struct FaceK {
@@ -83,7 +95,7 @@ struct FaceK {
 // cell), accumulated in T; the face coefficients are FP32 (the
 // operator every level's kernels share).  Z: walls carry a Dirichlet
 // coefficient with the exterior spacing; otherwise closed.
-template <bool Z, class T, class Src>
+template <bool Z, class T, class Src, bool H16 = false>
 __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int y, int z, int64_t i,
                                         T& diag, T& off) {
     const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
@@ -95,8 +107,9 @@ __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int 
     const int64_t iay = y + (int64_t)x * (a.ny + 1) + (int64_t)z * a.nx * (a.ny + 1);
     const int64_t iax = y + (int64_t)x * a.ny + (int64_t)z * (a.nx + 1) * a.ny;
     const bool ym = y > 0, yp = y < a.ny - 1, xm = x > 0, xp = x < a.nx - 1, zm = z > 0, zp = z < a.nz - 1;
-    const float f0 = a.ay[iay], f1 = a.ay[iay + 1], f2 = a.ax[iax], f3 = a.ax[iax + a.ny], f4 = a.az[i],
-                f5 = a.az[i + sz];
+    const float f0 = frac_at<H16>(a.ay, a.ayh, iay), f1 = frac_at<H16>(a.ay, a.ayh, iay + 1),
+                f2 = frac_at<H16>(a.ax, a.axh, iax), f3 = frac_at<H16>(a.ax, a.axh, iax + a.ny),
+                f4 = frac_at<H16>(a.az, a.azh, i), f5 = frac_at<H16>(a.az, a.azh, i + sz);
     const T n0 = src(ym ? i - 1 : i, ym ? -1 : 0, 0, 0), n1 = src(yp ? i + 1 : i, yp ? 1 : 0, 0, 0);
     const T n2 = src(xm ? i - sx : i, 0, xm ? -1 : 0, 0), n3 = src(xp ? i + sx : i, 0, xp ? 1 : 0, 0);
     const T n4 = src(zm ? i - sz : i, 0, 0, zm ? -1 : 0), n5 = src(zp ? i + sz : i, 0, 0, zp ? 1 : 0);
@@ -127,7 +140,7 @@ __device__ __forceinline__ void stencil(const LevelArgs& a, Src src, int x, int 
 // (SolveAddP1RN, cuProlongate3D.cu:594-616: every cell adds its parent), formed
 // per read as prolong_add_kernel would store it, so the post-smoothing's first
 // sweep needs no separate prolongation pass.
-template <bool Z, bool ZERO = false, bool PRO = false>
+template <bool Z, bool ZERO = false, bool PRO = false, bool H16 = false>
 __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__ src,
                               const float* __restrict__ f, LevelArgs a, float w,
                               const float* __restrict__ vc = nullptr, int cx = 0, int cy = 0) {
@@ -144,7 +157,7 @@ __global__ void jacobi_kernel(float* __restrict__ dst, const float* __restrict__
         else return src[j];
     };
     float diag, off;
-    stencil<Z, float>(a, in, x, y, z, i, diag, off);
+    stencil<Z, float, decltype(in), H16>(a, in, x, y, z, i, diag, off);
     const float v = in(i, 0, 0, 0);
     dst[i] = diag > 1e-12f ? (1.f - w) * v + w * (f[i] - off) / diag : 0.f;
 }
@@ -321,7 +334,7 @@ __global__ void pair_add_kernel(float* __restrict__ hi, float* __restrict__ lo,
 // The last sweep of a refinement cycle fused with the pair update: the
 // correction e = J(src) of each cell (jacobi_kernel's arithmetic) goes
 // straight into hi + lo (pair_add_kernel's TwoSum) and is not stored.
-template <bool Z>
+template <bool Z, bool H16 = false>
 __global__ void jacobi_pair_kernel(float* __restrict__ hi, float* __restrict__ lo,
                                    const float* __restrict__ src, const float* __restrict__ f, LevelArgs a,
                                    float w) {
@@ -331,7 +344,8 @@ __global__ void jacobi_pair_kernel(float* __restrict__ hi, float* __restrict__ l
     if (y >= a.ny || x >= a.nx) return;
     const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
     float diag, off;
-    stencil<Z, float>(a, [&](int64_t j, auto...) { return src[j]; }, x, y, z, i, diag, off);
+    auto rd = [&](int64_t j, auto...) { return src[j]; };
+    stencil<Z, float, decltype(rd), H16>(a, rd, x, y, z, i, diag, off);
     const float e = diag > 1e-12f ? (1.f - w) * src[i] + w * (f[i] - off) / diag : 0.f;
     const float hv = hi[i], t = lo[i] + e;
     const float s = hv + t;
@@ -362,7 +376,7 @@ __global__ void residual_kernel(float* __restrict__ r, const float* __restrict__
 // residual_kernel) in LDS; 64 of its threads then restrict them into the
 // 32 x 2 coarse cells they cover.  Same FP32 values and order as the two
 // separate kernels.
-template <bool Z>
+template <bool Z, bool H16 = false>
 __global__ void __launch_bounds__(512) residual_restrict_kernel(float* __restrict__ dst, const float* __restrict__ v,
                                                                 const float* __restrict__ f, LevelArgs a,
                                                                 int cx, int cy) {
@@ -373,7 +387,8 @@ __global__ void __launch_bounds__(512) residual_restrict_kernel(float* __restric
     if (y < a.ny && x < a.nx && z < a.nz) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
         float diag, off;
-        stencil<Z, float>(a, [&](int64_t j, auto...) { return v[j]; }, x, y, z, i, diag, off);
+        auto rd = [&](int64_t j, auto...) { return v[j]; };
+        stencil<Z, float, decltype(rd), H16>(a, rd, x, y, z, i, diag, off);
         r = diag > 1e-12f ? f[i] - (diag * v[i] + off) : 0.f;
     }
     rs[tz][tx][ty] = r;
@@ -459,6 +474,11 @@ __global__ void restrict_h_kernel(float* __restrict__ dst, const float* __restri
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     dst[i] = src[2 * i] + (2 * i + 1 < nf ? src[2 * i + 1] : 0.f);
+}
+
+__global__ void to_half_kernel(__half* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = __float2half_rn(src[i]);
 }
 
 __global__ void fill_kernel(float* __restrict__ p, float v, int64_t n) {
@@ -771,6 +791,7 @@ struct whhd_mgfrac3d_s {
     whhd::SmallCycle small{};        // the small levels' table (one-workgroup V-cycle)
     size_t small_lds = 0;            // its LDS bytes
     int fuse_from = -1;              // levels from here down run in one workgroup
+    bool fp16_fine = false;          // V-cycles read the finest level's FP16 fractions
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
     std::vector<void*> owned;
@@ -780,8 +801,10 @@ namespace whhd {
 namespace {
 
 LevelArgs args_of(const whhd_mgfrac3d_s* h, const Level& L) {
+    const bool h16 = h->fp16_fine && L.axh;
     return LevelArgs{L.ax, L.ay, L.az, L.hx, L.hy, L.hz, L.d.x, L.d.y, L.d.z,
-                     h->he[0], h->he[1], h->he[2], L.kf};
+                     h->he[0], h->he[1], h->he[2], L.kf,
+                     h16 ? L.axh : nullptr, h16 ? L.ayh : nullptr, h16 ? L.azh : nullptr};
 }
 
 hipError_t launch_status(const char* name) {
@@ -797,15 +820,21 @@ int jacobi(const whhd_mgfrac3d_s* h, const Level& L, float* dst, const float* sr
     const dim3 g = grid_of(L.d.x, L.d.y, L.d.z), b(64, 4);
     const float* vc = C ? C->v : nullptr;
     const int cx = C ? C->d.x : 0, cy = C ? C->d.y : 0;
+#define WHHD_JAC(Zb, H)                                                                                 \
+    do {                                                                                              \
+        if (zero) hipLaunchKernelGGL((jacobi_kernel<Zb, true, false, H>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);  \
+        else if (C) hipLaunchKernelGGL((jacobi_kernel<Zb, false, true, H>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy); \
+        else hipLaunchKernelGGL((jacobi_kernel<Zb, false, false, H>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);      \
+    } while (0)
+    const bool h16 = a.axh != nullptr;
     if (h->bc == 'z') {
-        if (zero) hipLaunchKernelGGL((jacobi_kernel<true, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
-        else if (C) hipLaunchKernelGGL((jacobi_kernel<true, false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
-        else hipLaunchKernelGGL((jacobi_kernel<true, false>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        if (h16) WHHD_JAC(true, true);
+        else WHHD_JAC(true, false);
     } else {
-        if (zero) hipLaunchKernelGGL((jacobi_kernel<false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
-        else if (C) hipLaunchKernelGGL((jacobi_kernel<false, false, true>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
-        else hipLaunchKernelGGL((jacobi_kernel<false, false>), g, b, 0, s, dst, src, f, a, w, vc, cx, cy);
+        if (h16) WHHD_JAC(false, true);
+        else WHHD_JAC(false, false);
     }
+#undef WHHD_JAC
     hipError_t e = launch_status("mg_jacobi_kernel");
     return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi kernel");
 }
@@ -858,12 +887,15 @@ int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t
         }
         if (hi && k == n - 1) {
             const LevelArgs la = args_of(h, L);
-            if (h->bc == 'z')
-                hipLaunchKernelGGL(jacobi_pair_kernel<true>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0, s,
-                                   hi, lo, a, L.f, la, w);
-            else
-                hipLaunchKernelGGL(jacobi_pair_kernel<false>, grid_of(L.d.x, L.d.y, L.d.z), dim3(64, 4), 0,
-                                   s, hi, lo, a, L.f, la, w);
+            const dim3 pg = grid_of(L.d.x, L.d.y, L.d.z), pb(64, 4);
+            const bool h16 = la.axh != nullptr;
+            if (h->bc == 'z') {
+                if (h16) hipLaunchKernelGGL((jacobi_pair_kernel<true, true>), pg, pb, 0, s, hi, lo, a, L.f, la, w);
+                else hipLaunchKernelGGL((jacobi_pair_kernel<true, false>), pg, pb, 0, s, hi, lo, a, L.f, la, w);
+            } else {
+                if (h16) hipLaunchKernelGGL((jacobi_pair_kernel<false, true>), pg, pb, 0, s, hi, lo, a, L.f, la, w);
+                else hipLaunchKernelGGL((jacobi_pair_kernel<false, false>), pg, pb, 0, s, hi, lo, a, L.f, la, w);
+            }
             hipError_t e = launch_status("mg_jacobi_pair_kernel");
             return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg jacobi pair kernel");
         }
@@ -936,10 +968,14 @@ int vcycle_body(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, h
     // residual and its restriction to the coarse rhs in one pass
     const LevelArgs a = args_of(h, L);
     const dim3 rg((L.d.y + 63) / 64, (L.d.x + 3) / 4, (L.d.z + 1) / 2), rb(64, 4, 2);
-    if (h->bc == 'z')
-        hipLaunchKernelGGL(residual_restrict_kernel<true>, rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
-    else
-        hipLaunchKernelGGL(residual_restrict_kernel<false>, rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+    const bool h16 = a.axh != nullptr;
+    if (h->bc == 'z') {
+        if (h16) hipLaunchKernelGGL((residual_restrict_kernel<true, true>), rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+        else hipLaunchKernelGGL((residual_restrict_kernel<true, false>), rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+    } else {
+        if (h16) hipLaunchKernelGGL((residual_restrict_kernel<false, true>), rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+        else hipLaunchKernelGGL((residual_restrict_kernel<false, false>), rg, rb, 0, s, C.f, L.v, L.f, a, C.d.x, C.d.y);
+    }
     hipError_t e = launch_status("mg_residual_restrict_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg residual-restrict kernel");
     TRY(vcycle(h, l + 1, nu1, nu2, w, s, true));
@@ -1043,6 +1079,26 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         last = d;
         d = {half_up(d.x), half_up(d.y), half_up(d.z)};
         min_dim = std::min(d.x, std::min(d.y, d.z));
+    }
+    if (st == WHHD_OK && !h->lv.empty()) {
+        // FP16 copies of the finest level's fractions for the tolerance solve's
+        // V-cycles (its FP64 residual keeps the FP32 fractions)
+        Level& L = h->lv[0];
+        const int64_t n[3] = {(int64_t)(L.d.x + 1) * L.d.y * L.d.z, (int64_t)L.d.x * (L.d.y + 1) * L.d.z,
+                              (int64_t)L.d.x * L.d.y * (L.d.z + 1)};
+        __half** dst[3] = {&L.axh, &L.ayh, &L.azh};
+        const float* src[3] = {L.ax, L.ay, L.az};
+        for (int c = 0; c < 3 && st == WHHD_OK; ++c) {
+            *dst[c] = (__half*)alloc((size_t)n[c] * sizeof(__half));
+            if (!*dst[c]) {
+                st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
+                break;
+            }
+            hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)((n[c] + 255) / 256)), dim3(256), 0, s, *dst[c],
+                               src[c], n[c]);
+            hipError_t e = launch_status("mg_to_half_kernel");
+            if (e != hipSuccess) st = hip_fail(e, "mgfrac3d_create: FP16 fractions");
+        }
     }
     for (Level& L : h->lv) {
         if (st != WHHD_OK) break;
@@ -1179,6 +1235,12 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: bad iteration limits");
     Level& L0 = h->lv[0];
     const int64_t n = cells(h->dim);
+    // the V-cycles read the finest level's FP16 fractions while this solve runs
+    struct Fp16Fine {
+        whhd_mgfrac3d_s* h;
+        ~Fp16Fine() { h->fp16_fine = false; }
+    } fp16_guard{h};
+    h->fp16_fine = true;
     if (pre_v && pre_v != dst) TRY(copy(dst, pre_v, n, s));
     else if (!pre_v) TRY(fill(dst, 0.f, n, s));
     TRY(fill(lo, 0.f, n, s));
