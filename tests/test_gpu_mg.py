@@ -113,3 +113,26 @@ def test_solve_to_tolerance_passes_the_fp32_floor(whhd, orc):
     # cycle for a tolerance above that floor
     c2, res2, ok2 = mg.SolveToTolerance(hi, lo, hi, df, 1e-5, 10)
     assert ok2 and c2 == 0 and res2 < 1e-5
+
+
+def test_plain_solve_unchanged_after_tolerance_solve(whhd, orc):
+    """The tolerance solve's V-cycles read FP16 copies of the finest level's
+    fractions; afterwards the plain Solve (the reference's operator, FP32
+    fractions) on the same handle gives bit-identical results to a fresh
+    handle's, and the tolerance solve itself reaches its tolerance.  (A cubic
+    grid: with unequal spacings such as 1/32, 1/24, 1/16 the V-cycle itself
+    diverges, in the oracle as on the GPU.)"""
+    n = (32, 32, 32)
+    fr, f, h = problem(orc, n, 20260077, True)
+    df = cuda(f)
+    used = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    fresh = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    hi, lo = torch.empty_like(df), torch.empty_like(df)
+    cycles, res, ok = used.SolveToTolerance(hi, lo, None, df, 1e-6, 200)
+    assert ok and res <= 1e-6 and cycles > 0
+    a, b = torch.empty_like(df), torch.empty_like(df)
+    used.Solve(a, None, df, 3, 2, 2, 0.8)
+    fresh.Solve(b, None, df, 3, 2, 2, 0.8)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
