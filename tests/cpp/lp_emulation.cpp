@@ -59,7 +59,8 @@ struct HostIO2 {
             *fetched_to = B;
         }
     }
-    template <int A, int B>
+    int* tail_step;    // next step of the aligned final stores
+    template <int A, int B, int AREA = 0>
     void unpack(const Pend& p, float* x) const {
         if constexpr (A < B) {
             if (p.a != A || p.b != B || A != *unpacked_to) ++*bad;
@@ -70,6 +71,18 @@ struct HostIO2 {
     template <int START, int CNT>
     void store_run(const float* vals) const {
         for (int v = 0; v < CNT; ++v) dst[START + v] = vals[v];
+    }
+    // the aligned final stores take the run [32B, 32B + 32) at step B and the
+    // sample N - 1 at the last step, in order (the wavefront-level storing is
+    // checked by lp_tail_model)
+    template <int N, int B>
+    void store_tail(const float* vals) const {
+        constexpr int K = whhd::lp::tail::steps<N>();
+        if (B != *tail_step) ++*bad;
+        *tail_step = B + 1;
+        if constexpr (B == K) dst[N - 1] = vals[0];
+        else
+            for (int v = 0; v < 32; ++v) dst[32 * B + v] = vals[v];
     }
 };
 
@@ -82,11 +95,13 @@ int run(float* dst, const float* src, int64_t lines, int64_t stride, bool fwd) {
             if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
             else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
         } else {
-            int f = 0, u = 0, bad = 0;
-            HostIO2 io{src + j * stride, dst + j * stride, n, &f, &u, &bad};
+            int f = 0, u = 0, bad = 0, t = 0;
+            HostIO2 io{src + j * stride, dst + j * stride, n, &f, &u, &bad, &t};
             if (fwd) whhd::lp::forward_line_lp<W, LEV, EXT>(io);
             else whhd::lp::inverse_line_lp<W, LEV, EXT>(io);
             if (bad || f != n || u != n) return -2;  // out of order, gaps, or not the whole line
+            const int steps = !fwd && whhd::lp::tail::applies<n>() ? whhd::lp::tail::steps<n>() + 1 : 0;
+            if (t != steps) return -2;                // aligned final stores incomplete
         }
     }
     return 0;
@@ -124,4 +139,63 @@ extern "C" int lp_emulate(float* dst, const float* src, long long lines, int n, 
     case -1: return by_wavelet<-1>(dst, src, lines, stride, levels, wt, fwd != 0);
     default: return -1;
     }
+}
+
+// Wavefront-level model of the inverse's aligned final stores (lp::tail), on
+// the same index functions as the kernel: 64 lines of n samples, the first
+// `lines_here` of which exist; sample i of line l is l * 1000 + i.  Each step
+// first writes every lane's run to the three LDS areas, then every lane stores
+// its rounds of 16-byte pieces (and, at the last step of a partial wavefront,
+// the last line's lane its final samples one by one).  out (64 * n floats)
+// receives the stores.  Returns 0, or the number of violations: an LDS word
+// outside the areas, a store outside the existing lines, a sample stored
+// twice or never, or a wrong value.
+extern "C" int lp_tail_model(int n, int lines_here, float* out) {
+    namespace T = whhd::lp::tail;
+    if (n % 32 != 1 || n < 33 || lines_here < 1 || lines_here > 64) return -1;
+    const int K = (n - 1) / 32;
+    static float lds[T::kAreas * T::kArea];
+    int* count = new int[64 * n]();
+    int bad = 0;
+    auto put = [&](int l, int i, float v) {
+        if (l < 0 || l >= lines_here || i < 0 || l * n + i >= lines_here * n) { ++bad; return; }
+        ++count[l * n + i];
+        out[l * n + i] = v;
+    };
+    auto get = [&](int w) -> float {
+        if (w < 0 || w >= T::kAreas * T::kArea) { ++bad; return 0.f; }
+        return lds[w];
+    };
+    for (int B = 0; B <= K; ++B) {
+        const bool fin = B == K;
+        float* cur = lds + T::cur_area(B) * T::kArea;
+        for (int lane = 0; lane < 64; ++lane) {
+            if (fin) cur[lane] = (float)(lane * 1000 + n - 1);
+            else
+                for (int v = 0; v < 32; ++v) cur[v * T::kRow + lane] = (float)(lane * 1000 + 32 * B + v);
+        }
+        if (B == 0) continue;
+        const int lone = T::lone(fin, lines_here);
+        for (int lane = 0; lane < 64; ++lane) {
+            const int q = lane & 7;
+            for (int r = 0; r < T::rounds(fin); ++r) {
+                const int l = T::line(lane, r), j0 = T::j0(l, r);
+                float w[4];
+                for (int e = 0; e < 4; ++e) w[e] = get(T::word(fin, B, j0 + 4 * q + e, l));
+                if (!T::stores(fin, r, lane, l, lines_here, lone)) continue;
+                // a piece runs on in memory from line l into line l + 1
+                for (int e = 0; e < 4; ++e) {
+                    const int i = T::first(B, j0, q) + e;
+                    put(l + i / n, i % n, w[e]);
+                }
+            }
+            if (fin && lane == lone)
+                for (int j = T::lead(lane); j <= 32; ++j) put(lane, T::first(B, j, 0), get(T::word(true, B, j, lane)));
+        }
+    }
+    for (int l = 0; l < lines_here; ++l)
+        for (int i = 0; i < n; ++i)
+            if (count[l * n + i] != 1 || out[l * n + i] != (float)(l * 1000 + i)) ++bad;
+    delete[] count;
+    return bad;
 }

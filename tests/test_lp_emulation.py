@@ -52,3 +52,17 @@ def test_lp_algorithm_matches_oracle(emu, orc, wt, name, bc, ext, lev, fwd):
         assert emu.lp_emulate(d32.ctypes.data, s32.ctypes.data, lines, n, n, lev, wt, fwd, look) == 0
         assert np.isfinite(d32).all(), "a coefficient was never written"
         assert np.abs(d32 - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n", [33, 65, 129, 257])
+def test_aligned_final_stores_cover_each_sample_once(emu, n):
+    """The inverse's aligned final stores for lines of 2^L + 1 samples (whole
+    128-byte lines, the line boundaries stored at the last step): on the
+    kernel's own index functions, every sample of every existing line of a
+    wavefront is stored exactly once with its value, nothing is stored past
+    the last existing line, and every LDS word read lies in the three staging
+    areas, for full and partial wavefronts."""
+    emu.lp_tail_model.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    for lines_here in (64, 63, 62, 33, 32, 31, 30, 17, 2, 1):
+        out = np.full(64 * n, np.nan, dtype=np.float32)
+        assert emu.lp_tail_model(n, lines_here, out.ctypes.data) == 0, lines_here

@@ -105,6 +105,60 @@ WHHD_DEV void static_for(F&& f) {
 #endif
 constexpr int kBlock = WHHD_KBLOCK;  // pairs written per block
 
+// ---- aligned final stores of the inverse (contiguous lines) ----------------
+// Lines of N = 2^LEV + 1 samples (N = 1 mod 32), 64 lines per wavefront
+// from a multiple of 64 lines: line l starts l mod 32 floats past a
+// 128-byte boundary, so its 128-byte lines begin at the samples i = lead(l)
+// mod 32.  The inverse's last level produces the run [32B, 32B + 32) at
+// step B < K = (N - 1) / 32 and the single sample N - 1 at step K.  Every
+// run goes to LDS; step B >= 1 stores, for every line, the whole 128-byte
+// line [32(B-1) + lead, 32B + lead) from the previous run (rows lead..31)
+// and this one (rows 0..lead-1).  Step 0's run stays in area 2 to the end:
+// its first lead samples share a 128-byte line with the previous line's
+// last samples.  Step K stores [32(K-1) + lead, N) of line l together with
+// [0, lead - 1) of line l + 1; a line with lead 0 stores its run [32(K-1),
+// 32K) and, in a ninth round, sample N - 1 with [0, 31) of line l + 1.  So
+// no 128-byte line is written in two parts at different times.  Lines 63
+// and 0 of two wavefronts never share a 128-byte line (64 N = 0 mod 32); in
+// a partial last wavefront the last line's own lane stores its final
+// samples one by one, since its last 128-byte line would reach into a line
+// that does not exist.  The runs alternate between areas 0 and 1, and the
+// unpack of step B uses area B mod 2, which holds no run still needed.
+// Each step stores 16-byte pieces: lane -> piece lane mod 8 of line
+// line(lane, r) in round r.
+namespace tail {
+constexpr int kRow = 64 + 1;           // LDS row: one value of each of the 64 lines
+constexpr int kArea = 32 * kRow;       // one run of 32 samples
+constexpr int kAreas = 3;
+template <int N>
+constexpr bool applies() { return N % 32 == 1 && N >= 33; }
+template <int N>
+constexpr int steps() { return (N - 1) / 32; }
+constexpr int lead(int l) { return (32 - (l & 31)) & 31; }
+constexpr int cur_area(int B) { return B == 0 ? 2 : B % 2; }
+constexpr int prv_area(int B) { return B == 1 ? 2 : (B + 1) % 2; }
+constexpr int rounds(bool fin) { return fin ? 9 : 8; }
+// round 8 (final step): lines 0 and 32 by lanes 0-15; the other lanes repeat them and do not store
+constexpr int line(int lane, int r) { return r < 8 ? (lane >> 3) + 8 * r : ((lane >> 3) & 1) * 32; }
+constexpr int j0(int l, int r) { return r < 8 ? lead(l) : 32; }
+// sample of line l that element j of its 128-byte line at step B is (j >= 33
+// at the final step: sample j - 33 of line l + 1)
+constexpr int first(int B, int j0, int q) { return 32 * (B - 1) + j0 + 4 * q; }
+// LDS word (from the wavefront's staging base) holding element j of line l's
+// 128-byte line at step B
+// (written as j * kRow + l plus a constant of the part j falls in)
+constexpr int word(bool fin, int B, int j, int l) {
+    return j * kRow + l +
+           (j < 32 ? prv_area(B) * kArea
+                   : (!fin || j == 32) ? cur_area(B) * kArea - kArea : 2 * kArea - 33 * kRow + 1);
+}
+// line whose lane stores its final samples itself (64: none)
+constexpr int lone(bool fin, int lines_here) { return fin && lines_here < 64 ? lines_here - 1 : 64; }
+constexpr bool stores(bool fin, int r, int lane, int l, int lines_here, int lone) {
+    return fin ? (r < 8 || lane < 16) && l < lines_here && l != lone : l < lines_here;
+}
+}  // namespace tail
+
 #ifndef WHHD_HOST_EMULATION
 // ---- I/O of one lane's line ----------------------------------------------
 // Strided lines (x and z axes): sample i of the lane's line is at
@@ -152,6 +206,7 @@ struct __attribute__((packed, aligned(4))) Quad {
 };
 constexpr int kStageRun = 32;       // longest run written at once
 constexpr int kStageRow = 64 + 1;   // LDS row: one value of each of the 64 lines
+static_assert(kStageRow == tail::kRow && kStageRun * kStageRow == tail::kArea, "staging layout");
 
 struct ContigIO {
     static constexpr int kLook = 0;            // (two-phase: one block ahead)
@@ -218,29 +273,75 @@ struct ContigIO {
         }
     }
     template <int A, int NQ, int K>
-    WHHD_DEV void unpack_chunks(const Pend& p, float* x) const {
+    WHHD_DEV void unpack_chunks(const Pend& p, float* x, float* st) const {
         if constexpr (K < NQ) {
             constexpr int c = chunk<NQ, K>();
 #pragma unroll
             for (int r = 0; r < c; ++r)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    stage[(4 * (lane % c) + e) * kStageRow + lane / c + r * (64 / c)] = p.q[K + r].v[e];
+                    st[(4 * (lane % c) + e) * kStageRow + lane / c + r * (64 / c)] = p.q[K + r].v[e];
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
 #pragma unroll
-            for (int s2 = 0; s2 < 4 * c; ++s2) x[A + 4 * K + s2] = stage[s2 * kStageRow + lane];
+            for (int s2 = 0; s2 < 4 * c; ++s2) x[A + 4 * K + s2] = st[s2 * kStageRow + lane];
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
-            unpack_chunks<A, NQ, K + c>(p, x);
+            unpack_chunks<A, NQ, K + c>(p, x, st);
         }
     }
-    template <int A, int B>
+    // AREA: which staging area passes the pieces (0 unless the aligned final
+    // stores below keep a run in area 0 or 1)
+    template <int A, int B, int AREA = 0>
     WHHD_DEV void unpack(const Pend& p, float* x) const {
         if constexpr (A < B) {
             constexpr int NQ = (B & ~3) / 4 - A / 4;
-            unpack_chunks<A, NQ, 0>(p, x);
+            unpack_chunks<A, NQ, 0>(p, x, area<AREA>());
             if constexpr ((B & 3) != 0) x[B - 1] = p.tail;
+        }
+    }
+    template <int K>
+    WHHD_DEV float* area() const { return stage + K * tail::kArea; }
+
+    // Aligned final stores of the inverse (lp::tail): the run of step B goes
+    // to LDS and whole 128-byte lines are stored from there.
+    template <int N, int B>
+    WHHD_DEV void store_tail(const float* vals) const {
+        constexpr bool FIN = B == tail::steps<N>();
+        float* cur = stage + tail::cur_area(B) * tail::kArea;
+        if constexpr (FIN) {
+            cur[lane] = vals[0];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 32; ++v) cur[v * kStageRow + lane] = vals[v];
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        if constexpr (B > 0) {
+            const int q = lane & 7;
+            const int lone = tail::lone(FIN, lines_here);
+#pragma unroll
+            for (int r = 0; r < tail::rounds(FIN); ++r) {
+                const int l = tail::line(lane, r), j0 = tail::j0(l, r);
+                U4 w;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    w[e] = __builtin_bit_cast(uint32_t, stage[tail::word(FIN, B, j0 + 4 * q + e, l)]);
+                const uint32_t off = tail::stores(FIN, r, lane, l, lines_here, lone)
+                                         ? first_off + (uint32_t)((l * N + tail::first(B, j0, q)) * 4)
+                                         : kOutOfRange;
+                __builtin_amdgcn_raw_buffer_store_b128(w, rd, off, 0, 0);
+            }
+            if constexpr (FIN) {
+                if (lane == lone) {
+                    for (int j = tail::lead(lane); j <= 32; ++j)
+                        __builtin_amdgcn_raw_buffer_store_b32(
+                            __builtin_bit_cast(uint32_t, stage[tail::word(true, B, j, lane)]), rd, voff_st,
+                            tail::first(B, j, 0) * 4, 0);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
         }
     }
     // Samples [START, START+CNT) of every lane's line.  Whole 16-byte pieces
@@ -488,7 +589,8 @@ WHHD_DEV void inv_level(const IO& io, typename IO::Pend& pend, float* x, const f
             constexpr int prev = b > 0 ? inv_need2<H, EXT, N, M>(l, b - 1)
                                        : (l == 2 ? inv_need2<H, EXT, N, M>(1, 0)
                                                  : inv_need2<H, EXT, N, M>(l - 1, (((1 << (l - 1)) + EXT + kBlock - 1) / kBlock) - 1));
-            io.template unpack<prev, inv_need2<H, EXT, N, M>(l, b)>(pend, x);
+            // the last level's aligned stores keep runs in areas 0 and 1
+            io.template unpack<prev, inv_need2<H, EXT, N, M>(l, b), (LAST && tail::applies<N>()) ? b % 2 : 0>(pend, x);
             io.template fetch<inv_need2<H, EXT, N, M>(l, b), inv_need2_next<H, EXT, N, M, LEV>(l, b)>(pend);
         } else {
             issue<inv_front_before<H, EXT, N, M, IO::kLook>(l, b), inv_front<H, EXT, N, M, IO::kLook>(l, b)>(io, x);
@@ -528,7 +630,10 @@ WHHD_DEV void inv_level(const IO& io, typename IO::Pend& pend, float* x, const f
                 else out[2 * k + 1] = D[j];
             }
         });
-        if constexpr (LAST && ns > 0) io.template store_run<2 * b0, ns>(run);
+        if constexpr (LAST && ns > 0) {
+            if constexpr (IO::kTwoPhase && tail::applies<N>()) io.template store_tail<N, b>(run);
+            else io.template store_run<2 * b0, ns>(run);
+        }
     });
 }
 
@@ -617,13 +722,15 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void*)dst, 0, bytes, kBufWord3);
     if constexpr (CONTIG) {
         // contiguous lines are numbered in memory order: line li starts at li * n
-        // one kStageRun x kStageRow staging area per wavefront (dynamic LDS)
+        // one staging area of kStageRun x kStageRow per wavefront (dynamic
+        // LDS), three for the inverse's aligned stores
         extern __shared__ float lp_stage[];
+        constexpr int kWaveStage = (!FWD && tail::applies<n>() ? tail::kAreas : 1) * kStageRun * kStageRow;
         const int64_t here = g.total_lines - wave_first;
         const uint32_t voff = (uint32_t)(l * n * 4);
         ContigIO io{src + l * n, rs, rd, valid ? voff : kOutOfRange,
                     (uint32_t)(wave_first * n * 4), (int)(here < 64 ? here : 64), n,
-                    lp_stage + (threadIdx.x >> 6) * (kStageRun * kStageRow), lane_};
+                    lp_stage + (threadIdx.x >> 6) * kWaveStage, lane_};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     } else {
@@ -644,7 +751,10 @@ bool lp_launch_one(int levels, float* dst, const float* src, const LpGeom& g, in
     const dim3 block(64 * waves);
     const int64_t lines_per_block = 64 * waves;
     const dim3 grid((unsigned)((g.total_lines + lines_per_block - 1) / lines_per_block), (unsigned)nb);
-    const size_t lds = CONTIG ? (size_t)waves * kStageRun * kStageRow * sizeof(float) : 0;
+    // the inverse of lines of 2^levels + 1 samples stores through three runs
+    const int n = (1 << levels) + EXT;
+    const int rows = !FWD && n % 32 == 1 && n >= 33 ? tail::kAreas * kStageRun : kStageRun;
+    const size_t lds = CONTIG ? (size_t)waves * rows * kStageRow * sizeof(float) : 0;
     switch (levels) {
     case 5: hipLaunchKernelGGL((lp_kernel<W, 5, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
     case 6: hipLaunchKernelGGL((lp_kernel<W, 6, EXT, FWD, CONTIG>), grid, block, lds, s, dst, src, g); return true;
