@@ -207,6 +207,8 @@ struct __attribute__((packed, aligned(4))) Quad {
 constexpr int kStageRun = 32;       // longest run written at once
 constexpr int kStageRow = 64 + 1;   // LDS row: one value of each of the 64 lines
 static_assert(kStageRow == tail::kRow && kStageRun * kStageRow == tail::kArea, "staging layout");
+static_assert(64 * 32 <= kStageRun * kStageRow && (kStageRun * kStageRow) % 4 == 0,
+              "a staging area holds 64 rows of 32 floats, 16-byte aligned");
 
 struct ContigIO {
     static constexpr int kLook = 0;            // (two-phase: one block ahead)
@@ -272,19 +274,30 @@ struct ContigIO {
             WHHD_SCHED_BARRIER();
         }
     }
+    // Pieces pass through LDS as rows of 32 floats, one row per line, with
+    // the 16-byte piece j of line l at quad j ^ (l mod 8): a piece is one
+    // 16-byte LDS access, and the 8 lanes of a group (8 lines read by their
+    // own lanes, or 8 pieces of one line written) reach 8 different quads,
+    // so no bank is hit twice.  A staging area (32 x 65 floats) holds the 64 rows.
+    static WHHD_DEV int quad(int l, int j) { return 32 * l + 4 * (j ^ (l & 7)); }
     template <int A, int NQ, int K>
     WHHD_DEV void unpack_chunks(const Pend& p, float* x, float* st) const {
         if constexpr (K < NQ) {
             constexpr int c = chunk<NQ, K>();
+            // piece lane mod c of line lane / c + r * (64 / c); lines 64 / c
+            // apart share l mod 8, so one address serves every r
+            const int w0 = quad(lane / c, lane % c);
 #pragma unroll
             for (int r = 0; r < c; ++r)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    st[(4 * (lane % c) + e) * kStageRow + lane / c + r * (64 / c)] = p.q[K + r].v[e];
+                *(U4*)(st + w0 + 32 * r * (64 / c)) = __builtin_bit_cast(U4, p.q[K + r]);
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
 #pragma unroll
-            for (int s2 = 0; s2 < 4 * c; ++s2) x[A + 4 * K + s2] = st[s2 * kStageRow + lane];
+            for (int j = 0; j < c; ++j) {
+                const Quad v = __builtin_bit_cast(Quad, *(const U4*)(st + quad(lane, j)));
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[A + 4 * K + 4 * j + e] = v.v[e];
+            }
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
             unpack_chunks<A, NQ, K + c>(p, x, st);
@@ -359,16 +372,21 @@ struct ContigIO {
             constexpr int LPI = 64 / NQ;               // lines per pass
             static_assert(64 % NQ == 0, "pieces per line must divide 64");
 #pragma unroll
-            for (int v = 0; v < CQ; ++v) stage[v * kStageRow + lane] = vals[v];
+            for (int j = 0; j < NQ; ++j) {
+                Quad v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v.v[e] = vals[4 * j + e];
+                *(U4*)(stage + quad(lane, j)) = __builtin_bit_cast(U4, v);
+            }
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
             const int q = lane % NQ;
+            // lines LPI apart share l mod 8: one LDS address for every r
+            const int w0 = quad(lane / NQ, q);
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 const int l = lane / NQ + r * LPI;
-                U4 w;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) w[e] = __builtin_bit_cast(uint32_t, stage[(4 * q + e) * kStageRow + l]);
+                const U4 w = *(const U4*)(stage + w0 + 32 * r * LPI);
                 const uint32_t off = l < lines_here ? first_off + (uint32_t)((l * n + START + 4 * q) * 4)
                                                     : kOutOfRange;
                 __builtin_amdgcn_raw_buffer_store_b128(w, rd, off, 0, 0);
@@ -724,7 +742,7 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
         // contiguous lines are numbered in memory order: line li starts at li * n
         // one staging area of kStageRun x kStageRow per wavefront (dynamic
         // LDS), three for the inverse's aligned stores
-        extern __shared__ float lp_stage[];
+        extern __shared__ __attribute__((aligned(16))) float lp_stage[];
         constexpr int kWaveStage = (!FWD && tail::applies<n>() ? tail::kAreas : 1) * kStageRun * kStageRow;
         const int64_t here = g.total_lines - wave_first;
         const uint32_t voff = (uint32_t)(l * n * 4);
