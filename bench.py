@@ -318,6 +318,7 @@ def bench_batch(args):
           file=sys.stderr, flush=True)
     b.Capture()
     b.Autotune(reps=3)
+    calibrate()
     for _ in range(args.warmup):
         b.Replay()
     torch.cuda.synchronize()
