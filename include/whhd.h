@@ -275,6 +275,23 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
                                      const float* pre_v, const float* f, int max_cycles,
                                      double tol, int nu1, int nu2, float weight, int* cycles,
                                      double* rel, void* stream);
+/* Solve to a relative residual by conjugate gradients preconditioned by one
+ * V-cycle (no reference counterpart; the same V-cycle as Solve, nu1 / nu2
+ * sweeps, on the residual from zero, reading the finest level's fractions
+ * rounded to FP16).  The iterate is the pair dst + dst_lo as in
+ * whhd_mgfrac3d_solve_tol; each iteration forms r = f - A (dst + dst_lo) in
+ * FP64, stops when ||r|| <= tol ||f|| or after max_iters iterations, applies
+ * the V-cycle z = M r, and updates the direction p = z + beta p, A p (FP32
+ * fractions) and the pair += alpha p with beta, alpha kept on the device.
+ * Needs fewer V-cycles than the refinement for the same tolerance (a
+ * symmetric V-cycle, nu1 = nu2, is what conjugate gradients assume).
+ * pre_v (NULL: zero; may equal dst) starts dst.  Writes the iterations (one
+ * V-cycle each) and the last relative residual; returns WHHD_NONCONVERGED if
+ * tol was not reached.  Synchronizes `stream` once per iteration. */
+WHHD_API int whhd_mgfrac3d_solve_pcg(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
+                                     const float* pre_v, const float* f, int max_iters,
+                                     double tol, int nu1, int nu2, float weight, int* iters,
+                                     double* rel, void* stream);
 /* ||f - A v||_2 / ||f||_2 with the sums in FP64 (CuWeightedJacobi3D::GetFracRhs,
  * WaveletPotentials/cuWeightedJacobi3D.cu:1384-1407).  Synchronizes `stream`. */
 WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f, double* rel,

@@ -136,3 +136,62 @@ def test_plain_solve_unchanged_after_tolerance_solve(whhd, orc):
     torch.cuda.synchronize()
     assert torch.equal(a, b)
 
+
+def test_pcg_reaches_tolerance_in_fewer_vcycles(whhd, orc):
+    """Conjugate gradients with one V-cycle as the preconditioner on the
+    128x64x64 sphere problem: reaches 1e-8 (the two-float iterate) in fewer
+    V-cycles than the refinement needs, and the oracle's FP64 residual of the
+    returned pair agrees with the one it reports; a warm start from the
+    answer's high part needs no iteration for a tolerance above the FP32
+    floor."""
+    p = I.batch_problems(64)[1]
+    n, h = p["dims"], p["dx"]
+    ls = orc.levelset_spheres(n, h, p["centers"], p["radii"])
+    fr = orc.frac_from_ls(ls, h)
+    q = I.spectral_potential3d(n, p["kmax"], p["seed"], p["extent"])
+    u = I.curl3d(*q, h)
+    au = [a * b for a, b in zip(fr, u)]
+    f = -((au[0][:, 1:, :] - au[0][:, :-1, :]) / h[0] + (au[1][:, :, 1:] - au[1][:, :, :-1]) / h[1]
+          + (au[2][1:] - au[2][:-1]) / h[2])
+    mg = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    df = cuda(f)
+    hi, lo = torch.empty_like(df), torch.empty_like(df)
+    c_ref, r_ref, ok_ref = mg.SolveToTolerance(hi, lo, None, df, 1e-8, 200)
+    it, res, ok = mg.SolvePCG(hi, lo, None, df, 1e-8, 200)
+    assert ok_ref and ok and res <= 1e-8
+    assert it < c_ref / 2, (it, c_ref)
+    torch.cuda.synchronize()
+    pair = hi.double().cpu().numpy() + lo.double().cpu().numpy()
+    fr32 = [a.astype(np.float32).astype(np.float64) for a in fr]
+    r_orc = orc.mg_rel_residual(pair, df.double().cpu().numpy(), *fr32, h, "n")
+    assert abs(r_orc - res) <= 0.05 * res + 1e-12
+    it2, res2, ok2 = mg.SolvePCG(hi, lo, hi, df, 1e-5, 10)
+    assert ok2 and it2 == 0 and res2 < 1e-5
+
+
+@pytest.mark.parametrize("bc", ["d", "z"])
+def test_pcg_wall_conditions(whhd, orc, bc):
+    """The walls' Dirichlet variants: the conjugate-gradient solve reaches 1e-7
+    and its reported residual matches the handle's own FP64 residual of the
+    high part to within the pair's low part."""
+    n = (64, 64, 64)
+    fr, f, h = problem(orc, n, 11, True)
+    mg = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, bc)
+    df = cuda(f)
+    hi, lo = torch.empty_like(df), torch.empty_like(df)
+    it, res, ok = mg.SolvePCG(hi, lo, None, df, 1e-7, 100)
+    assert ok and res <= 1e-7 and 0 < it <= 30
+    total = (hi.double() + lo.double()).float()
+    assert mg.GetResdualError(total, df) < 1e-5
+
+
+def test_pcg_rejects_bad_arguments(whhd, orc):
+    n = (16, 16, 16)
+    fr, f, h = problem(orc, n, 3, True)
+    mg = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    df = cuda(f)
+    hi = torch.empty_like(df)
+    with pytest.raises(Exception):
+        mg.SolvePCG(hi, hi, None, df, 1e-6, 10)   # dst and dst_lo alias
+    with pytest.raises(Exception):
+        mg.SolvePCG(hi, torch.empty_like(df), None, df, 1e-6, -1)

@@ -78,6 +78,8 @@ SIGNATURES = {
     "whhd_mgfrac3d_solve": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.c_float, _P]),
     "whhd_mgfrac3d_solve_tol": (_I, [_P, _P, _P, _P, _P, _I, _D, _I, _I, ctypes.c_float, _P, _P,
                                      _P]),
+    "whhd_mgfrac3d_solve_pcg": (_I, [_P, _P, _P, _P, _P, _I, _D, _I, _I, ctypes.c_float, _P, _P,
+                                     _P]),
     "whhd_mgfrac3d_residual": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "whhd_mgfrac3d_destroy": (None, [_P]),
     "whhd_unet_create": (_I, [_I, _I, _I, _I, ctypes.c_ulonglong, ctypes.POINTER(_P)]),

@@ -521,6 +521,21 @@ class CuMultigridFrac3D(_Handle):
             check(st, "SolveToTolerance")
         return cyc.value, rel.value, st == 0
 
+    def SolvePCG(self, dst, dst_lo, pre_v, f, tol=1e-6, max_iters=100, nu1=2, nu2=2, weight=0.8):
+        """Conjugate gradients preconditioned by one V-cycle until
+        ||f - A (dst + dst_lo)|| <= tol ||f|| (no reference counterpart).
+        Returns (iterations, residual, converged)."""
+        pv = None if pre_v is None else _ptr(pre_v, numel=self._n, name="pre_v")
+        it, rel = ctypes.c_int(0), ctypes.c_double(0.0)
+        st = lib().whhd_mgfrac3d_solve_pcg(self._h, _ptr(dst, numel=self._n, name="dst"),
+                                           _ptr(dst_lo, numel=self._n, name="dst_lo"), pv,
+                                           _ptr(f, numel=self._n, name="f"), int(max_iters),
+                                           float(tol), int(nu1), int(nu2), float(weight),
+                                           ctypes.byref(it), ctypes.byref(rel), _stream())
+        if st not in (0, 6):
+            check(st, "SolvePCG")
+        return it.value, rel.value, st == 0
+
     def GetResdualError(self, dst, f):
         """||f - A dst|| / ||f|| in FP64 (the reference's GetResdualError / GetFracRhs)."""
         rel = ctypes.c_double(0.0)

@@ -550,6 +550,106 @@ __global__ void __launch_bounds__(1024) norm_final_kernel(double* __restrict__ o
     }
 }
 
+// ---- preconditioned conjugate gradients ------------------------------------
+// Scalars stay on the device: coef[0] = r.z of the current iteration, coef[1]
+// = beta, coef[2] = alpha.
+
+// Sum of a 1024-thread block's values (FP64), valid in thread 0.
+__device__ double block_sum_1024(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ double l[16];
+    if ((threadIdx.x & 63) == 0) l[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 16; ++k) t += l[k];
+    return t;
+}
+
+// part[block] = the block's share of sum a[i] * b[i] in FP64.
+__global__ void dot_partial_kernel(double* __restrict__ part, const float* __restrict__ a,
+                                   const float* __restrict__ b, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double l[4];
+    if ((threadIdx.x & 63) == 0) l[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = l[0] + l[1] + l[2] + l[3];
+}
+
+// beta = (r.z) / (previous r.z), 0 at the first iteration; r.z kept in coef[0].
+__global__ void __launch_bounds__(1024) pcg_beta_kernel(double* __restrict__ coef,
+                                                        const double* __restrict__ part, int nparts,
+                                                        int first) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    const double rz = block_sum_1024(s);
+    if (threadIdx.x == 0) {
+        coef[1] = first || !(coef[0] != 0.0) ? 0.0 : rz / coef[0];
+        coef[0] = rz;
+    }
+}
+
+// alpha = (r.z) / (p.Ap); 0 when p.Ap is not positive (a zero direction).
+__global__ void __launch_bounds__(1024) pcg_alpha_kernel(double* __restrict__ coef,
+                                                         const double* __restrict__ part, int nparts) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    const double pap = block_sum_1024(s);
+    if (threadIdx.x == 0) coef[2] = pap > 0.0 ? coef[0] / pap : 0.0;
+}
+
+// The new direction p = z + beta p_old, formed per read, its product Ap with
+// the finest operator (FP32 fractions; stencil()), and each workgroup's share
+// of p.Ap in part.  pn and po are different arrays (neighbours read p_old).
+template <bool Z>
+__global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ pn, float* __restrict__ ap,
+                                                            const float* __restrict__ z,
+                                                            const float* __restrict__ po, LevelArgs a,
+                                                            const double* __restrict__ coef,
+                                                            double* __restrict__ part) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.y * 4 + threadIdx.y;
+    const int zc = blockIdx.z;
+    const float beta = (float)coef[1];
+    double s = 0.0;
+    if (y < a.ny && x < a.nx) {
+        const int64_t i = y + (int64_t)x * a.ny + (int64_t)zc * a.nx * a.ny;
+        auto in = [&](int64_t j, int, int, int) { return z[j] + beta * po[j]; };
+        float diag, off;
+        stencil<Z, float>(a, in, x, y, zc, i, diag, off);
+        const float v = in(i, 0, 0, 0);
+        const float av = diag > 1e-12f ? diag * v + off : 0.f;
+        pn[i] = v;
+        ap[i] = av;
+        s = (double)v * (double)av;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double l[4];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    if ((t & 63) == 0) l[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        part[b] = l[0] + l[1] + l[2] + l[3];
+    }
+}
+
+// hi + lo <- hi + lo + alpha p, the pair rounded from the FP64 sum.
+__global__ void pair_axpy_kernel(float* __restrict__ hi, float* __restrict__ lo,
+                                 const float* __restrict__ p, const double* __restrict__ coef,
+                                 int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = (double)hi[i] + (double)lo[i] + coef[2] * (double)p[i];
+    const float h = (float)v;
+    hi[i] = h;
+    lo[i] = (float)(v - (double)h);
+}
+
 // ---- the coarse levels of a V-cycle in one workgroup -----------------------
 // Below a few thousand cells a level's launches are latency-bound (a few
 // microseconds each, six per level per cycle).  This kernel runs the V-cycle
@@ -794,6 +894,8 @@ struct whhd_mgfrac3d_s {
     bool fp16_fine = false;          // V-cycles read the finest level's FP16 fractions
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
+    float* pcg_p[2] = {nullptr, nullptr};  // conjugate directions (ping-pong), finest level
+    double* coef = nullptr;                // PCG scalars: r.z, beta, alpha
     std::vector<void*> owned;
 };
 
@@ -1121,7 +1223,11 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         h->cbuf = (double*)alloc((size_t)cells(h->lv.back().d) * 2 * sizeof(double));
         h->rpart = (double*)alloc((size_t)(2 * h->rblocks + 2) * sizeof(double));
         h->fac64 = (double*)alloc((size_t)2 * (nx + ny + nz) * sizeof(double));
-        if (!h->r || !h->part || !h->cbuf || !h->rpart || !h->fac64)
+        h->pcg_p[0] = (float*)alloc((size_t)cells(h->dim) * 4);
+        h->pcg_p[1] = (float*)alloc((size_t)cells(h->dim) * 4);
+        h->coef = (double*)alloc(4 * sizeof(double));
+        if (!h->r || !h->part || !h->cbuf || !h->rpart || !h->fac64 || !h->pcg_p[0] || !h->pcg_p[1] ||
+            !h->coef)
             st = fail(WHHD_OOM, "mgfrac3d_create: allocation failed");
     }
     if (st == WHHD_OK) {
@@ -1225,6 +1331,31 @@ int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const flo
 // Each cycle forms r = f - A (dst + lo) in FP64 (into the finest level's rhs)
 // with its FP64 norm, stops once ||r|| <= tol ||f||, else runs one V-cycle on
 // A e = r from e = 0 and adds e to the pair.
+// r = f - A (hi + lo) in FP64 into the finest level's rhs (FP32), and
+// ||r|| / ||f|| (FP64 sums) read back: synchronizes the stream.
+int residual64(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float* f, hipStream_t s,
+               double* rel) {
+    const Level& L0 = h->lv[0];
+    const LevelArgs a = args_of(h, L0);
+    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    const int nx = L0.d.x, ny = L0.d.y;
+    const Factors64 k{h->fac64, h->fac64 + ny, h->fac64 + 2 * ny, h->fac64 + 2 * ny + nx,
+                      h->fac64 + 2 * (ny + nx), h->fac64 + 2 * (ny + nx) + L0.d.z};
+    if (h->bc == 'z')
+        hipLaunchKernelGGL(residual64_kernel<true>, g, dim3(64, 4), 0, s, L0.f, hi, lo, f, a, k, h->rpart);
+    else
+        hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, hi, lo, f, a, k, h->rpart);
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
+                       h->rpart, (int)h->rblocks);
+    hipError_t e = launch_status("mg_residual64_kernel");
+    if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
+    e = hipMemcpyAsync(h->host, h->rpart + 2 * h->rblocks, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "mgfrac3d: residual read-back");
+    *rel = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
+    return WHHD_OK;
+}
+
 int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* pre_v,
                        const float* f, int max_cycles, double tol, int nu1, int nu2, float w,
                        int* cycles, double* rel, hipStream_t s) {
@@ -1233,7 +1364,6 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: dst, dst_lo and f must be distinct");
     if (max_cycles < 0 || nu1 < 0 || nu2 < 0 || !(tol >= 0))
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_tol: bad iteration limits");
-    Level& L0 = h->lv[0];
     const int64_t n = cells(h->dim);
     // the V-cycles read the finest level's FP16 fractions while this solve runs
     struct Fp16Fine {
@@ -1244,30 +1374,11 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     if (pre_v && pre_v != dst) TRY(copy(dst, pre_v, n, s));
     else if (!pre_v) TRY(fill(dst, 0.f, n, s));
     TRY(fill(lo, 0.f, n, s));
-    const LevelArgs a = args_of(h, L0);
-    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
-    const int nx = L0.d.x, ny = L0.d.y;
-    const Factors64 k{h->fac64, h->fac64 + ny, h->fac64 + 2 * ny, h->fac64 + 2 * ny + nx,
-                      h->fac64 + 2 * (ny + nx), h->fac64 + 2 * (ny + nx) + L0.d.z};
     double res = 0.0;
     int c = 0, last_k = 1;
     for (;; ++c) {
-        if (h->bc == 'z')
-            hipLaunchKernelGGL(residual64_kernel<true>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
-                               k, h->rpart);
-        else
-            hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, dst, lo, f, a,
-                               k, h->rpart);
-        hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
-                           h->rpart, (int)h->rblocks);
-        hipError_t e = launch_status("mg_residual64_kernel");
-        if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
-        e = hipMemcpyAsync(h->host, h->rpart + 2 * h->rblocks, 2 * sizeof(double),
-                           hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "mgfrac3d_solve_tol: read-back");
         const double prev = res;
-        res = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
+        TRY(residual64(h, dst, lo, f, s, &res));
         if (res <= tol || c >= max_cycles) break;
         // Up to kMaxK V-cycles per FP64 residual (all but the first on the
         // FP32 residual of the correction equation, exact enough while e is a
@@ -1286,6 +1397,64 @@ int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         last_k = k;
     }
     if (cycles) *cycles = c;
+    if (rel) *rel = res;
+    return res <= tol ? WHHD_OK : WHHD_NONCONVERGED;
+}
+
+// Conjugate gradients on A x = f preconditioned by one V-cycle from zero
+// (symmetric for nu1 = nu2: weighted-Jacobi sweeps around the coarse
+// correction, spacing-weighted averaging restriction, injection
+// prolongation, a fixed-sweep coarsest solve), x the pair hi + lo, every
+// residual formed afresh in FP64.
+int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* pre_v, const float* f,
+                       int max_iters, double tol, int nu1, int nu2, float w, int* iters, double* rel,
+                       hipStream_t s) {
+    if (!h || !dst || !lo || !f) return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: null argument");
+    if (dst == lo || lo == f || dst == f)
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: dst, dst_lo and f must be distinct");
+    if (max_iters < 0 || nu1 < 0 || nu2 < 0 || !(tol >= 0))
+        return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: bad iteration limits");
+    Level& L0 = h->lv[0];
+    const int64_t n = cells(h->dim);
+    // the preconditioning V-cycles read the finest level's FP16 fractions
+    struct Fp16Fine {
+        whhd_mgfrac3d_s* h;
+        ~Fp16Fine() { h->fp16_fine = false; }
+    } fp16_guard{h};
+    h->fp16_fine = true;
+    if (pre_v && pre_v != dst) TRY(copy(dst, pre_v, n, s));
+    else if (!pre_v) TRY(fill(dst, 0.f, n, s));
+    TRY(fill(lo, 0.f, n, s));
+    TRY(fill(h->pcg_p[0], 0.f, n, s));  // beta * p_old reads it at the first iteration
+    // the direction's product with A uses the FP32 fractions (the operator
+    // the FP64 residual measures)
+    LevelArgs a = args_of(h, L0);
+    a.axh = a.ayh = a.azh = nullptr;
+    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    const unsigned eb = (unsigned)((n + 255) / 256);
+    double res = 0.0;
+    int c = 0;
+    for (;; ++c) {
+        TRY(residual64(h, dst, lo, f, s, &res));
+        if (res <= tol || c >= max_iters) break;
+        TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
+        hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
+                           c == 0 ? 1 : 0);
+        float* pn = h->pcg_p[(c + 1) & 1];
+        const float* po = h->pcg_p[c & 1];
+        if (h->bc == 'z')
+            hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                               h->coef, h->rpart);
+        else
+            hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                               h->coef, h->rpart);
+        hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
+        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, pn, h->coef, n);
+        hipError_t e = launch_status("mg_pcg_kernels");
+        if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient kernels");
+    }
+    if (iters) *iters = c;
     if (rel) *rel = res;
     return res <= tol ? WHHD_OK : WHHD_NONCONVERGED;
 }
@@ -1345,6 +1514,14 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
                                      double* rel, void* stream) {
     return whhd::mgfrac3d_solve_tol(h, dst, dst_lo, pre_v, f, max_cycles, tol, nu1, nu2, weight,
                                     cycles, rel, (hipStream_t)stream);
+}
+
+WHHD_API int whhd_mgfrac3d_solve_pcg(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
+                                     const float* pre_v, const float* f, int max_iters,
+                                     double tol, int nu1, int nu2, float weight, int* iters,
+                                     double* rel, void* stream) {
+    return whhd::mgfrac3d_solve_pcg(h, dst, dst_lo, pre_v, f, max_iters, tol, nu1, nu2, weight,
+                                    iters, rel, (hipStream_t)stream);
 }
 
 WHHD_API int whhd_mgfrac3d_residual(whhd_mgfrac3d_t h, const float* v, const float* f,

@@ -87,9 +87,11 @@ def train(net, n, samples, epochs, lr, log=sys.stderr, bf16_cache=True, flips=Tr
 
 def evaluate(net, n, seeds, tol=1e-6, max_cycles=400):
     """Per held-out seed: V-cycles and GPU milliseconds to tol from zero and from
-    the prediction (prediction included), plus the prediction's own residual."""
+    the prediction (prediction included), plus the prediction's own residual;
+    by the V-cycle refinement and (keys pcg_*) by conjugate gradients with a
+    V-cycle as the preconditioner."""
     rows = []
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
     for s in seeds:
         P = WL.solid_problem(n, s)
         mg = api.CuMultigridFrac3D(P["ax"], P["ay"], P["az"], n, P["dx"], "n")
@@ -105,6 +107,11 @@ def evaluate(net, n, seeds, tol=1e-6, max_cycles=400):
         net.predict(p0, f, P["ls"], P["length"])
         _, _, c_learn, r_learn = WL.mg_solve_from(mg, f, p0, tol, max_cycles)
         ev[2].record()
+        _, _, g_plain, _ = WL.mg_solve_from(mg, f, None, tol, max_cycles, "pcg")
+        ev[3].record()
+        net.predict(p0, f, P["ls"], P["length"])
+        _, _, g_learn, _ = WL.mg_solve_from(mg, f, p0, tol, max_cycles, "pcg")
+        ev[4].record()
         torch.cuda.synchronize()
         r0 = mg.GetResdualError(p0, f)
         ps = hi.double() + lo.double()
@@ -114,7 +121,9 @@ def evaluate(net, n, seeds, tol=1e-6, max_cycles=400):
                      "plain_res": r_plain, "learned_cycles": c_learn,
                      "learned_ms": ev[1].elapsed_time(ev[2]), "learned_res": r_learn,
                      "prediction_res": r0,
-                     "prediction_rel_err": float(e.norm() / ps.norm())})
+                     "prediction_rel_err": float(e.norm() / ps.norm()),
+                     "pcg_plain_iters": g_plain, "pcg_plain_ms": ev[2].elapsed_time(ev[3]),
+                     "pcg_learned_iters": g_learn, "pcg_learned_ms": ev[3].elapsed_time(ev[4])})
     return rows
 
 

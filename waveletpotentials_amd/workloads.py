@@ -29,9 +29,9 @@ def face_fractions(dims, dx, centers, radii):
 
 def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-6, max_cycles=80):
     """In place: u <- a (u - grad p) with p from the fractional multigrid solve of
-    -div(a grad p) = -div(a u), V-cycles (2 pre / 2 post weighted-Jacobi sweeps,
-    w = 0.8) until the FP64 relative residual of the two-float iterate is
-    <= tol.  Returns (cycles, residual)."""
+    -div(a grad p) = -div(a u), conjugate gradients preconditioned by a V-cycle
+    (2 pre / 2 post weighted-Jacobi sweeps, w = 0.8) until the FP64 relative
+    residual of the two-float iterate is <= tol.  Returns (V-cycles, residual)."""
     if len(radii) == 0:
         return 0, 0.0
     (ax, ay, az), _ = face_fractions(dims, dx, centers, radii)
@@ -40,7 +40,7 @@ def solid_velocity(xv, yv, zv, dims, dx, centers, radii, tol=1e-6, max_cycles=80
     api.CuConvergence.GetInstance().GetB_3d_frac_vel(b, xv, yv, zv, ax, ay, az, dims, dx, "n")
     mg = api.CuMultigridFrac3D(ax, ay, az, dims, dx, "n")
     p, p_lo = torch.empty_like(b), torch.empty_like(b)
-    cycles, res, _ = mg.SolveToTolerance(p, p_lo, None, b, tol, max_cycles)
+    cycles, res, _ = mg.SolvePCG(p, p_lo, None, b, tol, max_cycles)
     # u <- a (u - grad p_hi - grad p_lo) in one pass
     api.CuGradient.GetInstance().SubtractGradient3D_Frac_P2(xv, yv, zv, p, p_lo, ax, ay, az, dims,
                                                            dx, "n")
@@ -118,9 +118,15 @@ def solid_problem(n, seed, kmax=12, count=8, sweep_iters=None):
             "f": f, "xv": xv, "yv": yv, "zv": zv, "centers": c, "radii": r, "sweeps": iters}
 
 
-def mg_solve_from(mg, f, pre_v=None, tol=1e-6, max_cycles=200):
-    """Solve to `tol` from pre_v (None: zero) with the mixed-precision
-    refinement; returns (p_hi, p_lo, cycles, residual)."""
+def mg_solve_from(mg, f, pre_v=None, tol=1e-6, max_cycles=200, method="vcycles"):
+    """Solve to `tol` from pre_v (None: zero) by the mixed-precision V-cycle
+    refinement (method "vcycles") or by conjugate gradients with a V-cycle as
+    the preconditioner ("pcg"); returns (p_hi, p_lo, V-cycles, residual)."""
     hi, lo = torch.empty_like(f), torch.empty_like(f)
-    cycles, res, _ = mg.SolveToTolerance(hi, lo, pre_v, f, tol, max_cycles)
+    if method == "pcg":
+        cycles, res, _ = mg.SolvePCG(hi, lo, pre_v, f, tol, max_cycles)
+    elif method == "vcycles":
+        cycles, res, _ = mg.SolveToTolerance(hi, lo, pre_v, f, tol, max_cycles)
+    else:
+        raise ValueError(f"unknown method {method!r}")
     return hi, lo, cycles, res
