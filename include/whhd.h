@@ -287,7 +287,11 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
  * symmetric V-cycle, nu1 = nu2, is what conjugate gradients assume).
  * pre_v (NULL: zero; may equal dst) starts dst.  Writes the iterations (one
  * V-cycle each) and the last relative residual; returns WHHD_NONCONVERGED if
- * tol was not reached.  Synchronizes `stream` once per iteration. */
+ * tol was not reached.  Waits once per iteration for the residual norm (an
+ * event after its 16-byte copy) while the iteration's V-cycle and direction
+ * are already enqueued; on return the stream may still run that work of the
+ * last iteration, which writes only the handle's own buffers (dst, dst_lo
+ * and f are no longer accessed). */
 WHHD_API int whhd_mgfrac3d_solve_pcg(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
                                      const float* pre_v, const float* f, int max_iters,
                                      double tol, int nu1, int nu2, float weight, int* iters,

@@ -896,6 +896,7 @@ struct whhd_mgfrac3d_s {
     int64_t rblocks = 0;
     float* pcg_p[2] = {nullptr, nullptr};  // conjugate directions (ping-pong), finest level
     double* coef = nullptr;                // PCG scalars: r.z, beta, alpha
+    hipEvent_t norm_ready = nullptr;       // the residual norms' copy to `host` is done
     std::vector<void*> owned;
 };
 
@@ -1095,6 +1096,7 @@ void release(whhd_mgfrac3d_s* h) {
     if (!h) return;
     for (void* p : h->owned) (void)hipFree(p);
     if (h->host) (void)hipHostFree(h->host);
+    if (h->norm_ready) (void)hipEventDestroy(h->norm_ready);
     delete h;
 }
 
@@ -1300,6 +1302,10 @@ int mgfrac3d_create(const float* ax, const float* ay, const float* az, int nx, i
         h->host = nullptr;
         st = fail(WHHD_OOM, "mgfrac3d_create: pinned allocation failed");
     }
+    if (st == WHHD_OK && hipEventCreateWithFlags(&h->norm_ready, hipEventDisableTiming) != hipSuccess) {
+        h->norm_ready = nullptr;
+        st = fail(WHHD_HIP_ERROR, "mgfrac3d_create: event");
+    }
     if (st == WHHD_OK) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = hip_fail(e, "mgfrac3d_create: setup");
@@ -1331,10 +1337,10 @@ int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const flo
 // Each cycle forms r = f - A (dst + lo) in FP64 (into the finest level's rhs)
 // with its FP64 norm, stops once ||r|| <= tol ||f||, else runs one V-cycle on
 // A e = r from e = 0 and adds e to the pair.
-// r = f - A (hi + lo) in FP64 into the finest level's rhs (FP32), and
-// ||r|| / ||f|| (FP64 sums) read back: synchronizes the stream.
-int residual64(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float* f, hipStream_t s,
-               double* rel) {
+// r = f - A (hi + lo) in FP64 into the finest level's rhs (FP32), its FP64
+// sums copied to `host` and norm_ready recorded after the copy.
+int residual64_enqueue(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float* f,
+                       hipStream_t s) {
     const Level& L0 = h->lv[0];
     const LevelArgs a = args_of(h, L0);
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
@@ -1350,10 +1356,24 @@ int residual64(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float
     hipError_t e = launch_status("mg_residual64_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
     e = hipMemcpyAsync(h->host, h->rpart + 2 * h->rblocks, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipEventRecord(h->norm_ready, s);
+    if (e != hipSuccess) return hip_fail(e, "mgfrac3d: residual read-back");
+    return WHHD_OK;
+}
+
+// ||r|| / ||f|| of the last residual64_enqueue, once its copy is done.
+int residual64_wait(whhd_mgfrac3d_s* h, double* rel) {
+    const hipError_t e = hipEventSynchronize(h->norm_ready);
     if (e != hipSuccess) return hip_fail(e, "mgfrac3d: residual read-back");
     *rel = h->host[1] > 0 ? std::sqrt(h->host[0]) / std::sqrt(h->host[1]) : 0.0;
     return WHHD_OK;
+}
+
+// Both: waits for the norms.
+int residual64(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float* f, hipStream_t s,
+               double* rel) {
+    TRY(residual64_enqueue(h, hi, lo, f, s));
+    return residual64_wait(h, rel);
 }
 
 int mgfrac3d_solve_tol(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* pre_v,
@@ -1432,27 +1452,48 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     a.axh = a.ayh = a.azh = nullptr;
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
     const unsigned eb = (unsigned)((n + 255) / 256);
-    double res = 0.0;
+    double res = 0.0, prev = 0.0;
     int c = 0;
     for (;; ++c) {
-        TRY(residual64(h, dst, lo, f, s, &res));
+        // The iteration's work up to the direction does not touch dst or
+        // dst_lo, so while the observed rate says this residual is still well
+        // above tol it is enqueued before the residual norm arrives and only
+        // the pair update waits for the decision; near tol the norm is read
+        // first (a converged solve would leave a whole V-cycle's work to run
+        // for nothing).
+        TRY(residual64_enqueue(h, dst, lo, f, s));
+        const bool ahead = c < 2 || !(res < prev) || res * (res / prev) > 4.0 * tol;
+        if (!ahead) {
+            prev = res;
+            TRY(residual64_wait(h, &res));
+        }
+        const bool more = c < max_iters && (ahead || res > tol);
+        if (more) {
+            TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
+            hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
+            hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
+                               c == 0 ? 1 : 0);
+            float* pn = h->pcg_p[(c + 1) & 1];
+            const float* po = h->pcg_p[c & 1];
+            if (h->bc == 'z')
+                hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                                   h->coef, h->rpart);
+            else
+                hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                                   h->coef, h->rpart);
+            hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
+            const hipError_t e = launch_status("mg_pcg_kernels");
+            if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient kernels");
+        }
+        if (ahead) {
+            prev = res;
+            TRY(residual64_wait(h, &res));
+        }
         if (res <= tol || c >= max_iters) break;
-        TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
-        hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
-        hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
-                           c == 0 ? 1 : 0);
-        float* pn = h->pcg_p[(c + 1) & 1];
-        const float* po = h->pcg_p[c & 1];
-        if (h->bc == 'z')
-            hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
-                               h->coef, h->rpart);
-        else
-            hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
-                               h->coef, h->rpart);
-        hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
-        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, pn, h->coef, n);
-        hipError_t e = launch_status("mg_pcg_kernels");
-        if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient kernels");
+        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, h->pcg_p[(c + 1) & 1],
+                           h->coef, n);
+        const hipError_t e = launch_status("mg_pair_axpy_kernel");
+        if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient pair update");
     }
     if (iters) *iters = c;
     if (rel) *rel = res;
