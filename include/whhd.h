@@ -291,7 +291,9 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
  * event after its 16-byte copy) while the iteration's V-cycle and direction
  * are already enqueued; on return the stream may still run that work of the
  * last iteration, which writes only the handle's own buffers (dst, dst_lo
- * and f are no longer accessed). */
+ * and f are no longer accessed).  tol < 0: exactly max_iters iterations and
+ * nothing read back or waited for (iters = max_iters, rel = -1), so the call
+ * can be captured in a HIP graph. */
 WHHD_API int whhd_mgfrac3d_solve_pcg(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
                                      const float* pre_v, const float* f, int max_iters,
                                      double tol, int nu1, int nu2, float weight, int* iters,

@@ -195,3 +195,29 @@ def test_pcg_rejects_bad_arguments(whhd, orc):
         mg.SolvePCG(hi, hi, None, df, 1e-6, 10)   # dst and dst_lo alias
     with pytest.raises(Exception):
         mg.SolvePCG(hi, torch.empty_like(df), None, df, 1e-6, -1)
+
+
+def test_pcg_fixed_count_captured_in_a_graph(whhd, orc):
+    """tol < 0: exactly max_iters conjugate-gradient iterations with nothing
+    read back.  Captured in a graph, every replay (eager work and
+    synchronizations between the replays) equals the eager call bit for bit,
+    and 12 iterations bring the 64^3 sphere problem to the FP32 floor."""
+    n = (64, 64, 64)
+    fr, f, h = problem(orc, n, 11, True)
+    mg = whhd.CuMultigridFrac3D(*(cuda(a) for a in fr), n, h, "n")
+    df = cuda(f)
+    hi, lo = torch.empty_like(df), torch.empty_like(df)
+    assert mg.SolvePCG(hi, lo, None, df, -1.0, 12) == (12, -1.0, True)
+    torch.cuda.synchronize()
+    assert mg.GetResdualError((hi.double() + lo.double()).float(), df) < 1e-5
+    ghi, glo = torch.empty_like(df), torch.empty_like(df)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        mg.SolvePCG(ghi, glo, None, df, -1.0, 12)
+    for k in range(3):
+        ghi.fill_(float(k))
+        glo.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ghi, hi) and torch.equal(glo, lo), k

@@ -1340,7 +1340,7 @@ int mgfrac3d_solve(whhd_mgfrac3d_s* h, float* dst, const float* pre_v, const flo
 // r = f - A (hi + lo) in FP64 into the finest level's rhs (FP32), its FP64
 // sums copied to `host` and norm_ready recorded after the copy.
 int residual64_enqueue(whhd_mgfrac3d_s* h, const float* hi, const float* lo, const float* f,
-                       hipStream_t s) {
+                       hipStream_t s, bool read_back = true) {
     const Level& L0 = h->lv[0];
     const LevelArgs a = args_of(h, L0);
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
@@ -1351,10 +1351,12 @@ int residual64_enqueue(whhd_mgfrac3d_s* h, const float* hi, const float* lo, con
         hipLaunchKernelGGL(residual64_kernel<true>, g, dim3(64, 4), 0, s, L0.f, hi, lo, f, a, k, h->rpart);
     else
         hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, hi, lo, f, a, k, h->rpart);
-    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
-                       h->rpart, (int)h->rblocks);
+    if (read_back)
+        hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
+                           h->rpart, (int)h->rblocks);
     hipError_t e = launch_status("mg_residual64_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
+    if (!read_back) return WHHD_OK;
     e = hipMemcpyAsync(h->host, h->rpart + 2 * h->rblocks, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(h->norm_ready, s);
     if (e != hipSuccess) return hip_fail(e, "mgfrac3d: residual read-back");
@@ -1432,7 +1434,7 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     if (!h || !dst || !lo || !f) return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: null argument");
     if (dst == lo || lo == f || dst == f)
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: dst, dst_lo and f must be distinct");
-    if (max_iters < 0 || nu1 < 0 || nu2 < 0 || !(tol >= 0))
+    if (max_iters < 0 || nu1 < 0 || nu2 < 0 || std::isnan(tol))
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: bad iteration limits");
     Level& L0 = h->lv[0];
     const int64_t n = cells(h->dim);
@@ -1452,6 +1454,42 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     a.axh = a.ayh = a.azh = nullptr;
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
     const unsigned eb = (unsigned)((n + 255) / 256);
+    // z = M r, the new direction and alpha for iteration c (nothing read back)
+    auto direction = [&](int c) -> int {
+        TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
+        hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
+                           c == 0 ? 1 : 0);
+        float* pn = h->pcg_p[(c + 1) & 1];
+        const float* po = h->pcg_p[c & 1];
+        if (h->bc == 'z')
+            hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                               h->coef, h->rpart);
+        else
+            hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+                               h->coef, h->rpart);
+        hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
+        const hipError_t e = launch_status("mg_pcg_kernels");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient kernels");
+    };
+    auto update = [&](int c) -> int {
+        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, h->pcg_p[(c + 1) & 1],
+                           h->coef, n);
+        const hipError_t e = launch_status("mg_pair_axpy_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient pair update");
+    };
+    if (tol < 0) {
+        // exactly max_iters iterations and nothing read back, so that the
+        // whole solve can be captured in a graph
+        for (int c = 0; c < max_iters; ++c) {
+            TRY(residual64_enqueue(h, dst, lo, f, s, false));
+            TRY(direction(c));
+            TRY(update(c));
+        }
+        if (iters) *iters = max_iters;
+        if (rel) *rel = -1.0;
+        return WHHD_OK;
+    }
     double res = 0.0, prev = 0.0;
     int c = 0;
     for (;; ++c) {
@@ -1467,33 +1505,13 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
             prev = res;
             TRY(residual64_wait(h, &res));
         }
-        const bool more = c < max_iters && (ahead || res > tol);
-        if (more) {
-            TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
-            hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
-            hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
-                               c == 0 ? 1 : 0);
-            float* pn = h->pcg_p[(c + 1) & 1];
-            const float* po = h->pcg_p[c & 1];
-            if (h->bc == 'z')
-                hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
-                                   h->coef, h->rpart);
-            else
-                hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
-                                   h->coef, h->rpart);
-            hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
-            const hipError_t e = launch_status("mg_pcg_kernels");
-            if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient kernels");
-        }
+        if (c < max_iters && (ahead || res > tol)) TRY(direction(c));
         if (ahead) {
             prev = res;
             TRY(residual64_wait(h, &res));
         }
         if (res <= tol || c >= max_iters) break;
-        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, h->pcg_p[(c + 1) & 1],
-                           h->coef, n);
-        const hipError_t e = launch_status("mg_pair_axpy_kernel");
-        if (e != hipSuccess) return hip_fail(e, "mg conjugate-gradient pair update");
+        TRY(update(c));
     }
     if (iters) *iters = c;
     if (rel) *rel = res;
