@@ -183,9 +183,15 @@ struct StridedIO {
     WHHD_DEV float load(int i) const {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff_ld, i * step_bytes, 0));
     }
+    // The stores' scalar offsets come from step_st, an opaque copy of
+    // step_bytes in the forward: otherwise the compiler keeps every load's
+    // offset i * step for the store of sample i much later, and the forward
+    // spills ~70 of them to VGPR lanes (a write and a read VALU instruction
+    // each) instead of multiplying again.
+    int step_st;
     WHHD_DEV void store(int i, float v) const {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rd, voff_st,
-                                              i * step_bytes, 0);
+                                              i * step_st, 0);
     }
     // samples [START, START+CNT) of the line; each store is already one row
     template <int START, int CNT>
@@ -753,7 +759,9 @@ lp_kernel(float* __restrict__ dst, const float* __restrict__ src, LpGeom g) {
         else inverse_line_lp<W, LEV, EXT>(io);
     } else {
         const uint32_t voff = (uint32_t)((inner * g.inner_stride + outer * g.outer_stride) * 4);
-        StridedIO io{rs, rd, voff, valid ? voff : kOutOfRange, (int)(g.step * 4), n};
+        int step_st = (int)(g.step * 4);
+        if constexpr (FWD) asm volatile("s_mov_b32 %0, %0" : "+s"(step_st));
+        StridedIO io{rs, rd, voff, valid ? voff : kOutOfRange, (int)(g.step * 4), n, step_st};
         if (FWD) forward_line_lp<W, LEV, EXT>(io);
         else inverse_line_lp<W, LEV, EXT>(io);
     }
