@@ -12,9 +12,11 @@ every launch on the stream the library launches on (the library's stage
 trace), giving each launch's duration in context.  `cpu_baseline` times the
 C oracle (a port of the reference algorithm) built in FP32 on this host, with
 the FP64 oracle's figure beside it.  The line also carries `learned_init`
-(the learned initial guess against the plain multigrid on held-out 192^3
-seeds) and `secondary` (the batch, solids and sweep workloads' figures from
-a few steps each; `--workload` runs any of them alone).
+(the learned initial guess against the multigrid from zero on held-out 192^3
+seeds, with the V-cycle refinement as the solver and, under `pcg`, with
+conjugate gradients) and `secondary` (the batch, solids and sweep workloads'
+figures from a few steps each; the solids step solves by conjugate gradients
+with a V-cycle preconditioner; `--workload` runs any of them alone).
 """
 import argparse
 import json
@@ -46,7 +48,8 @@ def parse():
                    default="recover",
                    help="recover: one N^3 grid (the headline); batch: 64 grids of 32^3..128^3 "
                         "with spheres, one HIP graph per batch; solids: 192^3 with 8 spheres, "
-                        "fractional multigrid to 1e-6 and the velocity correction; learned: "
+                        "fractional multigrid (conjugate gradients, V-cycle preconditioner) to "
+                        "1e-6 and the velocity correction; learned: "
                         "the same solve from the U-Net's prediction against from zero; sweep: "
                         "the sweep + DST comparator on the same N^3 input as recover")
     return p.parse_args()
