@@ -279,21 +279,21 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
  * V-cycle (no reference counterpart; the same V-cycle as Solve, nu1 / nu2
  * sweeps, on the residual from zero, reading the finest level's fractions
  * rounded to FP16).  The iterate is the pair dst + dst_lo as in
- * whhd_mgfrac3d_solve_tol; each iteration forms r = f - A (dst + dst_lo) in
- * FP64, stops when ||r|| <= tol ||f|| or after max_iters iterations, applies
- * the V-cycle z = M r, and updates the direction p = z + beta p, A p (FP32
- * fractions) and the pair += alpha p with beta, alpha kept on the device.
- * Needs fewer V-cycles than the refinement for the same tolerance (a
- * symmetric V-cycle, nu1 = nu2, is what conjugate gradients assume).
- * pre_v (NULL: zero; may equal dst) starts dst.  Writes the iterations (one
- * V-cycle each) and the last relative residual; returns WHHD_NONCONVERGED if
- * tol was not reached.  Waits once per iteration for the residual norm (an
- * event after its 16-byte copy) while the iteration's V-cycle and direction
- * are already enqueued; on return the stream may still run that work of the
- * last iteration, which writes only the handle's own buffers (dst, dst_lo
- * and f are no longer accessed).  tol < 0: exactly max_iters iterations and
- * nothing read back or waited for (iters = max_iters, rel = -1), so the call
- * can be captured in a HIP graph. */
+ * whhd_mgfrac3d_solve_tol.  r = f - A (dst + dst_lo) is formed in FP64 at the
+ * start; each iteration applies the V-cycle z = M r, builds the direction
+ * p = z + beta p and A p (FP32 fractions, FP64 sums), and in one pass adds
+ * alpha p to the pair and updates r <- r - alpha A p (beta, alpha stay on the
+ * device).  Once that recurrence reaches max(1e-5, 10 tol) relative, every
+ * further iteration forms the FP64 residual of the pair afresh, and that
+ * decides convergence.  Needs fewer V-cycles than the refinement for the same
+ * tolerance (a symmetric V-cycle, nu1 = nu2, is what conjugate gradients
+ * assume).  pre_v (NULL: zero; may equal dst) starts dst.  Writes the
+ * iterations (one V-cycle each) and the last FP64 relative residual; returns
+ * WHHD_NONCONVERGED if tol was not reached.  Waits once per iteration for a
+ * read-back of 8 or 16 bytes.  tol < 0: exactly max_iters iterations (the
+ * FP64 residual formed every iteration) and nothing read
+ * back or waited for (iters = max_iters, rel = -1), so the call can be
+ * captured in a HIP graph. */
 WHHD_API int whhd_mgfrac3d_solve_pcg(whhd_mgfrac3d_t h, float* dst, float* dst_lo,
                                      const float* pre_v, const float* f, int max_iters,
                                      double tol, int nu1, int nu2, float weight, int* iters,

@@ -618,14 +618,18 @@ __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ 
     double s = 0.0;
     if (y < a.ny && x < a.nx) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)zc * a.nx * a.ny;
+        // p in FP32 (as stored), A p summed in FP64: the residual recurrence
+        // r - alpha A p stays close to the true residual only if A p is
+        // accurate, and the stencil's terms cancel for smooth p
         auto in = [&](int64_t j, int, int, int) { return z[j] + beta * po[j]; };
-        float diag, off;
-        stencil<Z, float>(a, in, x, y, zc, i, diag, off);
+        auto in64 = [&](int64_t j, int dy, int dx, int dz) { return (double)in(j, dy, dx, dz); };
+        double diag, off;
+        stencil<Z, double>(a, in64, x, y, zc, i, diag, off);
         const float v = in(i, 0, 0, 0);
-        const float av = diag > 1e-12f ? diag * v + off : 0.f;
+        const double av = diag > 1e-12 ? diag * (double)v + off : 0.0;
         pn[i] = v;
-        ap[i] = av;
-        s = (double)v * (double)av;
+        ap[i] = (float)av;
+        s = (double)v * av;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     __shared__ double l[4];
@@ -638,17 +642,39 @@ __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ 
     }
 }
 
-// hi + lo <- hi + lo + alpha p, the pair rounded from the FP64 sum.
-__global__ void pair_axpy_kernel(float* __restrict__ hi, float* __restrict__ lo,
-                                 const float* __restrict__ p, const double* __restrict__ coef,
-                                 int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double v = (double)hi[i] + (double)lo[i] + coef[2] * (double)p[i];
-    const float h = (float)v;
-    hi[i] = h;
-    lo[i] = (float)(v - (double)h);
+// hi + lo <- hi + lo + alpha p (the pair rounded from the FP64 sum) and the
+// recurrence residual r <- r - alpha Ap, with each block's share of r.r.
+__global__ void pcg_update_kernel(float* __restrict__ hi, float* __restrict__ lo, float* __restrict__ r,
+                                  const float* __restrict__ p, const float* __restrict__ ap,
+                                  const double* __restrict__ coef, double* __restrict__ part, int64_t n) {
+    const double alpha = coef[2];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)hi[i] + (double)lo[i] + alpha * (double)p[i];
+        const float h = (float)v;
+        hi[i] = h;
+        lo[i] = (float)(v - (double)h);
+        const float ri = (float)((double)r[i] - alpha * (double)ap[i]);
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double l[4];
+    if ((threadIdx.x & 63) == 0) l[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = l[0] + l[1] + l[2] + l[3];
 }
+
+// out[0] = the sum of nparts partials (one block).
+__global__ void __launch_bounds__(1024) sum_final_kernel(double* __restrict__ out,
+                                                         const double* __restrict__ part, int nparts) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    const double t = block_sum_1024(s);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
 
 // ---- the coarse levels of a V-cycle in one workgroup -----------------------
 // Below a few thousand cells a level's launches are latency-bound (a few
@@ -1453,13 +1479,13 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     LevelArgs a = args_of(h, L0);
     a.axh = a.ayh = a.azh = nullptr;
     const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
-    const unsigned eb = (unsigned)((n + 255) / 256);
-    // z = M r, the new direction and alpha for iteration c (nothing read back)
-    auto direction = [&](int c) -> int {
+    // z = M r, the new direction and alpha for iteration c (nothing read
+    // back; first: beta = 0, p = z)
+    auto direction = [&](int c, bool first) -> int {
         TRY(vcycle(h, 0, nu1, nu2, w, s, true));  // z = M r in L0.v
         hipLaunchKernelGGL(dot_partial_kernel, dim3(kNormBlocks), dim3(256), 0, s, h->part, L0.f, L0.v, n);
         hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->part, kNormBlocks,
-                           c == 0 ? 1 : 0);
+                           first ? 1 : 0);
         float* pn = h->pcg_p[(c + 1) & 1];
         const float* po = h->pcg_p[c & 1];
         if (h->bc == 'z')
@@ -1472,46 +1498,64 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         const hipError_t e = launch_status("mg_pcg_kernels");
         return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient kernels");
     };
+    // x += alpha p, r -= alpha Ap (the recurrence residual in L0.f), partial r.r
     auto update = [&](int c) -> int {
-        hipLaunchKernelGGL(pair_axpy_kernel, dim3(eb), dim3(256), 0, s, dst, lo, h->pcg_p[(c + 1) & 1],
-                           h->coef, n);
-        const hipError_t e = launch_status("mg_pair_axpy_kernel");
-        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient pair update");
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(kNormBlocks), dim3(256), 0, s, dst, lo, L0.f,
+                           h->pcg_p[(c + 1) & 1], h->r, h->coef, h->part, n);
+        const hipError_t e = launch_status("mg_pcg_update_kernel");
+        return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient update");
     };
     if (tol < 0) {
         // exactly max_iters iterations and nothing read back, so that the
         // whole solve can be captured in a graph
+        // (the FP64 residual every iteration: the count may run past
+        // convergence, where the recurrence no longer describes the iterate)
         for (int c = 0; c < max_iters; ++c) {
             TRY(residual64_enqueue(h, dst, lo, f, s, false));
-            TRY(direction(c));
+            TRY(direction(c, c == 0));
             TRY(update(c));
         }
         if (iters) *iters = max_iters;
         if (rel) *rel = -1.0;
         return WHHD_OK;
     }
-    double res = 0.0, prev = 0.0;
-    int c = 0;
-    for (;; ++c) {
-        // The iteration's work up to the direction does not touch dst or
-        // dst_lo, so while the observed rate says this residual is still well
-        // above tol it is enqueued before the residual norm arrives and only
-        // the pair update waits for the decision; near tol the norm is read
-        // first (a converged solve would leave a whole V-cycle's work to run
-        // for nothing).
-        TRY(residual64_enqueue(h, dst, lo, f, s));
-        const bool ahead = c < 2 || !(res < prev) || res * (res / prev) > 4.0 * tol;
-        if (!ahead) {
-            prev = res;
-            TRY(residual64_wait(h, &res));
-        }
-        if (c < max_iters && (ahead || res > tol)) TRY(direction(c));
-        if (ahead) {
-            prev = res;
-            TRY(residual64_wait(h, &res));
-        }
-        if (res <= tol || c >= max_iters) break;
+    // While the residual is far above tol it follows the recurrence
+    // r <- r - alpha Ap, updated with the pair in one pass (its drift from the
+    // true residual, ~1e-7 of ||f|| from FP32 rounding, is then negligible);
+    // from max(1e-5, 10 tol) on, every iteration forms the FP64 residual of
+    // the pair afresh (below its drift the recurrence stops describing the
+    // iterate and the iteration would wander), restarting the directions once
+    // at the switch.  The FP64 residual always decides convergence.
+    const double exact_below = std::max(1e-5, 10.0 * tol);
+    double res = 0.0;
+    TRY(residual64(h, dst, lo, f, s, &res));
+    const double fnorm2 = h->host[1];
+    bool exact = res <= exact_below;
+    int c = 0, restart = 0;  // restart: an iteration whose direction starts afresh (beta = 0)
+    while (res > tol && c < max_iters) {
+        TRY(direction(c, c == restart));
         TRY(update(c));
+        ++c;
+        if (exact) {
+            TRY(residual64(h, dst, lo, f, s, &res));
+            continue;
+        }
+        hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(1024), 0, s, h->part + kNormBlocks, h->part,
+                           kNormBlocks);
+        hipError_t e = launch_status("mg_sum_final_kernel");
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->host, h->part + kNormBlocks, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(h->norm_ready, s);
+        if (e == hipSuccess) e = hipEventSynchronize(h->norm_ready);
+        if (e != hipSuccess) return hip_fail(e, "mgfrac3d_solve_pcg: read-back");
+        const double rec = fnorm2 > 0 ? std::sqrt(h->host[0] / fnorm2) : 0.0;
+        if (rec <= exact_below || c >= max_iters) {
+            TRY(residual64(h, dst, lo, f, s, &res));
+            exact = true;
+            restart = c;
+        } else {
+            res = rec;
+        }
     }
     if (iters) *iters = c;
     if (rel) *rel = res;
