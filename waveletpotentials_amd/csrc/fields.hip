@@ -47,13 +47,16 @@ __device__ __forceinline__ int64_t icell(int x, int y, int z, int nx, int ny) {
 // Gradient component on the face between cells (lo) and (hi) along one axis
 // with c cells: interior faces (p_hi - p_lo)/h; wall faces use the wall
 // distance e (zero wall value) or are zero when closed (e < 0).  Faces whose
-// fraction is not above eps get 0.
+// fraction is not above eps get 0.  Both loads are issued from the (clamped,
+// always valid) indices and the case is selected afterwards: a branch per
+// load would make each face wait out its own memory latency.
 __device__ __forceinline__ float face_grad(const float* __restrict__ p, float a, int k, int c,
                                           int64_t i_hi, int64_t i_lo, float h, float e) {
-    if (!(a > kFracEps)) return 0.f;
-    if (k > 0 && k < c) return (p[i_hi] - p[i_lo]) / h;
-    if (e < 0.f) return 0.f;
-    return k == 0 ? p[i_hi] / e : -p[i_lo] / e;
+    const float ph = p[i_hi], pl = p[i_lo];
+    const float inner = (ph - pl) / h;
+    const float wall = k == 0 ? ph / e : -pl / e;
+    const float g = (k > 0 && k < c) ? inner : (e < 0.f ? 0.f : wall);
+    return a > kFracEps ? g : 0.f;
 }
 
 // FUSE = false: out = grad p (Gradient3D_Frac_P); FUSE = true: u <- a * (u - grad p),
