@@ -276,9 +276,10 @@ WHHD_API int whhd_mgfrac3d_solve_tol(whhd_mgfrac3d_t h, float* dst, float* dst_l
                                      double tol, int nu1, int nu2, float weight, int* cycles,
                                      double* rel, void* stream);
 /* Solve to a relative residual by conjugate gradients preconditioned by one
- * V-cycle (no reference counterpart; the same V-cycle as Solve, nu1 / nu2
+ * V-cycle (no reference counterpart; the V-cycle of Solve, nu1 / nu2
  * sweeps, on the residual from zero, reading the finest level's fractions
- * rounded to FP16).  The iterate is the pair dst + dst_lo as in
+ * rounded to FP16 and running 10 coarsest-level sweeps instead of 50 for
+ * bc 'n').  The iterate is the pair dst + dst_lo as in
  * whhd_mgfrac3d_solve_tol.  r = f - A (dst + dst_lo) is formed in FP64 at the
  * start; each iteration applies the V-cycle z = M r, builds the direction
  * p = z + beta p and A p (FP32 fractions, FP64 sums), and in one pass adds

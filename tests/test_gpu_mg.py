@@ -116,10 +116,11 @@ def test_solve_to_tolerance_passes_the_fp32_floor(whhd, orc):
 
 
 def test_plain_solve_unchanged_after_tolerance_solve(whhd, orc):
-    """The tolerance solve's V-cycles read FP16 copies of the finest level's
-    fractions; afterwards the plain Solve (the reference's operator, FP32
+    """The tolerance solves' V-cycles read FP16 copies of the finest level's
+    fractions (and the conjugate-gradient preconditioner runs fewer coarsest
+    sweeps); afterwards the plain Solve (the reference's V-cycle, FP32
     fractions) on the same handle gives bit-identical results to a fresh
-    handle's, and the tolerance solve itself reaches its tolerance.  (A cubic
+    handle's, and both tolerance solves reach their tolerance.  (A cubic
     grid: with unequal spacings such as 1/32, 1/24, 1/16 the V-cycle itself
     diverges, in the oracle as on the GPU.)"""
     n = (32, 32, 32)
@@ -130,6 +131,8 @@ def test_plain_solve_unchanged_after_tolerance_solve(whhd, orc):
     hi, lo = torch.empty_like(df), torch.empty_like(df)
     cycles, res, ok = used.SolveToTolerance(hi, lo, None, df, 1e-6, 200)
     assert ok and res <= 1e-6 and cycles > 0
+    it, res, ok = used.SolvePCG(hi, lo, None, df, 1e-6, 100)
+    assert ok and res <= 1e-6 and 0 < it < cycles
     a, b = torch.empty_like(df), torch.empty_like(df)
     used.Solve(a, None, df, 3, 2, 2, 0.8)
     fresh.Solve(b, None, df, 3, 2, 2, 0.8)

@@ -918,6 +918,7 @@ struct whhd_mgfrac3d_s {
     size_t small_lds = 0;            // its LDS bytes
     int fuse_from = -1;              // levels from here down run in one workgroup
     bool fp16_fine = false;          // V-cycles read the finest level's FP16 fractions
+    int coarsest_n = 50;             // coarsest-level sweeps for bc 'n' (the reference's 50)
     double* rpart = nullptr;  // per-workgroup sums of the FP64 residual kernel, + 2 totals
     int64_t rblocks = 0;
     float* pcg_p[2] = {nullptr, nullptr};  // conjugate directions (ping-pong), finest level
@@ -1041,7 +1042,7 @@ int smooth(const whhd_mgfrac3d_s* h, const Level& L, int n, float w, hipStream_t
 // sweeps run here, in FP64, by one workgroup.
 int coarsest(const whhd_mgfrac3d_s* h, const Level& L, float w, hipStream_t s) {
     const LevelArgs a = args_of(h, L);
-    const int sweeps = h->bc == 'z' ? 20 : 50;
+    const int sweeps = h->bc == 'z' ? 20 : h->coarsest_n;
     if (h->bc == 'z')
         hipLaunchKernelGGL(coarsest_kernel<true>, dim3(1), dim3(256), 0, s, L.v, L.f, a, h->cbuf,
                            sweeps, (double)w);
@@ -1078,7 +1079,7 @@ int vcycle_body(const whhd_mgfrac3d_s* h, size_t l, int nu1, int nu2, float w, h
     if (h->fuse_from >= 0 && (int)l == h->fuse_from) {
         // the small levels start from a zero iterate in LDS (coarse levels are
         // entered with zero)
-        const int cs = h->bc == 'z' ? 20 : 50;
+        const int cs = h->bc == 'z' ? 20 : h->coarsest_n;
         if (h->bc == 'z')
             hipLaunchKernelGGL(vcycle_small_kernel<true>, dim3(1), dim3(kSmallThreads), h->small_lds, s,
                                h->small, nu1, nu2, w, cs);
@@ -1464,12 +1465,19 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         return fail(WHHD_INVALID_ARG, "mgfrac3d_solve_pcg: bad iteration limits");
     Level& L0 = h->lv[0];
     const int64_t n = cells(h->dim);
-    // the preconditioning V-cycles read the finest level's FP16 fractions
-    struct Fp16Fine {
+    // the preconditioning V-cycles read the finest level's FP16 fractions and
+    // run 10 coarsest sweeps instead of 50 for bc 'n' (at 192^3 the same 12
+    // iterations, 4.89 -> 4.80 ms): the preconditioner need not be the
+    // reference's V-cycle, only a fixed symmetric one
+    struct Precond {
         whhd_mgfrac3d_s* h;
-        ~Fp16Fine() { h->fp16_fine = false; }
-    } fp16_guard{h};
+        ~Precond() {
+            h->fp16_fine = false;
+            h->coarsest_n = 50;
+        }
+    } precond_guard{h};
     h->fp16_fine = true;
+    h->coarsest_n = 10;
     if (pre_v && pre_v != dst) TRY(copy(dst, pre_v, n, s));
     else if (!pre_v) TRY(fill(dst, 0.f, n, s));
     TRY(fill(lo, 0.f, n, s));
