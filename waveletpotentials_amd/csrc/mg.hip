@@ -605,6 +605,9 @@ __global__ void __launch_bounds__(1024) pcg_alpha_kernel(double* __restrict__ co
 // The new direction p = z + beta p_old, formed per read, its product Ap with
 // the finest operator (FP32 fractions; stencil()), and each workgroup's share
 // of p.Ap in part.  pn and po are different arrays (neighbours read p_old).
+// A workgroup takes kDirZ consecutive z planes of its 64 x 4 tile, so the
+// p.Ap partials are kDirZ times fewer for the one-block reduction after it.
+constexpr int kDirZ = 4;
 template <bool Z>
 __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ pn, float* __restrict__ ap,
                                                             const float* __restrict__ z,
@@ -613,10 +616,10 @@ __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ 
                                                             double* __restrict__ part) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
-    const int zc = blockIdx.z;
     const float beta = (float)coef[1];
     double s = 0.0;
-    if (y < a.ny && x < a.nx) {
+    const int z1 = min((int)(blockIdx.z + 1) * kDirZ, a.nz);
+    for (int zc = blockIdx.z * kDirZ; zc < z1 && y < a.ny && x < a.nx; ++zc) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)zc * a.nx * a.ny;
         // p in FP32 (as stored), A p summed in FP64: the residual recurrence
         // r - alpha A p stays close to the true residual only if A p is
@@ -629,7 +632,7 @@ __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ 
         const double av = diag > 1e-12 ? diag * (double)v + off : 0.0;
         pn[i] = v;
         ap[i] = (float)av;
-        s = (double)v * av;
+        s += (double)v * av;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     __shared__ double l[4];
@@ -1486,7 +1489,8 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     // the FP64 residual measures)
     LevelArgs a = args_of(h, L0);
     a.axh = a.ayh = a.azh = nullptr;
-    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    const dim3 gd((L0.d.y + 63) / 64, (L0.d.x + 3) / 4, (L0.d.z + kDirZ - 1) / kDirZ);
+    const int dparts = (int)(gd.x * gd.y * gd.z);
     // z = M r, the new direction and alpha for iteration c (nothing read
     // back; first: beta = 0, p = z)
     auto direction = [&](int c, bool first) -> int {
@@ -1497,12 +1501,12 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
         float* pn = h->pcg_p[(c + 1) & 1];
         const float* po = h->pcg_p[c & 1];
         if (h->bc == 'z')
-            hipLaunchKernelGGL(pcg_direction_kernel<true>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+            hipLaunchKernelGGL(pcg_direction_kernel<true>, gd, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
                                h->coef, h->rpart);
         else
-            hipLaunchKernelGGL(pcg_direction_kernel<false>, g, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
+            hipLaunchKernelGGL(pcg_direction_kernel<false>, gd, dim3(64, 4), 0, s, pn, h->r, L0.v, po, a,
                                h->coef, h->rpart);
-        hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, (int)h->rblocks);
+        hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(1024), 0, s, h->coef, h->rpart, dparts);
         const hipError_t e = launch_status("mg_pcg_kernels");
         return e == hipSuccess ? WHHD_OK : hip_fail(e, "mg conjugate-gradient kernels");
     };
