@@ -254,6 +254,12 @@ struct Factors64 {
 // Residual of the two-float iterate hi + lo in FP64, r = f - A (hi + lo), with
 // the operator's coefficients in FP64 (fraction times factor), stored as FP32,
 // with per-workgroup FP64 sums of r^2 and f^2 in part.
+// The FP64 residual and the conjugate-gradient direction: a workgroup takes
+// kZPlanes consecutive z planes of its 64 x 4 tile, so their per-workgroup
+// partial sums are kZPlanes times fewer for the one-block reductions after
+// them.
+constexpr int kZPlanes = 4;
+
 template <bool Z>
 __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
                                                          const float* __restrict__ hi,
@@ -262,9 +268,9 @@ __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
                                                          Factors64 k, double* __restrict__ part) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int x = blockIdx.y * 4 + threadIdx.y;
-    const int z = blockIdx.z;
     double sr = 0.0, sf = 0.0;
-    if (y < a.ny && x < a.nx) {
+    const int z1 = min((int)(blockIdx.z + 1) * kZPlanes, a.nz);
+    for (int z = blockIdx.z * kZPlanes; z < z1 && y < a.ny && x < a.nx; ++z) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)z * a.nx * a.ny;
         const int64_t sx = a.ny, sz = (int64_t)a.nx * a.ny;
         auto val = [&](int64_t j) { return (double)hi[j] + (double)lo[j]; };
@@ -298,8 +304,8 @@ __global__ void __launch_bounds__(256) residual64_kernel(float* __restrict__ r,
         const double fi = f[i];
         const double ri = diag > 1e-12 ? fi - (diag * val(i) + off) : 0.0;
         r[i] = (float)ri;
-        sr = ri * ri;
-        sf = fi * fi;
+        sr += ri * ri;
+        sf += fi * fi;
     }
     for (int o = 32; o > 0; o >>= 1) {
         sr += __shfl_xor(sr, o);
@@ -605,9 +611,6 @@ __global__ void __launch_bounds__(1024) pcg_alpha_kernel(double* __restrict__ co
 // The new direction p = z + beta p_old, formed per read, its product Ap with
 // the finest operator (FP32 fractions; stencil()), and each workgroup's share
 // of p.Ap in part.  pn and po are different arrays (neighbours read p_old).
-// A workgroup takes kDirZ consecutive z planes of its 64 x 4 tile, so the
-// p.Ap partials are kDirZ times fewer for the one-block reduction after it.
-constexpr int kDirZ = 4;
 template <bool Z>
 __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ pn, float* __restrict__ ap,
                                                             const float* __restrict__ z,
@@ -618,8 +621,8 @@ __global__ void __launch_bounds__(256) pcg_direction_kernel(float* __restrict__ 
     const int x = blockIdx.y * 4 + threadIdx.y;
     const float beta = (float)coef[1];
     double s = 0.0;
-    const int z1 = min((int)(blockIdx.z + 1) * kDirZ, a.nz);
-    for (int zc = blockIdx.z * kDirZ; zc < z1 && y < a.ny && x < a.nx; ++zc) {
+    const int z1 = min((int)(blockIdx.z + 1) * kZPlanes, a.nz);
+    for (int zc = blockIdx.z * kZPlanes; zc < z1 && y < a.ny && x < a.nx; ++zc) {
         const int64_t i = y + (int64_t)x * a.ny + (int64_t)zc * a.nx * a.ny;
         // p in FP32 (as stored), A p summed in FP64: the residual recurrence
         // r - alpha A p stays close to the true residual only if A p is
@@ -1373,7 +1376,8 @@ int residual64_enqueue(whhd_mgfrac3d_s* h, const float* hi, const float* lo, con
                        hipStream_t s, bool read_back = true) {
     const Level& L0 = h->lv[0];
     const LevelArgs a = args_of(h, L0);
-    const dim3 g = grid_of(L0.d.x, L0.d.y, L0.d.z);
+    const dim3 g((L0.d.y + 63) / 64, (L0.d.x + 3) / 4, (L0.d.z + kZPlanes - 1) / kZPlanes);
+    const int nparts = (int)(g.x * g.y * g.z);
     const int nx = L0.d.x, ny = L0.d.y;
     const Factors64 k{h->fac64, h->fac64 + ny, h->fac64 + 2 * ny, h->fac64 + 2 * ny + nx,
                       h->fac64 + 2 * (ny + nx), h->fac64 + 2 * (ny + nx) + L0.d.z};
@@ -1383,7 +1387,7 @@ int residual64_enqueue(whhd_mgfrac3d_s* h, const float* hi, const float* lo, con
         hipLaunchKernelGGL(residual64_kernel<false>, g, dim3(64, 4), 0, s, L0.f, hi, lo, f, a, k, h->rpart);
     if (read_back)
         hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(1024), 0, s, h->rpart + 2 * h->rblocks,
-                           h->rpart, (int)h->rblocks);
+                           h->rpart, nparts);
     hipError_t e = launch_status("mg_residual64_kernel");
     if (e != hipSuccess) return hip_fail(e, "mg FP64 residual kernel");
     if (!read_back) return WHHD_OK;
@@ -1489,7 +1493,7 @@ int mgfrac3d_solve_pcg(whhd_mgfrac3d_s* h, float* dst, float* lo, const float* p
     // the FP64 residual measures)
     LevelArgs a = args_of(h, L0);
     a.axh = a.ayh = a.azh = nullptr;
-    const dim3 gd((L0.d.y + 63) / 64, (L0.d.x + 3) / 4, (L0.d.z + kDirZ - 1) / kDirZ);
+    const dim3 gd((L0.d.y + 63) / 64, (L0.d.x + 3) / 4, (L0.d.z + kZPlanes - 1) / kZPlanes);
     const int dparts = (int)(gd.x * gd.y * gd.z);
     // z = M r, the new direction and alpha for iteration c (nothing read
     // back; first: beta = 0, p = z)
